@@ -1,0 +1,373 @@
+// mkf_api.hip - C entry points of libmkf_amd.so.
+//
+// The reference's blocking entry points (binMapCUDA, mkfCUDAGetBPFD[H],
+// mkfCUDAGetBPFDautoCtx, mkfAcc*) keep their names, argument meaning, return
+// conventions and library-owned buffers; the additive stream-aware forms
+// (mkf_ext.h) return error codes. All GPU work is done by the kernels in
+// mkf_kernels.hip; nothing here computes a result on the CPU.
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "mkf_internal.h"
+
+using namespace mkf;
+
+namespace {
+
+thread_local char gErr[256] = "";
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(gErr, sizeof(gErr), fmt, ap);
+  va_end(ap);
+  if (getenv("MKF_VERBOSE")) fprintf(stderr, "[mkf] %s\n", gErr);
+  return code;
+}
+
+int hipFail(hipError_t e, const char* what) {
+  return fail(MKF_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// Library-owned buffers of the blocking entry points (the reference keeps the
+// same statics: gpDevW, src/binMapCUDA.cu:142; gHD, src/mkfCUDA.cu:69). The
+// PBM buffer is regrown when a larger field arrives (the reference never
+// resizes it).
+std::mutex gLock;
+BMPackWord* gDevW = nullptr;
+size_t gDevWWords = 0;
+size_t* gHostBPFD = nullptr;
+unsigned long long* gDevBPFD = nullptr;
+
+// Event timer on one stream: the reference's CTimerCUDA (src/utilCUDA.hpp:21-60).
+struct EventTimer {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t s;
+  explicit EventTimer(hipStream_t st) : s(st) {
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) hipEventRecord(e0, s);
+  }
+  ~EventTimer() {
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+  }
+  float stopMs() {
+    float ms = 0.f;
+    if (!e0 || !e1) return ms;
+    hipEventRecord(e1, s);
+    hipEventSynchronize(e1);
+    hipEventElapsedTime(&ms, e0, e1);
+    return ms;
+  }
+};
+
+// Resolves the first valid field of pI into a row-contiguous view with the
+// dims > 1 compacted to the front (src/gridUtil.cu:30-56 validate, plus
+// src/binMapCUDA.cuh:149-176 for the pointer choice). Several enabled fields
+// (the reference's summed multi-field map) and non-unit x strides are not
+// supported.
+int resolveField(const BMFieldInfo* pI, FieldView* fv, FieldDef def[3]) {
+  if (!pI) return fail(MKF_ERR_ARG, "NULL field info");
+  int nField = 0;
+  if (pI->pFieldDevPtrTable) nField = countValidPtrByMask(pI->pFieldDevPtrTable, pI->fieldTableMask);
+  const int nDim = copyNonZeroDef(def, pI->pD, 3);
+  if (nDim <= 0 || nField <= 0) return fail(MKF_ERR_ARG, "no valid field (nDim=%d nField=%d)", nDim, nField);
+  if (nField > 1) return fail(MKF_ERR_UNSUPPORTED, "multi-field sum (%d fields) is not supported", nField);
+  if (pI->elemID != ENC_F32 && pI->elemID != ENC_F64)
+    return fail(MKF_ERR_ARG, "element encoding 0x%02X is not F32/F64", pI->elemID);
+  ConstFieldPtr fp;
+  copyValidPtrByMask(&fp, 1, pI->pFieldDevPtrTable, pI->fieldTableMask);
+  fv->p = fp.p;
+  fv->enc = pI->elemID;
+  fv->X = def[0];
+  fv->Y = def[1];
+  fv->Z = def[2];
+  if (pI->pS) {
+    // Strides follow pD's order; they are only meaningful here when no dim
+    // was dropped by the compaction above.
+    if (!pI->pD || pI->pD[0] != def[0] || pI->pD[1] != def[1] || pI->pD[2] != def[2])
+      return fail(MKF_ERR_UNSUPPORTED, "strided field with dims <= 1");
+    if (pI->pS[0] != 1) return fail(MKF_ERR_UNSUPPORTED, "x stride %ld != 1", (long)pI->pS[0]);
+    fv->sy = pI->pS[1];
+    fv->sz = pI->pS[2];
+  } else {
+    fv->sy = def[0];
+    fv->sz = (long long)def[0] * def[1];
+  }
+  return MKF_OK;
+}
+
+Tiling toTiling(const MkfTiling* pT, int X, int Y, int Z, int elemBytes) {
+  if (pT) return Tiling{pT->planePairs, pT->waves, pT->rowPairs};
+  return chooseTiling(X, Y, Z, elemBytes);
+}
+
+int checkTiling(const Tiling& tl) {
+  if (tl.TP != 8 || tl.ZW != 8 || tl.Ty < 1)
+    return fail(MKF_ERR_ARG, "unsupported tiling TP=%d ZW=%d Ty=%d", tl.TP, tl.ZW, tl.Ty);
+  return MKF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mkfLastError(void) { return gErr; }
+
+MkfTiling mkfDefaultTiling(const FieldDef def[3], int elemBytes) {
+  const Tiling tl = chooseTiling(def[0], def[1], def[2], elemBytes);
+  return MkfTiling{tl.TP, tl.ZW, tl.Ty};
+}
+
+int mkfPBMAsync(BMPackWord* pW, BMOrg* pO, const BMFieldInfo* pI, const BinMapF64* pM, void* stream) {
+  if (!pW || !pM) return fail(MKF_ERR_ARG, "NULL PBM or map");
+  FieldView fv;
+  FieldDef def[3];
+  int rc = resolveField(pI, &fv, def);
+  if (rc) return rc;
+  BMOrg org;
+  setBMO(&org, def, 0);
+  if (pO) *pO = org;
+  const hipError_t e = launchPBM(fv, *pM, pW, org.rowWS, org.planeWS, (hipStream_t)stream);
+  return e == hipSuccess ? MKF_OK : hipFail(e, "PBM launch");
+}
+
+int mkfBPFDAsync(size_t* pBPFD, const BMOrg* pO, const BMPackWord* pW, const MkfTiling* pT, void* stream) {
+  if (!pBPFD || !pO || !pW) return fail(MKF_ERR_ARG, "NULL argument");
+  const int X = (int)pO->rowElem, Y = (int)pO->rowPairs + 1, Z = (int)pO->planePairs + 1;
+  const Tiling tl = toTiling(pT, X, Y, Z, 4);
+  int rc = checkTiling(tl);
+  if (rc) return rc;
+  const hipError_t e = launchBPFD(pW, X, Y, Z, pO->rowWS, pO->planeWS, (unsigned long long*)pBPFD, tl,
+                                  (hipStream_t)stream);
+  return e == hipSuccess ? MKF_OK : hipFail(e, "BPFD launch");
+}
+
+int mkfFusedBPFD(KernInfo* pK, size_t* pBPFD, BMOrg* pO, const BMFieldInfo* pI, const BinMapF64* pM,
+                 const MkfTiling* pT, void* stream) {
+  if (!pBPFD || !pM) return fail(MKF_ERR_ARG, "NULL histogram or map");
+  FieldView fv;
+  FieldDef def[3];
+  int rc = resolveField(pI, &fv, def);
+  if (rc) return rc;
+  if (pO) setBMO(pO, def, 0);
+  const Tiling tl = toTiling(pT, fv.X, fv.Y, fv.Z, fv.enc & ENC_MASK_NUM);
+  rc = checkTiling(tl);
+  if (rc) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  if (pK) {
+    EventTimer t(s);
+    const hipError_t e = launchFused(fv, *pM, (unsigned long long*)pBPFD, tl, s);
+    pK->dtms[0] = 0.f;
+    pK->dtms[1] = t.stopMs();
+    if (e != hipSuccess) return hipFail(e, "fused launch");
+    const hipError_t e2 = hipGetLastError();
+    return e2 == hipSuccess ? MKF_OK : hipFail(e2, "fused kernel");
+  }
+  const hipError_t e = launchFused(fv, *pM, (unsigned long long*)pBPFD, tl, s);
+  return e == hipSuccess ? MKF_OK : hipFail(e, "fused launch");
+}
+
+// src/binMapCUDA.cu:145-264.
+BMPackWord* binMapCUDA(KernInfo* pK, BMPackWord* pW, BMOrg* pO, const BMFieldInfo* pI, const BinMapF64* pM) {
+  FieldView fv;
+  FieldDef def[3];
+  if (!pM || resolveField(pI, &fv, def) != MKF_OK) return NULL;
+  BMOrg org;
+  const size_t words = setBMO(&org, def, (char)pI->profID);
+  if (0 == words) return NULL;
+  if (pI->profID & 0x0F) {
+    fail(MKF_ERR_UNSUPPORTED, "partial PBM buffer (profID 0x%02X)", pI->profID);
+    return NULL;
+  }
+  if (pO) *pO = org;
+  if (NULL == pW) {
+    std::lock_guard<std::mutex> g(gLock);
+    if (gDevWWords < words) {
+      if (gDevW) hipFree(gDevW);
+      gDevW = nullptr;
+      gDevWWords = 0;
+      if (hipMalloc((void**)&gDevW, words * sizeof(BMPackWord)) != hipSuccess) {
+        gDevW = nullptr;
+        fail(MKF_ERR_MEM, "PBM buffer of %zu words", words);
+        return NULL;
+      }
+      gDevWWords = words;
+    }
+    pW = gDevW;
+  }
+  EventTimer t(0);
+  const hipError_t e = launchPBM(fv, *pM, pW, org.rowWS, org.planeWS, 0);
+  const float ms = t.stopMs();
+  if (pK) {
+    pK->dtms[0] = ms;
+  } else if (getenv("MKF_VERBOSE")) {
+    fprintf(stderr, "[mkf] binMapCUDA() - dt= %Gms\n", ms);
+  }
+  const hipError_t e2 = hipGetLastError();
+  if (e != hipSuccess || e2 != hipSuccess) {
+    hipFail(e != hipSuccess ? e : e2, "binMapCUDA");
+    return NULL;
+  }
+  return pW;
+}
+
+void binMapCUDACleanup(void) {
+  std::lock_guard<std::mutex> g(gLock);
+  if (gDevW) hipFree(gDevW);
+  gDevW = nullptr;
+  gDevWWords = 0;
+}
+
+// src/mkfCUDA.cu:13-61: every profile gives the same counts; all use the
+// same kernel here.
+size_t* mkfCUDAGetBPFD(KernInfo* pK, size_t* pBPFD, const BMOrg* pO, const BMPackWord* pW, const int profile) {
+  (void)profile;
+  if (pBPFD && pO && pW) {
+    EventTimer t(0);
+    const int rc = mkfBPFDAsync(pBPFD, pO, pW, NULL, 0);
+    const float ms = t.stopMs();
+    if (pK) {
+      pK->dtms[1] = ms;
+    } else if (getenv("MKF_VERBOSE")) {
+      fprintf(stderr, "[mkf] mkfCUDAGetBPFD() - dt= %Gms\n", ms);
+    }
+    const hipError_t e = hipGetLastError();
+    if (rc != MKF_OK || e != hipSuccess) fprintf(stderr, "[mkf] mkfCUDAGetBPFD: %s\n", gErr);
+  }
+  return pBPFD;
+}
+
+// src/mkfCUDA.cu:73-91.
+size_t* mkfCUDAGetBPFDH(KernInfo* pK, size_t* pBPFDH, const BMOrg* pO, const BMPackWord* pW, const int profile) {
+  const size_t bytes = sizeof(size_t) * MKF_BINS;
+  {
+    std::lock_guard<std::mutex> g(gLock);
+    if (NULL == pBPFDH) {
+      if (NULL == gHostBPFD && hipHostMalloc((void**)&gHostBPFD, bytes, hipHostMallocDefault) != hipSuccess)
+        gHostBPFD = nullptr;
+      pBPFDH = gHostBPFD;
+    }
+    if (NULL == gDevBPFD && hipMalloc((void**)&gDevBPFD, bytes) != hipSuccess) gDevBPFD = nullptr;
+  }
+  if (pBPFDH && gDevBPFD && pO && pW) {
+    if (hipMemset(gDevBPFD, 0, bytes) != hipSuccess) return NULL;
+    if (mkfCUDAGetBPFD(pK, (size_t*)gDevBPFD, pO, pW, profile)) {
+      if (hipMemcpy(pBPFDH, gDevBPFD, bytes, hipMemcpyDeviceToHost) == hipSuccess) return pBPFDH;
+    }
+  }
+  return NULL;
+}
+
+void mkfCUDACleanup(int lvl) {
+  {
+    std::lock_guard<std::mutex> g(gLock);
+    if (gHostBPFD) hipHostFree(gHostBPFD);
+    if (gDevBPFD) hipFree(gDevBPFD);
+    gHostBPFD = nullptr;
+    gDevBPFD = nullptr;
+  }
+  if (lvl > 0) {
+    binMapCUDACleanup();
+    hipDeviceReset();
+  }
+}
+
+// src/mkfCUDA.cu:102-167 (deprecated all-in-one path), same buffer protocol:
+// host field -> device, PBM (optionally copied back), histogram into pDZ
+// (copied back to pHZ when given). Returns 1, or 0 when the PBM failed.
+int mkfCUDAGetBPFDautoCtx(Context* pC, const FieldDef def[3], const BinMapF64* pM, const int profile) {
+  if (pC->pHF) {
+    if (NULL == pC->pDF && hipMalloc(&(pC->pDF), pC->bytesF) != hipSuccess) pC->pDF = NULL;
+    if (pC->pDF) hipMemcpy(pC->pDF, pC->pHF, pC->bytesF, hipMemcpyHostToDevice);
+    if (NULL == pC->pDU && hipMalloc((void**)&(pC->pDU), pC->bytesU) != hipSuccess) pC->pDU = NULL;
+    if (pC->pDF && pC->pDU) {
+      ConstFieldPtr fieldPtr[BMCU_FIELD_MAX];
+      BMFieldInfo fi;
+      setupFields(&fi, autoFieldPtr(fieldPtr, pC), pC->nField, def, pC->bytesElem, profile);
+      if (NULL == binMapCUDA(&(pC->ki), pC->pDU, &(pC->bmo), &fi, pM)) return 0;
+      if (pC->pHU) hipMemcpy(pC->pHU, pC->pDU, pC->bytesU, hipMemcpyDeviceToHost);
+    }
+  } else if (pC->pHU) {
+    hipMemcpy(pC->pDU, pC->pHU, pC->bytesU, hipMemcpyHostToDevice);
+    if (NULL == pC->pDZ && hipMalloc(&(pC->pDZ), pC->bytesZ) != hipSuccess) pC->pDZ = NULL;
+  }
+  if (pC->pDU && pC->pDZ) {
+    hipMemset(pC->pDZ, 0, pC->bytesZ);
+    mkfCUDAGetBPFD(&(pC->ki), (size_t*)(pC->pDZ), &(pC->bmo), pC->pDU, MKFCU_PROFILE_FAST);
+    if (pC->pHZ) hipMemcpy(pC->pHZ, pC->pDZ, pC->bytesZ, hipMemcpyDeviceToHost);
+  }
+  return 1;
+}
+
+// src/mkfACC.c:206-239: host double field in; host PBM (full organisation,
+// zero tail bits) and host histogram out; computed on the GPU.
+Bool32 mkfAccGetBPFDSimple(size_t rBPFD[MKF_BINS], BMPackWord* pW, const double* pF, const FieldDef def[3],
+                           const BinMapF64* pM) {
+  if (!rBPFD || !pW || !pF || !def || !pM) return 0;
+  memset(rBPFD, 0, sizeof(size_t) * MKF_BINS);
+  const size_t nF = (size_t)def[0] * def[1] * def[2];
+  BMOrg org;
+  const size_t words = setBMO(&org, def, 0);
+  void* dF = nullptr;
+  BMPackWord* dW = nullptr;
+  unsigned long long* dH = nullptr;
+  Bool32 ok = 0;
+  if (hipMalloc(&dF, nF * sizeof(double)) == hipSuccess && hipMalloc((void**)&dW, words * 4) == hipSuccess &&
+      hipMalloc((void**)&dH, MKF_BINS * 8) == hipSuccess &&
+      hipMemcpy(dF, pF, nF * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+      hipMemset(dH, 0, MKF_BINS * 8) == hipSuccess) {
+    FieldView fv{dF, ENC_F64, def[0], def[1], def[2], def[0], (long long)def[0] * def[1]};
+    const Tiling tl = chooseTiling(fv.X, fv.Y, fv.Z, 8);
+    if (launchPBM(fv, *pM, dW, org.rowWS, org.planeWS, 0) == hipSuccess &&
+        launchBPFD(dW, fv.X, fv.Y, fv.Z, org.rowWS, org.planeWS, dH, tl, 0) == hipSuccess &&
+        hipMemcpy(pW, dW, words * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+        hipMemcpy(rBPFD, dH, MKF_BINS * 8, hipMemcpyDeviceToHost) == hipSuccess) {
+      ok = 1;
+    } else {
+      hipFail(hipGetLastError(), "mkfAccGetBPFDSimple");
+    }
+  } else {
+    fail(MKF_ERR_MEM, "mkfAccGetBPFDSimple: device buffers for %zu voxels", nF);
+  }
+  if (dF) hipFree(dF);
+  if (dW) hipFree(dW);
+  if (dH) hipFree(dH);
+  return ok;
+}
+
+// src/mkfACC.c:248-318: host field of either encoding in, host histogram
+// out. The PBM is not needed by the caller, so the fused kernel is used.
+Bool32 mkfAccCUDAGetBPFD(size_t rBPFD[MKF_BINS], const void* pF, const FieldDef def[3], const NumEnc enc,
+                         const BinMapF64* pM) {
+  if (!rBPFD || !pF || !def || !pM || (enc != ENC_F32 && enc != ENC_F64)) return 0;
+  const size_t eb = enc & ENC_MASK_NUM;
+  FieldDef d[3];
+  if (copyNonZeroDef(d, def, 3) <= 0) return 0;
+  const size_t nF = (size_t)d[0] * d[1] * d[2];
+  void* dF = nullptr;
+  unsigned long long* dH = nullptr;
+  Bool32 ok = 0;
+  if (hipMalloc(&dF, nF * eb) == hipSuccess && hipMalloc((void**)&dH, MKF_BINS * 8) == hipSuccess &&
+      hipMemcpy(dF, pF, nF * eb, hipMemcpyHostToDevice) == hipSuccess &&
+      hipMemset(dH, 0, MKF_BINS * 8) == hipSuccess) {
+    FieldView fv{dF, enc, d[0], d[1], d[2], d[0], (long long)d[0] * d[1]};
+    const Tiling tl = chooseTiling(fv.X, fv.Y, fv.Z, (int)eb);
+    if (launchFused(fv, *pM, dH, tl, 0) == hipSuccess &&
+        hipMemcpy(rBPFD, dH, MKF_BINS * 8, hipMemcpyDeviceToHost) == hipSuccess) {
+      ok = 1;
+    } else {
+      hipFail(hipGetLastError(), "mkfAccCUDAGetBPFD");
+    }
+  } else {
+    fail(MKF_ERR_MEM, "mkfAccCUDAGetBPFD: device buffers for %zu voxels", nF);
+  }
+  if (dF) hipFree(dF);
+  if (dH) hipFree(dH);
+  return ok;
+}
+
+}  // extern "C"

@@ -1,0 +1,44 @@
+// mkf_internal.h - launchers shared by the C entry points (host side only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mkf.h"
+
+namespace mkf {
+
+// A planar-row field on the device: x is contiguous, y and z have element
+// strides sy and sz.
+struct FieldView {
+  const void* p;
+  NumEnc enc;  // ENC_F32 or ENC_F64
+  int X, Y, Z;
+  long long sy, sz;
+};
+
+// Work decomposition of the histogram kernels: each wave walks Ty row pairs
+// of TP plane pairs over one 64-wide x-segment; a workgroup holds ZW waves
+// stacked in z.
+struct Tiling {
+  int TP;  // plane pairs per wave (4, 8 or 16)
+  int ZW;  // waves per workgroup (4 or 8)
+  int Ty;  // row pairs per wave
+};
+
+Tiling chooseTiling(int X, int Y, int Z, int elemBytes);
+
+// Thresholds fv into the PBM pW (rowWS / planeWS word strides).
+hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint32_t rowWS, uint32_t planeWS,
+                     hipStream_t s);
+
+// Adds the BPFD of fv thresholded by m to bpfd (device, 256 x u64), reading
+// the field once; no PBM is written.
+hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, const Tiling& tl,
+                       hipStream_t s);
+
+// Adds the BPFD of the PBM pW (dims X, Y, Z) to bpfd.
+hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
+                      unsigned long long* bpfd, const Tiling& tl, hipStream_t s);
+
+}  // namespace mkf

@@ -1,0 +1,114 @@
+"""Seeded synthetic scalar fields for tests and benchmarks.
+
+Every generator is deterministic for a seed and returns a (Z, Y, X) tensor
+(x fastest, the planar layout the kernels read). Fields are generated once;
+the same bytes go to the GPU kernels and to the CPU checker.
+
+  uniform_field  i.i.d. U[0,1) float32 from a counter hash: (h >> 40) * 2^-24
+  grf_field      Gaussian random field: N(0,1) white noise, periodic Gaussian
+                 blur (exact Gaussian transfer function exp(-2 pi^2 s^2 k^2)
+                 applied in Fourier space), standardised to zero mean and unit
+                 variance in float64; float32 is the rounded float64 field
+  sdf_field      signed distance to a union of spheres and axis-aligned boxes
+"""
+
+import math
+
+import numpy as np
+import torch
+
+_MASK64 = (1 << 64) - 1
+
+
+def _mix64(x):
+    """splitmix64 finaliser on a uint64 numpy array."""
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def uniform_field(shape, seed=0):
+    """float32 U[0,1) field of shape (Z, Y, X) from a counter-based hash."""
+    Z, Y, X = (int(s) for s in shape)
+    out = np.empty((Z, Y, X), dtype=np.float32)
+    key = np.uint64((seed * 0x9E3779B97F4A7C15) & _MASK64)
+    plane = np.arange(Y * X, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        for z in range(Z):
+            c = plane + np.uint64(z * Y * X)
+            h = _mix64(c ^ key)
+            out[z] = ((h >> np.uint64(40)).astype(np.float32) * np.float32(2.0 ** -24)).reshape(Y, X)
+    return out
+
+
+def grf_field(shape, sigma, seed=0, device="cpu", dtype=torch.float64):
+    """Gaussian random field (Z, Y, X) with periodic Gaussian blur sigma.
+
+    Returns a torch tensor of `dtype` on `device`; float32 is obtained by
+    rounding the float64 field."""
+    Z, Y, X = (int(s) for s in shape)
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    f = torch.randn((Z, Y, X), generator=g, dtype=torch.float64, device=device)
+    F = torch.fft.rfftn(f)
+    del f
+    c = -2.0 * math.pi ** 2 * float(sigma) ** 2
+    kz = torch.fft.fftfreq(Z, device=device, dtype=torch.float64)
+    ky = torch.fft.fftfreq(Y, device=device, dtype=torch.float64)
+    kx = torch.fft.rfftfreq(X, device=device, dtype=torch.float64)
+    F *= torch.exp(c * kz * kz)[:, None, None]
+    F *= torch.exp(c * ky * ky)[None, :, None]
+    F *= torch.exp(c * kx * kx)[None, None, :]
+    f = torch.fft.irfftn(F, s=(Z, Y, X))
+    del F
+    mean = f.mean()
+    f -= mean
+    std = torch.sqrt((f * f).mean())
+    f /= std
+    if dtype != torch.float64:
+        f = f.to(dtype)
+    return f
+
+
+def sdf_shapes(shape, seed=0, n_spheres=32, n_boxes=8, r_range=(8.0, 60.0), box_range=(10.0, 80.0)):
+    """Seeded spheres (centre, radius) and boxes (lo, hi) inside a volume."""
+    Z, Y, X = (int(s) for s in shape)
+    rng = np.random.default_rng(seed)
+    dims = np.array([X, Y, Z], dtype=np.float64)
+    spheres = [(rng.uniform(0, dims), rng.uniform(*r_range)) for _ in range(n_spheres)]
+    boxes = []
+    for _ in range(n_boxes):
+        side = rng.uniform(box_range[0], box_range[1], size=3)
+        lo = rng.uniform(0, np.maximum(dims - side, 1.0))
+        boxes.append((lo, lo + side))
+    return spheres, boxes
+
+
+def sdf_field(shape, seed=0, device="cpu", dtype=torch.float32, **kw):
+    """Signed distance (negative inside) to the union of seeded spheres and
+    axis-aligned boxes, evaluated at voxel centres (x, y, z) = indices."""
+    Z, Y, X = (int(s) for s in shape)
+    spheres, boxes = sdf_shapes(shape, seed, **kw)
+    out = torch.empty((Z, Y, X), dtype=dtype, device=device)
+    x = torch.arange(X, device=device, dtype=torch.float64)[None, :]
+    y = torch.arange(Y, device=device, dtype=torch.float64)[:, None]
+    for z in range(Z):
+        zz = float(z)
+        d = torch.full((Y, X), float("inf"), dtype=torch.float64, device=device)
+        for c, r in spheres:
+            ds = torch.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (zz - c[2]) ** 2) - r
+            d = torch.minimum(d, ds)
+        for lo, hi in boxes:
+            cx, cy, cz = (lo + hi) / 2
+            hx, hy, hz = (hi - lo) / 2
+            qx = torch.abs(x - cx) - hx
+            qy = torch.abs(y - cy) - hy
+            qz = abs(zz - cz) - hz
+            ox = torch.clamp(qx, min=0)
+            oy = torch.clamp(qy, min=0)
+            oz = max(qz, 0.0)
+            outside = torch.sqrt(ox * ox + oy * oy + oz * oz)
+            inside = torch.clamp(torch.maximum(torch.maximum(qx, qy), torch.full_like(qx, qz)), max=0)
+            d = torch.minimum(d, outside + inside)
+        out[z] = d.to(dtype)
+    return out

@@ -1,0 +1,210 @@
+"""CPU tests of libmkf_amd.so's host side: exports, struct layouts, the
+reference's helper functions, and the measures (host code) against the
+oracle. No GPU entry point is called here."""
+
+import ctypes as C
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import mkf_amd as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("mkf.h", "mkf_ext.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+            name = m.group(1)
+            if name not in ("if", "defined"):
+                names.add(name)
+    return names
+
+
+def test_every_declared_symbol_is_exported(lib):
+    decl = declared_functions()
+    assert len(decl) >= 30
+    out = subprocess.run(["nm", "-D", "--defined-only", M.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = decl - exported
+    assert not missing, missing
+    assert decl == set(M.EXPORTED), decl ^ set(M.EXPORTED)
+    for name in decl:
+        assert hasattr(lib, name)
+
+
+def test_struct_layouts():
+    assert C.sizeof(M.BinMapF64) == 16
+    assert C.sizeof(M.BMOrg) == 20
+    assert C.sizeof(M.BMFieldInfo) == 32
+    assert M.BMFieldInfo.pD.offset == 8
+    assert C.sizeof(M.KernInfo) == 8
+    assert M.Context.bytesF.offset == 16
+
+
+@pytest.mark.parametrize("relop,m", [(">=", 6), ("<=", 3), ("!=", 5), ("<", 1), ("=", 2), (">", 4),
+                                      ("", 0), ("!", 7), ("!<", 6), ("!>=", 1), ("<>", 5), (">=x<", 6),
+                                      ("=!", 2), ("<<", 1)])
+def test_setBinMapF64_parsing(lib, relop, m):
+    bm = M.bin_map(relop, 0.5)
+    assert bm.m == m
+    assert bm.t[0] == 0.5
+
+
+def test_setBinMapF64_stores_float_threshold(lib):
+    bm = M.bin_map(">=", 0.1)
+    assert bm.t[0] == float(np.float32(0.1))
+
+
+@pytest.mark.parametrize("X,Y,Z", [(256, 256, 256), (509, 511, 517), (32, 2, 2), (33, 7, 5), (1, 1, 1)])
+def test_setBMO_full(lib, X, Y, Z):
+    o = M.BMOrg()
+    n = lib.setBMO(C.byref(o), M.dims((X, Y, Z)), b"\x00")
+    rw = (X + 31) // 32
+    assert (o.rowElem, o.rowPairs, o.planePairs, o.rowWS, o.planeWS) == (X, Y - 1, Z - 1, rw, rw * Y)
+    assert n == rw * Y * Z
+
+
+def test_setBMO_partial_buffer(lib):
+    o = M.BMOrg()
+    n = lib.setBMO(C.byref(o), M.dims((100, 50, 40)), b"\x01")
+    assert (o.rowPairs, o.planePairs, o.planeWS) == (1, 1, 4 * 2)
+    assert n == 4 * 2 * 2
+
+
+def test_copyNonZeroDef(lib):
+    d = (C.c_int * 3)()
+    assert lib.copyNonZeroDef(d, M.dims((1, 256, 256)), 3) == 2
+    assert list(d) == [256, 256, 1]
+    assert lib.copyNonZeroDef(d, M.dims((1, 1, 1)), 3) == 0
+    assert list(d) == [0, 0, 0]
+    assert lib.copyNonZeroDef(d, M.dims((5, 1, 7)), 3) == 2
+    assert list(d) == [5, 7, 1]
+
+
+def test_valid_ptr_by_mask(lib):
+    tab = (M.ConstFieldPtr * 4)()
+    tab[0].p, tab[1].p, tab[2].p, tab[3].p = 0x1000, None, 0x3000, 0x4000
+    assert lib.countValidPtrByMask(tab, 0b1111) == 3
+    assert lib.countValidPtrByMask(tab, 0b0010) == 0
+    out = (M.ConstFieldPtr * 4)()
+    assert lib.copyValidPtrByMask(out, 4, tab, 0b1101) == 3
+    assert [out[i].w for i in range(3)] == [0x1000, 0x3000, 0x4000]
+    assert lib.copyValidPtrByMask(out, 1, tab, 0b1100) == 1
+    assert out[0].w == 0x3000
+
+
+def test_genStride(lib):
+    fs = (C.c_long * 3)()
+    assert lib.genStride(fs, 3, 0, M.dims((4, 5, 6)), 1) == 3
+    assert list(fs) == [1, 4, 20]
+    assert lib.genStride(fs, 1, 2, M.dims((4, 5, 6)), 1) == 1
+    assert fs[0] == 20
+    assert lib.genStride(fs, 3, 0, M.dims((4, 5, 6)), 0) == 0
+    assert lib.copyOrGenStride(fs, 2, 1, M.dims((4, 5, 6)), (C.c_long * 3)(1, 7, 70)) == 2
+    assert list(fs)[:2] == [7, 70]
+
+
+def test_encodings(lib):
+    bits = C.c_int(0)
+    assert lib.encSizeN(C.byref(bits), 10, M.ENC_F32) == 40 and bits.value == 32
+    assert lib.encSizeN(C.byref(bits), 10, M.ENC_F64) == 80 and bits.value == 64
+    assert lib.encSizeN(C.byref(bits), 64, M.ENC_U1) == 8 and bits.value == 1
+    assert lib.encAlignment(M.ENC_F32) == 4 and lib.encAlignment(M.ENC_F64) == 8
+    assert lib.encAlignment(M.ENC_U1) == 1
+
+
+def test_setupFields(lib):
+    fi = M.BMFieldInfo()
+    ptrs = (C.c_void_p * 2)(0x1000, 0x2000)
+    d = M.dims((8, 8, 8))
+    assert lib.setupFields(C.byref(fi), ptrs, 2, d, 8, 3)
+    assert fi.elemID == M.ENC_F64 and fi.fieldTableMask == 3 and fi.profID == 3
+    assert not lib.setupFields(C.byref(fi), ptrs, 1, d, 2, 0)
+
+
+# ---------------------------------------------------------------------------
+# measures (host code in the product) vs the oracle's derivation
+# ---------------------------------------------------------------------------
+
+def product_measures(lib, h, s):
+    m = (C.c_float * 4)()
+    sym = C.create_string_buffer(16)
+    a = np.ascontiguousarray(h, dtype=np.uint64)
+    n = lib.mkfSelectMeasuresBPFD(m, sym, a.ctypes.data_as(C.POINTER(C.c_size_t)), C.c_float(s), 2)
+    return n, np.array(list(m), dtype=np.float32), sym.raw[:12]
+
+
+def test_tables_match_derivation(lib, orc):
+    group = np.ctypeslib.as_array(lib.mkfPatternGroupTable(), shape=(256,)).copy()
+    weight = np.ctypeslib.as_array(lib.mkfGroupWeightKTable(), shape=(22,)).copy()
+    lab, n = orc.orbits()
+    assert n == 22 and group.max() == 21
+    # same partition of the 256 patterns
+    for p in range(256):
+        for q in range(256):
+            if lab[p] == lab[q]:
+                assert group[p] == group[q]
+    assert len(set(group.tolist())) == 22
+    # Euler weights: the cubical-complex count everywhere except the two
+    # orbits where the reference's table holds other values.
+    ew = orc.euler_weights()
+    ref = weight[group].astype(np.int32)
+    differ = sorted({int(lab[p]) for p in range(256) if ref[p] != ew[p]})
+    assert differ == [0x3D, 0x7E]
+    assert ref[0x3D] == -8 and ew[0x3D] == -9
+    assert ref[0x7E] == -12 and ew[0x7E] == -18
+    np.testing.assert_array_equal(ref, orc.reference_weights())
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_measures_match_oracle(lib, orc, seed):
+    rng = np.random.default_rng(seed)
+    scale = [10, 10 ** 6, 10 ** 9, 10 ** 12, 2 ** 40, 3][seed]
+    h = rng.integers(0, scale, size=256, dtype=np.uint64)
+    for s in (0.0, 3.0 / 768, 1.0, -1.0):
+        n, m, sym = product_measures(lib, h, s)
+        mo, _ = orc.measures(h, s)
+        assert n == 2 and sym == b"Kv       Vv\x00"
+        np.testing.assert_array_equal(m, mo)
+
+
+def test_measures_int32_wrap_is_reproduced(lib, orc):
+    h = np.zeros(256, dtype=np.uint64)
+    h[0b11101111] = 3 * 10 ** 9  # one group, weight -3 or similar, count > 2^31
+    h[1] = 2 ** 33 + 5            # truncated to 5 by the int conversion
+    n, m, _ = product_measures(lib, h, 0.0)
+    mo, ovf = orc.measures(h, 0.0)
+    assert ovf
+    np.testing.assert_array_equal(m, mo)
+
+
+def test_measures_on_shapes(lib, orc):
+    # box with chi = 1 inside a padded volume: Kv numerator = 4 pi
+    bits = np.pad(np.ones((5, 6, 7), bool), 2)
+    h = orc.bpfd_numpy(bits)
+    _, m, _ = product_measures(lib, h, 0.0)
+    assert m[0] == np.float32(24 * math.pi / 6)
+    lower = bits[:-1, :-1, :-1]
+    assert m[3] == np.float32(lower.sum()) / np.float32(lower.size)
+
+
+def test_ref_measures_are_undefined(lib):
+    m = (C.c_float * 4)()
+    h = np.ones(256, dtype=np.uint64)
+    lib.mkfRefMeasureBPFD(m, h.ctypes.data_as(C.POINTER(C.c_size_t)), C.c_float(1.0))
+    assert list(m) == [-1, -1, -1, -1]
+
+
+def test_volFrac(lib):
+    h = np.zeros(256, dtype=np.uint64)
+    h[1::2] = 3
+    h[0::2] = 1
+    assert lib.volFrac(h.ctypes.data_as(C.POINTER(C.c_size_t))) == np.float32(384) / np.float32(512)
