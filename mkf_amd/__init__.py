@@ -114,10 +114,18 @@ _lib = None
 
 
 def load():
-    """Loads libmkf_amd.so (raises if it has not been built)."""
+    """Loads libmkf_amd.so (raises if it has not been built).
+
+    torch, when installed, is imported first: its wheel carries its own HIP
+    runtime with the same soname (libamdhip64.so.7), and loading torch first
+    makes this library bind to that one runtime instead of a second copy."""
     global _lib
     if _lib is not None:
         return _lib
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python mkf_amd/build.py` "
                           "(or __graft_entry__.build()); there is no CPU fallback")

@@ -70,7 +70,7 @@ def build_library(verbose=False):
 
 def build_oracle(verbose=False):
     tmp = ORACLE_LIB + ".tmp"
-    _run(["gcc", "-O3", "-march=x86-64-v2", "-fopenmp", "-std=gnu11", "-fPIC", "-shared", "-Wall",
+    _run(["gcc", "-O3", "-march=x86-64-v2", "-std=gnu11", "-fPIC", "-shared", "-Wall", "-pthread",
           ORACLE_SRC, "-o", tmp, "-lm"])
     os.replace(tmp, ORACLE_LIB)
     if verbose:

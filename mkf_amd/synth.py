@@ -8,7 +8,9 @@ the same bytes go to the GPU kernels and to the CPU checker.
   grf_field      Gaussian random field: N(0,1) white noise, periodic Gaussian
                  blur (exact Gaussian transfer function exp(-2 pi^2 s^2 k^2)
                  applied in Fourier space), standardised to zero mean and unit
-                 variance in float64; float32 is the rounded float64 field
+                 variance in float64; float32 is the rounded float64 field.
+                 On the GPU torch generates it; on the CPU numpy does (the
+                 two give different fields for one seed)
   sdf_field      signed distance to a union of spheres and axis-aligned boxes
 """
 
@@ -47,6 +49,8 @@ def grf_field(shape, sigma, seed=0, device="cpu", dtype=torch.float64):
     Returns a torch tensor of `dtype` on `device`; float32 is obtained by
     rounding the float64 field."""
     Z, Y, X = (int(s) for s in shape)
+    if str(device) == "cpu":
+        return torch.from_numpy(_grf_numpy((Z, Y, X), sigma, seed)).to(dtype)
     g = torch.Generator(device=device)
     g.manual_seed(int(seed))
     f = torch.randn((Z, Y, X), generator=g, dtype=torch.float64, device=device)
@@ -70,6 +74,22 @@ def grf_field(shape, sigma, seed=0, device="cpu", dtype=torch.float64):
     return f
 
 
+def _grf_numpy(shape, sigma, seed):
+    """CPU variant of grf_field (numpy's FFT; torch's CPU FFT is not used)."""
+    Z, Y, X = shape
+    rng = np.random.default_rng(int(seed))
+    F = np.fft.rfftn(rng.standard_normal((Z, Y, X)))
+    c = -2.0 * math.pi ** 2 * float(sigma) ** 2
+    kz, ky, kx = np.fft.fftfreq(Z), np.fft.fftfreq(Y), np.fft.rfftfreq(X)
+    F *= np.exp(c * kz * kz)[:, None, None]
+    F *= np.exp(c * ky * ky)[None, :, None]
+    F *= np.exp(c * kx * kx)[None, None, :]
+    f = np.fft.irfftn(F, s=(Z, Y, X))
+    f -= f.mean()
+    f /= np.sqrt((f * f).mean())
+    return np.ascontiguousarray(f)
+
+
 def sdf_shapes(shape, seed=0, n_spheres=32, n_boxes=8, r_range=(8.0, 60.0), box_range=(10.0, 80.0)):
     """Seeded spheres (centre, radius) and boxes (lo, hi) inside a volume."""
     Z, Y, X = (int(s) for s in shape)
@@ -84,31 +104,27 @@ def sdf_shapes(shape, seed=0, n_spheres=32, n_boxes=8, r_range=(8.0, 60.0), box_
     return spheres, boxes
 
 
-def sdf_field(shape, seed=0, device="cpu", dtype=torch.float32, **kw):
+def sdf_field(shape, seed=0, device="cpu", dtype=torch.float32, zchunk=32, **kw):
     """Signed distance (negative inside) to the union of seeded spheres and
     axis-aligned boxes, evaluated at voxel centres (x, y, z) = indices."""
     Z, Y, X = (int(s) for s in shape)
     spheres, boxes = sdf_shapes(shape, seed, **kw)
     out = torch.empty((Z, Y, X), dtype=dtype, device=device)
-    x = torch.arange(X, device=device, dtype=torch.float64)[None, :]
-    y = torch.arange(Y, device=device, dtype=torch.float64)[:, None]
-    for z in range(Z):
-        zz = float(z)
-        d = torch.full((Y, X), float("inf"), dtype=torch.float64, device=device)
+    x = torch.arange(X, device=device, dtype=torch.float64)[None, None, :]
+    y = torch.arange(Y, device=device, dtype=torch.float64)[None, :, None]
+    for z0 in range(0, Z, zchunk):
+        z1 = min(Z, z0 + zchunk)
+        z = torch.arange(z0, z1, device=device, dtype=torch.float64)[:, None, None]
+        d = torch.full((z1 - z0, Y, X), float("inf"), dtype=torch.float64, device=device)
         for c, r in spheres:
-            ds = torch.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (zz - c[2]) ** 2) - r
-            d = torch.minimum(d, ds)
+            d = torch.minimum(d, torch.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) - r)
         for lo, hi in boxes:
-            cx, cy, cz = (lo + hi) / 2
-            hx, hy, hz = (hi - lo) / 2
-            qx = torch.abs(x - cx) - hx
-            qy = torch.abs(y - cy) - hy
-            qz = abs(zz - cz) - hz
-            ox = torch.clamp(qx, min=0)
-            oy = torch.clamp(qy, min=0)
-            oz = max(qz, 0.0)
-            outside = torch.sqrt(ox * ox + oy * oy + oz * oz)
-            inside = torch.clamp(torch.maximum(torch.maximum(qx, qy), torch.full_like(qx, qz)), max=0)
+            ctr = (lo + hi) / 2
+            half = (hi - lo) / 2
+            q = [torch.abs(x - ctr[0]) - half[0], torch.abs(y - ctr[1]) - half[1], torch.abs(z - ctr[2]) - half[2]]
+            o = [torch.clamp(qa, min=0) for qa in q]
+            outside = torch.sqrt(o[0] ** 2 + o[1] ** 2 + o[2] ** 2)
+            inside = torch.clamp(torch.maximum(torch.maximum(q[0], q[1]), q[2]), max=0)
             d = torch.minimum(d, outside + inside)
-        out[z] = d.to(dtype)
+        out[z0:z1] = d.to(dtype)
     return out

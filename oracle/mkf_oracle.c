@@ -25,14 +25,54 @@
  * Pinning: see the header of oracle/oracle.py.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
-#ifdef _OPENMP
-#include <omp.h>
-#endif
 
 #define ORC_BINS 256
+#define ORC_MAX_THREADS 256
+
+/* Worker threads (plain pthreads: the process may already hold another
+ * OpenMP runtime, e.g. the one bundled with torch). 1 = serial. */
+static int gThreads = 1;
+
+typedef void (*orc_range_fn)(void *ctx, long beg, long end, int tid);
+
+typedef struct {
+  orc_range_fn fn;
+  void *ctx;
+  long beg, end;
+  int tid;
+} orc_job;
+
+static void *orc_job_main(void *p) {
+  orc_job *j = (orc_job *)p;
+  j->fn(j->ctx, j->beg, j->end, j->tid);
+  return NULL;
+}
+
+/* Splits [0, n) into gThreads contiguous ranges and runs fn on each. */
+static void orc_parallel(long n, orc_range_fn fn, void *ctx) {
+  int nt = gThreads < 1 ? 1 : gThreads;
+  if (nt > n) nt = (int)(n > 0 ? n : 1);
+  if (nt == 1) {
+    fn(ctx, 0, n, 0);
+    return;
+  }
+  pthread_t th[ORC_MAX_THREADS];
+  orc_job jobs[ORC_MAX_THREADS];
+  for (int t = 0; t < nt; t++) {
+    jobs[t].fn = fn;
+    jobs[t].ctx = ctx;
+    jobs[t].beg = n * t / nt;
+    jobs[t].end = n * (t + 1) / nt;
+    jobs[t].tid = t;
+    if (t > 0) pthread_create(&th[t], NULL, orc_job_main, &jobs[t]);
+  }
+  orc_job_main(&jobs[0]);
+  for (int t = 1; t < nt; t++) pthread_join(th[t], NULL);
+}
 
 /* ---- threshold ---------------------------------------------------------- */
 
@@ -48,20 +88,38 @@ uint32_t orc_relbit_f64(double f, double t, uint32_t m) { return orc_relbit(f, t
 
 /* Row-aligned PBM of a double field: word (x>>5) + y*rowWS + z*planeWS, bit
  * x & 31; 32 compares per word (src/binMapACC.c:43-58, 77-93). */
-void orc_pbm_f64(uint32_t *pW, const double *pF, int X, int Y, int Z, double t, uint32_t m) {
-  const int rowWS = (X + 31) >> 5;
-  const long nRows = (long)Y * Z;
-#pragma omp parallel for schedule(static)
-  for (long r = 0; r < nRows; r++) {
-    const double *pr = pF + r * (long)X;
-    uint32_t *pw = pW + r * (long)rowWS;
+typedef struct {
+  uint32_t *pW;
+  const void *pF;
+  int X, f32;
+  double t;
+  uint32_t m;
+} orc_pbm_ctx;
+
+static void orc_pbm_rows(void *p, long beg, long end, int tid) {
+  (void)tid;
+  const orc_pbm_ctx *c = (const orc_pbm_ctx *)p;
+  const int X = c->X, rowWS = (X + 31) >> 5;
+  for (long r = beg; r < end; r++) {
+    uint32_t *pw = c->pW + r * (long)rowWS;
     for (int wi = 0; wi < rowWS; wi++) {
       uint32_t w = 0;
       const int n = (X - wi * 32) < 32 ? (X - wi * 32) : 32;
-      for (int j = 0; j < n; j++) w |= orc_relbit(pr[wi * 32 + j], t, m) << j;
+      if (c->f32) {
+        const float *pr = (const float *)c->pF + r * (long)X + wi * 32;
+        for (int j = 0; j < n; j++) w |= orc_relbit((double)pr[j], c->t, c->m) << j;
+      } else {
+        const double *pr = (const double *)c->pF + r * (long)X + wi * 32;
+        for (int j = 0; j < n; j++) w |= orc_relbit(pr[j], c->t, c->m) << j;
+      }
       pw[wi] = w;
     }
   }
+}
+
+void orc_pbm_f64(uint32_t *pW, const double *pF, int X, int Y, int Z, double t, uint32_t m) {
+  orc_pbm_ctx c = {pW, pF, X, 0, t, m};
+  orc_parallel((long)Y * Z, orc_pbm_rows, &c);
 }
 
 /* The reference CPU path is double only (src/binMapACC.h:20); a float field
@@ -69,20 +127,8 @@ void orc_pbm_f64(uint32_t *pW, const double *pF, int X, int Y, int Z, double t, 
  * float, which is what the reference GPU path does for float fields
  * (src/binMapCUDA.cuh:138-139). */
 void orc_pbm_f32(uint32_t *pW, const float *pF, int X, int Y, int Z, double t, uint32_t m) {
-  const int rowWS = (X + 31) >> 5;
-  const long nRows = (long)Y * Z;
-  const double tf = (double)(float)t;
-#pragma omp parallel for schedule(static)
-  for (long r = 0; r < nRows; r++) {
-    const float *pr = pF + r * (long)X;
-    uint32_t *pw = pW + r * (long)rowWS;
-    for (int wi = 0; wi < rowWS; wi++) {
-      uint32_t w = 0;
-      const int n = (X - wi * 32) < 32 ? (X - wi * 32) : 32;
-      for (int j = 0; j < n; j++) w |= orc_relbit((double)pr[wi * 32 + j], tf, m) << j;
-      pw[wi] = w;
-    }
-  }
+  orc_pbm_ctx c = {pW, pF, X, 1, (double)(float)t, m};
+  orc_parallel((long)Y * Z, orc_pbm_rows, &c);
 }
 
 /* ---- histogram: chunk walk (restatement) -------------------------------- */
@@ -133,23 +179,36 @@ static void orc_add_row(uint64_t h[ORC_BINS], const uint32_t *r0, const uint32_t
 /* ADDS the histogram of plane pairs [zBeg, zEnd) (src/mkfACC.c:156-188).
  * Requires X >= 32. Plane pairs are split over threads, each with a private
  * histogram merged by integer addition. Returns 0, or -1 for X < 32. */
+typedef struct {
+  const uint32_t *pW;
+  int X, Y, rowWS, zBeg;
+  long planeWS;
+  uint64_t (*loc)[ORC_BINS];
+} orc_walk_ctx;
+
+static void orc_walk_planes(void *p, long beg, long end, int tid) {
+  const orc_walk_ctx *c = (const orc_walk_ctx *)p;
+  uint64_t *loc = c->loc[tid];
+  for (long zi = beg; zi < end; zi++) {
+    const uint32_t *p0 = c->pW + (c->zBeg + zi) * c->planeWS;
+    const uint32_t *p1 = p0 + c->planeWS;
+    for (int y = 0; y < c->Y - 1; y++)
+      orc_add_row(loc, p0 + (long)y * c->rowWS, p1 + (long)y * c->rowWS, c->rowWS, c->X);
+  }
+}
+
 int orc_bpfd_chunk_range(uint64_t h[ORC_BINS], const uint32_t *pW, int X, int Y, int Z, int rowWS, long planeWS,
                          int zBeg, int zEnd) {
   (void)Z;
   if (X < 32) return -1;
-#pragma omp parallel
-  {
-    uint64_t loc[ORC_BINS];
-    memset(loc, 0, sizeof(loc));
-#pragma omp for schedule(static)
-    for (int z = zBeg; z < zEnd; z++) {
-      const uint32_t *p0 = pW + (long)z * planeWS;
-      const uint32_t *p1 = p0 + planeWS;
-      for (int y = 0; y < Y - 1; y++) orc_add_row(loc, p0 + (long)y * rowWS, p1 + (long)y * rowWS, rowWS, X);
-    }
-#pragma omp critical
-    for (int i = 0; i < ORC_BINS; i++) h[i] += loc[i];
-  }
+  if (zEnd <= zBeg) return 0;
+  uint64_t (*loc)[ORC_BINS] = calloc(ORC_MAX_THREADS, sizeof(*loc));
+  if (!loc) return -2;
+  orc_walk_ctx c = {pW, X, Y, rowWS, zBeg, planeWS, loc};
+  orc_parallel(zEnd - zBeg, orc_walk_planes, &c);
+  for (int t = 0; t < ORC_MAX_THREADS; t++)
+    for (int i = 0; i < ORC_BINS; i++) h[i] += loc[t][i];
+  free(loc);
   return 0;
 }
 
@@ -196,20 +255,12 @@ int orc_bpfd_field_f32(uint64_t h[ORC_BINS], uint32_t *pW, const float *pF, int 
   return orc_bpfd_chunk(h, pW, X, Y, Z, rowWS, (long)rowWS * Y);
 }
 
-int orc_num_threads(void) {
-#ifdef _OPENMP
-  return omp_get_max_threads();
-#else
-  return 1;
-#endif
-}
+int orc_num_threads(void) { return gThreads; }
 
 void orc_set_threads(int n) {
-#ifdef _OPENMP
-  if (n > 0) omp_set_num_threads(n);
-#else
-  (void)n;
-#endif
+  if (n < 1) n = 1;
+  if (n > ORC_MAX_THREADS) n = ORC_MAX_THREADS;
+  gThreads = n;
 }
 
 /* ---- cell geometry: Euler weights and symmetry orbits ------------------- */

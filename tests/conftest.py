@@ -35,6 +35,7 @@ def lib():
 def orc():
     from oracle import oracle
     oracle.load()
+    oracle.set_threads(min(16, os.cpu_count() or 1))
     return oracle
 
 

@@ -1,0 +1,240 @@
+"""GPU parity: every result of libmkf_amd.so's HIP kernels, called through
+the C ABI, equals the CPU oracle bit for bit (integer/bit work: exact)."""
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import mkf_amd as M
+from mkf_amd import synth
+from tests.gpu_util import dev_bpfd_from_pbm, dev_fused, dev_pbm, host_np
+
+pytestmark = pytest.mark.gpu
+
+RELOPS = ["<", "=", "<=", ">", "!=", ">=", "", "!"]
+
+
+def oracle_all(orc, f_np, relop, thr):
+    """(pbm words, hist) of the oracle for a host field and relation."""
+    m = M.bin_map(relop, thr)
+    words = orc.pbm_c(f_np, m.t[0], m.m)
+    X = f_np.shape[2]
+    if min(f_np.shape) < 2:
+        return words, np.zeros(256, np.uint64)
+    hist = orc.bpfd_chunk(words, X) if X >= 32 else orc.bpfd_direct_c(words, X)
+    return words, hist
+
+
+def check_all_paths(lib, orc, field, relop, thr):
+    f_np = host_np(field)
+    w_ref, h_ref = oracle_all(orc, f_np, relop, thr)
+    words, dw, org, _ = dev_pbm(lib, field, relop, thr)
+    np.testing.assert_array_equal(words, w_ref)
+    for prof in (M.MKFCU_PROFILE_FAST, M.MKFCU_PROFILE_FLEX, 0):
+        h, _ = dev_bpfd_from_pbm(lib, dw, org, prof)
+        np.testing.assert_array_equal(h, h_ref)
+    h, _ = dev_fused(lib, field, relop, thr)
+    np.testing.assert_array_equal(h, h_ref)
+    Z, Y, X = field.shape
+    assert int(h_ref.sum()) == max(X - 1, 0) * max(Y - 1, 0) * max(Z - 1, 0)
+    return h_ref
+
+
+SMALL = [(2, 2, 2), (3, 3, 3), (5, 4, 31), (4, 5, 32), (3, 7, 33), (6, 3, 63), (3, 3, 64), (4, 4, 65),
+         (5, 6, 127), (3, 4, 128), (4, 3, 129), (9, 17, 200), (2, 70, 1000), (5, 9, 1025), (17, 2, 70),
+         (70, 33, 2)]
+
+
+@pytest.mark.parametrize("shape", SMALL)
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_small_shapes(lib, orc, cuda, shape, dtype):
+    print(f"small shape {shape} {dtype}", flush=True)
+    g = torch.Generator(device=cuda).manual_seed(hash(shape) & 0xFFFF)
+    f = torch.rand(shape, generator=g, device=cuda, dtype=torch.float64).to(dtype)
+    check_all_paths(lib, orc, f, ">=", 0.5)
+
+
+@pytest.mark.parametrize("relop", RELOPS)
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_relations_and_special_values(lib, orc, cuda, relop, dtype):
+    print(f"relation {relop!r} {dtype}", flush=True)
+    rng = np.random.default_rng(5)
+    choices = np.array([0.0, -0.0, 1e-40, -1e-40, 5e-324, np.nan, np.inf, -np.inf, 0.25, -0.25, 1.0])
+    vals = rng.choice(choices, size=(7, 9, 97)).astype(np.float64)
+    f = torch.from_numpy(vals).to(dtype).to(cuda)
+    for thr in (0.0, 0.25, -0.25):
+        check_all_paths(lib, orc, f, relop, thr)
+
+
+def test_f32_threshold_rounding(lib, orc, cuda):
+    # setBinMapF64 stores float(0.1); a float field of float(0.1) is EQ.
+    f = torch.full((4, 4, 40), float(np.float32(0.1)), dtype=torch.float32, device=cuda)
+    words, _, _, _ = dev_pbm(lib, f, "=", 0.1)
+    assert np.all(words[..., 0] == 0xFFFFFFFF) and np.all(words[..., 1] == 0xFF)
+    # A double field holding the double 0.1 is not EQ to float(0.1).
+    f64 = torch.full((4, 4, 40), 0.1, dtype=torch.float64, device=cuda)
+    words, _, _, _ = dev_pbm(lib, f64, "=", 0.1)
+    assert np.all(words == 0)
+
+
+@pytest.mark.parametrize("ty", [1, 2, 5, 13, 64, 200])
+def test_fused_tilings(lib, orc, cuda, ty):
+    f = synth.grf_field((80, 90, 100), sigma=2.0, seed=3, device=cuda, dtype=torch.float32)
+    _, h_ref = oracle_all(orc, host_np(f), ">=", 0.0)
+    h, _ = dev_fused(lib, f, ">=", 0.0, tiling=(8, 8, ty))
+    np.testing.assert_array_equal(h, h_ref)
+
+
+def test_fused_accumulates(lib, orc, cuda):
+    f = synth.grf_field((30, 40, 70), sigma=1.5, seed=4, device=cuda, dtype=torch.float64)
+    _, h_ref = oracle_all(orc, host_np(f), "<", 0.2)
+    pre = np.arange(256, dtype=np.uint64) * 7
+    h, _ = dev_fused(lib, f, "<", 0.2, pre=pre)
+    np.testing.assert_array_equal(h, h_ref + pre)
+
+
+def test_dims_of_one_are_compacted(lib, orc, cuda):
+    f = torch.rand((40, 50, 1), device=cuda, dtype=torch.float32)  # X = 1
+    Z, Y, X = f.shape
+    # The reference drops X = 1 and treats the field as 50 x 40 x 1.
+    bm = M.bin_map(">=", 0.5)
+    org = M.BMOrg()
+    words = torch.zeros(4096, dtype=torch.int32, device=cuda)
+    fi = M.FieldInfo(f.data_ptr(), (X, Y, Z), M.ENC_F32)
+    assert lib.binMapCUDA(None, words.data_ptr(), C.byref(org), fi.ref, C.byref(bm))
+    assert (org.rowElem, org.rowPairs, org.planePairs, org.rowWS) == (50, 39, 0, 2)
+    w_ref = orc.pbm_c(host_np(f).reshape(1, 40, 50), bm.t[0], bm.m)
+    got = words[: 2 * 40].cpu().numpy().view(np.uint32).reshape(1, 40, 2)
+    np.testing.assert_array_equal(got, w_ref)
+
+
+def test_strided_subvolume(lib, orc, cuda):
+    full = synth.grf_field((40, 50, 130), sigma=2.0, seed=8, device=cuda, dtype=torch.float32)
+    sub = full[3:33, 5:45, 7:107]  # (30, 40, 100) view
+    Z, Y, X = sub.shape
+    bm = M.bin_map(">=", 0.1)
+    h = torch.zeros(256, dtype=torch.int64, device=cuda)
+    fi = M.FieldInfo(sub.data_ptr(), (X, Y, Z), M.ENC_F32, strides=(1, 130, 130 * 50))
+    M.check(lib.mkfFusedBPFD(None, h.data_ptr(), None, fi.ref, C.byref(bm), None, None))
+    torch.cuda.synchronize()
+    _, h_ref = oracle_all(orc, np.ascontiguousarray(host_np(sub)), ">=", 0.1)
+    np.testing.assert_array_equal(h.cpu().numpy().view(np.uint64), h_ref)
+
+
+def test_invalid_inputs_return_null(lib, cuda):
+    bm = M.bin_map(">=", 0.5)
+    org = M.BMOrg()
+    f = torch.rand((4, 4, 40), device=cuda)
+    fi = M.FieldInfo(f.data_ptr(), (40, 4, 4), M.ENC_F32)
+    fi.info.fieldTableMask = 0
+    assert not lib.binMapCUDA(None, None, C.byref(org), fi.ref, C.byref(bm))
+    fi = M.FieldInfo(f.data_ptr(), (40, 4, 4), 0x08)  # ENC_U8: not a float field
+    assert not lib.binMapCUDA(None, None, C.byref(org), fi.ref, C.byref(bm))
+    fi = M.FieldInfo(f.data_ptr(), (40, 4, 4), M.ENC_F32, strides=(2, 80, 320))
+    assert not lib.binMapCUDA(None, None, C.byref(org), fi.ref, C.byref(bm))
+    assert "stride" in M.last_error()
+    assert lib.mkfCUDAGetBPFD(None, None, C.byref(org), None, 3) is None
+
+
+def test_library_pbm_buffer_grows(lib, orc, cuda):
+    bm = M.bin_map(">=", 0.5)
+    for shape in [(8, 8, 64), (40, 50, 300)]:
+        f = torch.rand(shape, device=cuda)
+        Z, Y, X = shape
+        fi = M.FieldInfo(f.data_ptr(), (X, Y, Z), M.ENC_F32)
+        org = M.BMOrg()
+        p = lib.binMapCUDA(None, None, C.byref(org), fi.ref, C.byref(bm))
+        assert p
+        n = org.planeWS * (org.planePairs + 1)
+        got = (C.c_uint32 * n)()
+        torch.cuda.synchronize()
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so.7")
+        assert hip.hipMemcpy(got, C.c_void_p(p), n * 4, 2) == 0
+        w_ref = orc.pbm_c(host_np(f), bm.t[0], bm.m)
+        np.testing.assert_array_equal(np.frombuffer(got, dtype=np.uint32).reshape(w_ref.shape), w_ref)
+    lib.binMapCUDACleanup()
+
+
+def test_host_result_and_host_field_entry_points(lib, orc, cuda):
+    f64 = synth.grf_field((33, 41, 75), sigma=2.0, seed=6, device="cpu", dtype=torch.float64).numpy()
+    f64 = np.ascontiguousarray(f64)
+    bm = M.bin_map(">=", 0.0)
+    w_ref, h_ref = oracle_all(orc, f64, ">=", 0.0)
+    Z, Y, X = f64.shape
+    d = M.dims((X, Y, Z))
+    # mkfAccGetBPFDSimple: host double field -> host PBM + host BPFD
+    h = np.zeros(256, dtype=np.uint64)
+    w = np.zeros(w_ref.size, dtype=np.uint32)
+    ok = lib.mkfAccGetBPFDSimple(h.ctypes.data_as(C.POINTER(C.c_size_t)), w.ctypes.data, f64.ctypes.data, d,
+                                 C.byref(bm))
+    assert ok
+    np.testing.assert_array_equal(w.reshape(w_ref.shape), w_ref)
+    np.testing.assert_array_equal(h, h_ref)
+    # mkfAccCUDAGetBPFD with both encodings
+    for arr, enc in ((f64, M.ENC_F64), (f64.astype(np.float32), M.ENC_F32)):
+        _, h_e = oracle_all(orc, arr, ">=", 0.0)
+        h = np.zeros(256, dtype=np.uint64)
+        assert lib.mkfAccCUDAGetBPFD(h.ctypes.data_as(C.POINTER(C.c_size_t)), arr.ctypes.data, d, enc, C.byref(bm))
+        np.testing.assert_array_equal(h, h_e)
+    # mkfCUDAGetBPFDH: library host buffer and caller buffer
+    field = torch.from_numpy(f64).to(cuda)
+    _, dw, org, _ = dev_pbm(lib, field, ">=", 0.0)
+    p = lib.mkfCUDAGetBPFDH(None, None, C.byref(org), dw.data_ptr(), 3)
+    assert p
+    got = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint64)), shape=(256,)).copy()
+    np.testing.assert_array_equal(got, h_ref)
+    h = np.zeros(256, dtype=np.uint64)
+    ki = M.KernInfo()
+    p2 = lib.mkfCUDAGetBPFDH(C.byref(ki), h.ctypes.data, C.byref(org), dw.data_ptr(), 2)
+    assert p2 == h.ctypes.data
+    np.testing.assert_array_equal(h, h_ref)
+    assert ki.dtms[1] > 0
+    lib.mkfCUDACleanup(0)
+
+
+def test_auto_context(lib, orc, cuda):
+    X, Y, Z = 96, 40, 30
+    f = synth.uniform_field((Z, Y, X), seed=2)
+    w_ref, h_ref = oracle_all(orc, f, ">=", 0.5)
+    ctx = M.Context()
+    ctx.nField, ctx.nElem, ctx.enc, ctx.bytesElem = 1, X * Y * Z, M.ENC_F32, 4
+    ctx.bytesF = f.nbytes
+    ctx.nU = w_ref.size
+    ctx.bytesU = w_ref.size * 4
+    ctx.nZ, ctx.bytesZ = 256, 2048
+    hu = np.zeros(w_ref.size, dtype=np.uint32)
+    hz = np.zeros(256, dtype=np.uint64)
+    ctx.pHF, ctx.pHU, ctx.pHZ = f.ctypes.data, hu.ctypes.data, hz.ctypes.data
+    dz = torch.zeros(256, dtype=torch.int64, device=cuda)
+    ctx.pDZ = dz.data_ptr()
+    assert lib.mkfCUDAGetBPFDautoCtx(C.byref(ctx), M.dims((X, Y, Z)), C.byref(M.bin_map(">=", 0.5)), 0) == 1
+    np.testing.assert_array_equal(hu.reshape(w_ref.shape), w_ref)
+    np.testing.assert_array_equal(hz, h_ref)
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipFree(C.c_void_p(ctx.pDF))
+    hip.hipFree(C.c_void_p(ctx.pDU))
+
+
+def test_config_small_uniform_256(lib, orc, cuda):
+    """256^3 U[0,1) float32, threshold >= 0.5: PBM, BPFD and measures."""
+    f_np = synth.uniform_field((256, 256, 256), seed=1)
+    f = torch.from_numpy(f_np).to(cuda)
+    h_ref = check_all_paths(lib, orc, f, ">=", 0.5)
+    s = 3.0 / (256 * 3)
+    m_ref, _ = orc.measures(h_ref, s)
+    m = (C.c_float * 4)()
+    sym = C.create_string_buffer(16)
+    h, _ = dev_fused(lib, f, ">=", 0.5)
+    lib.mkfSelectMeasuresBPFD(m, sym, h.ctypes.data_as(C.POINTER(C.c_size_t)), C.c_float(s), 2)
+    np.testing.assert_array_equal(np.array(list(m), np.float32), m_ref)
+    assert abs(m[3] - 0.5) < 1e-3
+
+
+def test_config_odd_dims_sdf(lib, orc, cuda):
+    """509 x 511 x 517 float32 SDF of 32 spheres and 8 boxes, "<=" 0."""
+    f = synth.sdf_field((517, 511, 509), seed=11, device=cuda, dtype=torch.float32)
+    check_all_paths(lib, orc, f, "<=", 0.0)
