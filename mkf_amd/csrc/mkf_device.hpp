@@ -20,38 +20,44 @@ namespace mkf {
 
 constexpr int kWave = 64;
 
-// Threshold relation as one predicate (src/binMapCUDA.cuh:141-145): with
-// d = 1 - (f < t) + (f > t) the bit is (m >> d) & 1, so an unordered compare
-// (NaN) selects the EQ bit. Each mask value is a single IEEE compare.
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// Threshold relation of a whole wave as a 64-bit lane mask
+// (src/binMapCUDA.cuh:141-145): with d = 1 - (f < t) + (f > t) the bit is
+// (m >> d) & 1, so an unordered compare (NaN) selects the EQ bit. Every mask
+// is built from the ballots of the two ORDERED compares f < t and f > t:
+// hipcc lowers a ballot of (f < t || f > t) to v_cmp_neq, which is true for
+// NaN, so composite predicates are never balloted directly. Lanes must all
+// be active (inactive lanes would read as EQ).
 template <int M, typename T>
-__device__ __forceinline__ bool relBit(T f, T t) {
+__device__ __forceinline__ uint64_t relMask(T f, T t) {
   if constexpr (M == 0) {
-    return false;
+    return 0ull;
   } else if constexpr (M == 1) {
-    return f < t;
+    return ballot(f < t);
   } else if constexpr (M == 2) {
-    return !(f < t) && !(f > t);
+    return ~(ballot(f < t) | ballot(f > t));
   } else if constexpr (M == 3) {
-    return !(f > t);
+    return ~ballot(f > t);
   } else if constexpr (M == 4) {
-    return f > t;
+    return ballot(f > t);
   } else if constexpr (M == 5) {
-    return (f < t) || (f > t);
+    return ballot(f < t) | ballot(f > t);
   } else if constexpr (M == 6) {
-    return !(f < t);
+    return ~ballot(f < t);
   } else {
-    return true;
+    return ~0ull;
   }
 }
 
-// Same relation with the mask held at run time (multi-level paths).
+// The same with the relation mask known only at run time.
 template <typename T>
-__device__ __forceinline__ bool relBitDyn(T f, T t, uint32_t m) {
-  const uint32_t d = 1u - (uint32_t)(f < t) + (uint32_t)(f > t);
-  return (m >> d) & 1u;
+__device__ __forceinline__ uint64_t relMaskDyn(T f, T t, uint32_t m) {
+  const uint64_t lt = ballot(f < t);
+  const uint64_t gt = ballot(f > t);
+  const uint64_t eq = ~(lt | gt);
+  return ((m & 1u) ? lt : 0ull) | ((m & 2u) ? eq : 0ull) | ((m & 4u) ? gt : 0ull);
 }
-
-__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // Lane l reads bit l of a wave-uniform 64-bit mask (one v_cndmask).
 __device__ __forceinline__ bool laneBit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
@@ -102,9 +108,10 @@ struct FieldRows {
   }
 
   // Returns the lane's pair value; R receives the PBM mask of the segment.
+  // The halo value is wave-uniform, so any lane of its mask holds its bit.
   __device__ __forceinline__ uint32_t pair(const Raw& r, uint64_t& R) const {
-    R = ballot(relBit<M>(r.v, thr)) & tail;
-    const uint64_t hb = (halo && relBit<M>(r.h, thr)) ? (1ull << 63) : 0ull;
+    R = relMask<M>(r.v, thr) & tail;
+    const uint64_t hb = halo ? ((relMask<M>(r.h, thr) & 1ull) << 63) : 0ull;
     return pairBits(R, (R >> 1) | hb);
   }
 };

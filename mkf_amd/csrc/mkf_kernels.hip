@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const T* __restrict__ f
     const long long row = row0 + u;
     if (row < nRows) {
       const long long y = row % Y, z = row / Y;
-      const uint64_t R = ballot(relBit<M>(v[u], thr)) & tail;
+      const uint64_t R = relMask<M>(v[u], thr) & tail;
       const size_t wi = (size_t)y * rowWS + (size_t)z * planeWS + 2 * seg;
       if (lane < nw) pW[wi + lane] = lane ? (uint32_t)(R >> 32) : (uint32_t)R;
     }
