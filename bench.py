@@ -91,9 +91,16 @@ def main():
         if rc != 0:
             raise RuntimeError(f"mkfFusedBPFD failed: {M.last_error()}")
 
-    for _ in range(args.warmup):
+    # Warm-up: at least W launches and at least 0.5 s of launches, so the
+    # GPU has left its idle clocks before the timed region.
+    t_w = time.perf_counter()
+    n_w = 0
+    while n_w < args.warmup or time.perf_counter() - t_w < 0.5:
         hist.zero_()
         launch()
+        n_w += 1
+        if n_w % 16 == 0:
+            torch.cuda.synchronize()
     torch.cuda.synchronize()
 
     K = args.steps

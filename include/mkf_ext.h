@@ -22,9 +22,9 @@ extern "C" {
 #define MKF_ERR_BUDGET (-5)      /* request exceeds the memory budget */
 
 /* Work decomposition of the histogram kernels; NULL selects the built-in
- * choice for the field's shape. Each wave walks rowPairs row pairs of
- * planePairs plane pairs (4, 8 or 16) over a 64-voxel x-segment; a workgroup
- * stacks `waves` (4 or 8) such waves along z. */
+ * choice for the field's shape. A workgroup of `waves` (1..16) waves sits
+ * side by side along x, one 64-voxel segment per wave, holds rowPairs
+ * (2, 4 or 8) row pairs and marches over planePairs (>= 1) plane pairs. */
 typedef struct {
   int32_t planePairs;
   int32_t waves;

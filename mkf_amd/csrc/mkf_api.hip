@@ -106,8 +106,8 @@ Tiling toTiling(const MkfTiling* pT, int X, int Y, int Z, int elemBytes) {
 }
 
 int checkTiling(const Tiling& tl) {
-  if (tl.TP != 8 || tl.ZW != 8 || tl.Ty < 1)
-    return fail(MKF_ERR_ARG, "unsupported tiling TP=%d ZW=%d Ty=%d", tl.TP, tl.ZW, tl.Ty);
+  if (!tilingSupported(tl))
+    return fail(MKF_ERR_ARG, "unsupported tiling planePairs=%d waves=%d rowPairs=%d", tl.TP, tl.ZW, tl.Ty);
   return MKF_OK;
 }
 

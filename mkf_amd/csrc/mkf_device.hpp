@@ -9,6 +9,7 @@
 //  * E = R >> 1 with the first voxel of the next segment shifted into bit 63
 //    holds bit x + 1 for lane x, so lane l's pair value (bit x, bit x+1) is
 //    read straight out of the two SGPR masks by v_cndmask (inverse ballot).
+//    That one bit comes from the neighbouring wave through LDS.
 //  * 2x2x2 cell code (src/bpfdCUDA.cuh:45-61, src/mkfACC.c:78-91):
 //      bit i = P(x + (i & 1), y + ((i >> 1) & 1), z + (i >> 2)).
 #pragma once
@@ -83,124 +84,143 @@ __device__ __forceinline__ uint32_t pairBits(uint64_t R, uint64_t E) {
   return (laneBit(R) ? 1u : 0u) | (laneBit(E) ? 2u : 0u);
 }
 
-// Rows of a scalar field thresholded on the fly (fused path).
+// Rows of a scalar field thresholded on the fly (fused path). Addresses are
+// a wave-uniform row pointer (scalar registers) plus a 32-bit lane offset,
+// so loads take the global_load saddr form. The wave's row mask R comes from
+// the compare ballots; the bit right of the segment (the next segment's
+// first voxel) is read by the wave itself only when that segment belongs to
+// another workgroup, else it is exchanged through LDS by the walker.
 template <typename T, int M>
 struct FieldRows {
-  const T* f;
+  const T* fw;       // field + xb, xb = min(x0, X - 1) (uniform)
   T thr;
   long long sy, sz;  // element strides of y and z (x stride is 1)
-  int xc;            // this lane's x, clamped into the row
-  int haloX;         // x of the next segment's first voxel, clamped into the row
-  bool halo;         // whether that voxel exists
+  uint32_t lx;       // lane offset from xb, clamped into the row
+  int hxo;           // offset from xb of the next segment's first voxel, clamped
   uint64_t tail;     // voxels of this segment that exist
 
   struct Raw {
-    T v, h;
+    T v;
   };
 
-  // Every address is inside the row, so loads need no branch.
-  __device__ __forceinline__ Raw load(int y, int z) const {
-    const long long b = (long long)y * sy + (long long)z * sz;
-    Raw r;
-    r.v = f[b + xc];
-    r.h = uniformLoad(f + b + haloX);
-    return r;
+  // Uniform row pointer + zero-extended 32-bit lane offset: the saddr form.
+  __device__ __forceinline__ Raw load(long long off) const {
+    const T* row = fw + off;
+    return Raw{row[lx]};
   }
-
-  // Returns the lane's pair value; R receives the PBM mask of the segment.
-  // The halo value is wave-uniform, so any lane of its mask holds its bit.
-  __device__ __forceinline__ uint32_t pair(const Raw& r, uint64_t& R) const {
-    R = relMask<M>(r.v, thr) & tail;
-    const uint64_t hb = halo ? ((relMask<M>(r.h, thr) & 1ull) << 63) : 0ull;
-    return pairBits(R, (R >> 1) | hb);
+  __device__ __forceinline__ uint64_t mask(const Raw& r) const { return relMask<M>(r.v, thr) & tail; }
+  // Bit of the voxel right of the segment, read from memory (wave-uniform).
+  __device__ __forceinline__ uint32_t haloBit(long long off) const {
+    return (uint32_t)(relMask<M>(uniformLoad(fw + (off + hxo)), thr) & 1ull);
   }
 };
 
-// Rows of an existing packed binary map (BPFD-only path).
+// Rows of an existing packed binary map (BPFD-only path): the segment's two
+// words are read by the scalar unit. Offsets are in words (sy = rowWS,
+// sz = planeWS).
 struct PbmRows {
-  const uint32_t* w;
-  uint32_t rowWS, planeWS;
-  uint32_t wi0;   // first word of the segment (2 * seg)
-  uint32_t o1;    // offset of the segment's second word, 0 if outside the row
-  uint32_t o2;    // offset of the next segment's first word, 0 if outside
-  bool halo;      // next segment's first bit exists
-  uint64_t tail;  // bits of this segment inside the row
+  const uint32_t* ws;  // PBM + first word of the segment, clamped into the row
+  long long sy, sz;
+  uint32_t o1;         // offset of the segment's second word, 0 if outside the row
+  uint32_t hw;         // offset of the word holding the next segment's first bit
+  uint64_t tail;       // bits of this segment inside the row
 
   struct Raw {
-    uint32_t w0, w1, w2;
+    uint32_t w0, w1;
   };
 
-  // Offsets outside the row are replaced by 0 and masked off after the load.
-  __device__ __forceinline__ Raw load(int y, int z) const {
-    const uint32_t* p = w + (size_t)y * rowWS + (size_t)z * planeWS + wi0;
-    Raw r;
-    r.w0 = uniformLoad(p);
-    r.w1 = uniformLoad(p + o1);
-    r.w2 = uniformLoad(p + o2);
-    return r;
+  __device__ __forceinline__ Raw load(long long off) const {
+    const uint32_t* p = ws + off;
+    return Raw{uniformLoad(p), uniformLoad(p + o1)};
   }
-
-  __device__ __forceinline__ uint32_t pair(const Raw& r, uint64_t& R) const {
-    R = ((((uint64_t)r.w1) << 32) | r.w0) & tail;
-    const uint64_t hb = halo ? (((uint64_t)(r.w2 & 1u)) << 63) : 0ull;
-    return pairBits(R, (R >> 1) | hb);
+  __device__ __forceinline__ uint64_t mask(const Raw& r) const {
+    return ((((uint64_t)r.w1) << 32) | r.w0) & tail;
   }
+  __device__ __forceinline__ uint32_t haloBit(long long off) const { return uniformLoad(ws + (off + hw)) & 1u; }
 };
 
-// Per-wave tally: all-zero and all-one cells are counted from ballots in
-// scalar registers; only mixed cells touch the wave's LDS sub-histogram.
+// Per-wave tally: all-one cells and mixed cells are counted from ballots in
+// scalar registers; only mixed cells touch the wave's LDS sub-histogram. The
+// all-zero count is what remains of the wave's cell count.
 struct Tally {
-  uint32_t c0 = 0, c255 = 0;
+  uint32_t c255 = 0, cmix = 0;
 
-  __device__ __forceinline__ void add(uint32_t code, bool laneCell, uint64_t vmask, uint32_t* hist) {
+  __device__ __forceinline__ void add(uint32_t code, uint64_t vmask, uint32_t* hist) {
     c255 += __builtin_popcountll(ballot(code == 255u) & vmask);
-    c0 += __builtin_popcountll(ballot(code == 0u) & vmask);
-    if (laneCell && (code - 1u) < 254u) {
-      __hip_atomic_fetch_add(hist + code, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    }
+    const uint64_t mixed = ballot((code - 1u) < 254u) & vmask;
+    cmix += __builtin_popcountll(mixed);
+    if (laneBit(mixed)) __hip_atomic_fetch_add(hist + code, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   }
 };
 
-// Walks row pairs y0 .. y0+nY-1 of plane pairs z0 .. z0+nT-1 for one
-// x-segment: each step loads row y of planes z0..z0+nT (nT + 1 loads, the
-// next step's loads are issued before this step's cells are counted), and
-// counts the cells between rows y-1 and y.
-template <int TP, class Src>
-__device__ __forceinline__ void walkSegment(const Src& src, int y0, int nY, int z0, int nT, bool laneCell,
-                                            uint64_t vmask, uint32_t* hist, Tally& tally) {
-  // Planes beyond z0 + nT re-read the last plane (always in bounds); their
-  // cells are never counted.
-  int zt[TP + 1];
+// How a wave obtains the bit right of its segment.
+enum HaloFrom : int { kHaloNone = 0, kHaloLds = 1, kHaloMem = 2 };
+
+// Row-stack walker. The workgroup's waves sit side by side along x; each
+// holds rows y0 .. y0+nY (nY <= TY) and marches z from z0 to z0+nZ, so one
+// z-step of the workgroup reads nY+1 consecutive rows of one plane. At every
+// step each wave publishes the low word of its row masks in LDS (double
+// buffered by z parity), one barrier later it reads its right neighbour's
+// word for the bit x+1 of its last lane, forms lane l's pair value
+// (bit x, bit x+1) and the cell codes between this plane and the previous:
+//   hc[r] = q(y0+r) | q(y0+r+1) << 2;  code = hc_prev[r] | hc[r] << 4.
+// Every wave of the workgroup runs the same number of steps (barriers).
+template <int TY, class Src>
+__device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1][16], int w, HaloFrom halo,
+                                          int y0, int nY, int z0, int nZ, uint64_t vmask, uint32_t* hist,
+                                          Tally& tally) {
+  const int lane = threadIdx.x & (kWave - 1);
+  // The neighbour slot read when the halo comes from LDS (clamped so the
+  // read stays inside the array for the workgroup's last wave).
+  const int nb = min(w + 1, 15);
+  const uint32_t ldsKeep = (halo == kHaloLds) ? 1u : 0u;
+  long long yo[TY + 1];
 #pragma unroll
-  for (int t = 0; t <= TP; ++t) zt[t] = z0 + min(t, nT);
-  typename Src::Raw cur[TP + 1], nxt[TP + 1];
+  for (int r = 0; r <= TY; ++r) yo[r] = (long long)(y0 + min(r, nY)) * src.sy;
+  typename Src::Raw cur[TY + 1], nxt[TY + 1];
+  {
+    const long long zo = (long long)z0 * src.sz;
 #pragma unroll
-  for (int t = 0; t <= TP; ++t) cur[t] = src.load(y0, zt[t]);
-  uint32_t qprev[TP + 1];
-  for (int y = y0; y <= y0 + nY; ++y) {
-    const int yn = min(y + 1, y0 + nY);
+    for (int r = 0; r <= TY; ++r) cur[r] = src.load(yo[r] + zo);
+  }
+  uint32_t hp[TY];
+  for (int z = z0; z <= z0 + nZ; ++z) {
+    const long long zon = (long long)min(z + 1, z0 + nZ) * src.sz;
 #pragma unroll
-    for (int t = 0; t <= TP; ++t) nxt[t] = src.load(yn, zt[t]);
-    uint32_t q[TP + 1];
+    for (int r = 0; r <= TY; ++r) nxt[r] = src.load(yo[r] + zon);
+    const int par = z & 1;
+    uint64_t R[TY + 1];
 #pragma unroll
-    for (int t = 0; t <= TP; ++t) {
-      uint64_t R;
-      q[t] = src.pair(cur[t], R);
+    for (int r = 0; r <= TY; ++r) {
+      R[r] = src.mask(cur[r]);
+      if (lane == 0) xch[par][r][w] = (uint32_t)R[r];
     }
-    if (y > y0) {
-      uint32_t h[TP + 1];
+    __syncthreads();
+    uint32_t hb[TY + 1];
+    if (halo == kHaloMem) {
+      const long long zo = (long long)z * src.sz;
 #pragma unroll
-      for (int t = 0; t <= TP; ++t) h[t] = qprev[t] | (q[t] << 2);
+      for (int r = 0; r <= TY; ++r) hb[r] = src.haloBit(yo[r] + zo);
+    } else {
 #pragma unroll
-      for (int t = 0; t < TP; ++t) {
-        if (t < nT) tally.add(h[t] | (h[t + 1] << 4), laneCell, vmask, hist);
+      for (int r = 0; r <= TY; ++r) hb[r] = __builtin_amdgcn_readfirstlane(xch[par][r][nb]) & ldsKeep;
+    }
+    uint32_t q[TY + 1];
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) q[r] = pairBits(R[r], (R[r] >> 1) | ((uint64_t)hb[r] << 63));
+    uint32_t hc[TY];
+#pragma unroll
+    for (int r = 0; r < TY; ++r) hc[r] = q[r] | (q[r + 1] << 2);
+    if (z > z0) {
+#pragma unroll
+      for (int r = 0; r < TY; ++r) {
+        if (r < nY) tally.add(hp[r] | (hc[r] << 4), vmask, hist);
       }
     }
 #pragma unroll
-    for (int t = 0; t <= TP; ++t) {
-      qprev[t] = q[t];
-      cur[t] = nxt[t];
-    }
+    for (int r = 0; r < TY; ++r) hp[r] = hc[r];
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) cur[r] = nxt[r];
   }
 }
 

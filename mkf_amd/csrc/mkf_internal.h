@@ -17,16 +17,17 @@ struct FieldView {
   long long sy, sz;
 };
 
-// Work decomposition of the histogram kernels: each wave walks Ty row pairs
-// of TP plane pairs over one 64-wide x-segment; a workgroup holds ZW waves
-// stacked in z.
+// Work decomposition of the histogram kernels: a workgroup of up to ZW waves
+// side by side along x (one 64-voxel segment each) holds Ty row pairs and
+// marches over TP plane pairs.
 struct Tiling {
-  int TP;  // plane pairs per wave (4, 8 or 16)
-  int ZW;  // waves per workgroup (4 or 8)
-  int Ty;  // row pairs per wave
+  int TP;  // plane pairs per tile (>= 1)
+  int ZW;  // waves per workgroup along x (1..16)
+  int Ty;  // row pairs per wave (2, 4 or 8)
 };
 
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes);
+bool tilingSupported(const Tiling& tl);
 
 // Thresholds fv into the PBM pW (rowWS / planeWS word strides).
 hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint32_t rowWS, uint32_t planeWS,

@@ -53,110 +53,130 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const T* __restrict__ f
 }
 
 // ---------------------------------------------------------------------------
-// Histogram kernels. Tiles are (x-segment, y-range, z-group); a block of ZW
-// waves takes ZW consecutive z-ranges of TP plane pairs so the plane shared by
-// neighbouring waves is fetched from HBM once and hit in cache by the other.
-// Block b runs tile (b & 7) * chunk + (b >> 3): consecutive tiles land on the
-// same XCD (blocks are dealt to the 8 XCDs round-robin), so the halo row a
-// tile shares with the next y-range stays in that XCD's L2.
+// Histogram kernels. A tile is (x-tile, y-range, z-range): a workgroup of
+// `waves` waves side by side along x (one 64-voxel segment each, so a
+// workgroup reads whole rows), TY row pairs and up to TZ plane pairs. Block
+// b runs tile (b & 7) * chunk + (b >> 3): blocks are dealt to the 8 XCDs
+// round-robin, so consecutive tiles -- which share their halo rows -- run on
+// the same XCD at about the same time and the shared row is an L2 hit.
 // ---------------------------------------------------------------------------
 struct TileGrid {
-  int nSegX, nYR, nZG, nTiles, chunk, Ty;
+  int nXT, nYR, nZR, nTiles, chunk;
+  int TZ, waves, nSegX;
 };
 
-template <int ZW>
-__device__ __forceinline__ void flushHistogram(uint32_t (*hist)[256], const Tally& tally,
+__device__ __forceinline__ bool tileOf(const TileGrid& g, int& xt, int& yr, int& zr) {
+  const int b = blockIdx.x;
+  const int L = (b & 7) * g.chunk + (b >> 3);
+  if (L >= g.nTiles) return false;
+  xt = L % g.nXT;
+  const int r = L / g.nXT;
+  yr = r % g.nYR;
+  zr = r / g.nYR;
+  return true;
+}
+
+// Adds the per-wave sub-histograms of the workgroup to the result with one
+// 64-bit atomic per nonzero bin. Bin 0 is each wave's cell count minus its
+// all-one and mixed cells.
+__device__ __forceinline__ void flushHistogram(uint32_t (*hist)[256], const Tally& tally, uint32_t cells,
                                                unsigned long long* __restrict__ bpfd) {
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
   if (lane == 0) {
-    hist[w][0] += tally.c0;
+    hist[w][0] += cells - tally.c255 - tally.cmix;
     hist[w][255] += tally.c255;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 256; i += ZW * kWave) {
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
     unsigned long long s = 0;
-#pragma unroll
-    for (int k = 0; k < ZW; ++k) s += hist[k][i];
+    for (int k = 0; k < nw; ++k) s += hist[k][i];
     if (s) atomicAdd(bpfd + i, s);
   }
 }
 
-__device__ __forceinline__ bool tileOf(const TileGrid& g, int& seg, int& yr, int& zg) {
-  const int b = blockIdx.x;
-  const int L = (b & 7) * g.chunk + (b >> 3);
-  if (L >= g.nTiles) return false;
-  seg = L % g.nSegX;
-  const int r = L / g.nSegX;
-  yr = r % g.nYR;
-  zg = r / g.nYR;
-  return true;
+// Geometry of one wave inside its tile.
+struct WaveGeom {
+  int w, seg, x0, nCells, y0, nY, z0, nZ;
+  HaloFrom halo;
+  uint64_t vmask;
+};
+
+__device__ __forceinline__ WaveGeom waveGeom(const TileGrid& g, int xt, int yr, int zr, int TY, int X, int Y,
+                                             int Z) {
+  WaveGeom q;
+  q.w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  q.seg = xt * g.waves + q.w;
+  q.x0 = q.seg * kWave;
+  q.nCells = max(0, min(kWave, X - 1 - q.x0));
+  q.vmask = lowBits(q.nCells);
+  q.y0 = yr * TY;
+  q.nY = min(TY, Y - 1 - q.y0);
+  q.z0 = zr * g.TZ;
+  q.nZ = min(g.TZ, Z - 1 - q.z0);
+  // The right neighbour's segment holds the bit x0+64; it is in this
+  // workgroup unless this wave is the last of its tile.
+  if (q.x0 + kWave >= X) {
+    q.halo = kHaloNone;
+  } else if (q.w + 1 < g.waves) {
+    q.halo = kHaloLds;
+  } else {
+    q.halo = kHaloMem;
+  }
+  return q;
 }
 
-template <typename T, int M, int TP, int ZW>
-__global__ __launch_bounds__(ZW* kWave) void kFusedBPFD(const T* __restrict__ f, T thr, int X, int Y, int Z,
-                                                          long long sy, long long sz, TileGrid g,
-                                                          unsigned long long* __restrict__ bpfd) {
-  __shared__ uint32_t hist[ZW][256];
-  int seg, yr, zg;
-  if (!tileOf(g, seg, yr, zg)) return;
+template <typename T, int M, int TY>
+__global__ __launch_bounds__(1024) void kFusedBPFD(const T* __restrict__ f, T thr, int X, int Y, int Z,
+                                                    long long sy, long long sz, TileGrid g,
+                                                    unsigned long long* __restrict__ bpfd) {
+  __shared__ uint32_t hist[16][256];
+  __shared__ uint32_t xch[2][TY + 1][16];
+  int xt, yr, zr;
+  if (!tileOf(g, xt, yr, zr)) return;
   const int lane = threadIdx.x & (kWave - 1);
-  const int w = threadIdx.x >> 6;
-  for (int i = lane; i < 256; i += kWave) hist[w][i] = 0;
+  const WaveGeom q = waveGeom(g, xt, yr, zr, TY, X, Y, Z);
+  for (int i = lane; i < 256; i += kWave) hist[q.w][i] = 0;
 
+  const int xb = min(q.x0, X - 1);
+  FieldRows<T, M> src;
+  src.fw = f + xb;
+  src.thr = thr;
+  src.sy = sy;
+  src.sz = sz;
+  src.lx = (uint32_t)(min(q.x0 + lane, X - 1) - xb);
+  src.hxo = min(q.x0 + kWave, X - 1) - xb;
+  src.tail = q.x0 < X ? lowBits(X - q.x0) : 0ull;
   Tally tally;
-  const int z0 = (zg * ZW + w) * TP;
-  const int nT = min(TP, Z - 1 - z0);
-  const int y0 = yr * g.Ty;
-  const int nY = min(g.Ty, Y - 1 - y0);
-  const int x0 = seg * kWave;
-  const int nCells = min(kWave, X - 1 - x0);
-  if (nT > 0 && nY > 0 && nCells > 0) {
-    FieldRows<T, M> src;
-    src.f = f;
-    src.thr = thr;
-    src.sy = sy;
-    src.sz = sz;
-    src.xc = min(x0 + lane, X - 1);
-    src.haloX = min(x0 + kWave, X - 1);
-    src.halo = x0 + kWave < X;
-    src.tail = lowBits(X - x0);
-    walkSegment<TP>(src, y0, nY, z0, nT, lane < nCells, lowBits(nCells), hist[w], tally);
-  }
-  flushHistogram<ZW>(hist, tally, bpfd);
+  walkStack<TY>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, q.vmask, hist[q.w], tally);
+  flushHistogram(hist, tally, (uint32_t)q.nCells * q.nY * q.nZ, bpfd);
 }
 
-template <int TP, int ZW>
-__global__ __launch_bounds__(ZW* kWave) void kBPFD(const uint32_t* __restrict__ pW, int X, int Y, int Z,
-                                                     uint32_t rowWS, uint32_t planeWS, TileGrid g,
-                                                     unsigned long long* __restrict__ bpfd) {
-  __shared__ uint32_t hist[ZW][256];
-  int seg, yr, zg;
-  if (!tileOf(g, seg, yr, zg)) return;
+template <int TY>
+__global__ __launch_bounds__(1024) void kBPFD(const uint32_t* __restrict__ pW, int X, int Y, int Z,
+                                              uint32_t rowWS, uint32_t planeWS, TileGrid g,
+                                              unsigned long long* __restrict__ bpfd) {
+  __shared__ uint32_t hist[16][256];
+  __shared__ uint32_t xch[2][TY + 1][16];
+  int xt, yr, zr;
+  if (!tileOf(g, xt, yr, zr)) return;
   const int lane = threadIdx.x & (kWave - 1);
-  const int w = threadIdx.x >> 6;
-  for (int i = lane; i < 256; i += kWave) hist[w][i] = 0;
+  const WaveGeom q = waveGeom(g, xt, yr, zr, TY, X, Y, Z);
+  for (int i = lane; i < 256; i += kWave) hist[q.w][i] = 0;
 
+  PbmRows src;
+  const uint32_t lastWord = rowWS - 1u;
+  const uint32_t wi0 = min(2u * q.seg, lastWord);
+  src.ws = pW + wi0;
+  src.sy = rowWS;
+  src.sz = planeWS;
+  src.o1 = (2u * q.seg + 1u <= lastWord) ? 1u : 0u;
+  src.hw = min(2u * q.seg + 2u, lastWord) - wi0;
+  src.tail = q.x0 < X ? lowBits(X - q.x0) : 0ull;
   Tally tally;
-  const int z0 = (zg * ZW + w) * TP;
-  const int nT = min(TP, Z - 1 - z0);
-  const int y0 = yr * g.Ty;
-  const int nY = min(g.Ty, Y - 1 - y0);
-  const int x0 = seg * kWave;
-  const int nCells = min(kWave, X - 1 - x0);
-  if (nT > 0 && nY > 0 && nCells > 0) {
-    PbmRows src;
-    src.w = pW;
-    src.rowWS = rowWS;
-    src.planeWS = planeWS;
-    src.wi0 = 2u * seg;
-    src.o1 = x0 + 32 < X ? 1u : 0u;
-    src.o2 = x0 + kWave < X ? 2u : 0u;
-    src.halo = x0 + kWave < X;
-    src.tail = lowBits(X - x0);
-    walkSegment<TP>(src, y0, nY, z0, nT, lane < nCells, lowBits(nCells), hist[w], tally);
-  }
-  flushHistogram<ZW>(hist, tally, bpfd);
+  walkStack<TY>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, q.vmask, hist[q.w], tally);
+  flushHistogram(hist, tally, (uint32_t)q.nCells * q.nY * q.nZ, bpfd);
 }
 
 // ---------------------------------------------------------------------------
@@ -167,26 +187,35 @@ static inline long long ceilDiv(long long a, long long b) { return (a + b - 1) /
 static TileGrid makeGrid(int X, int Y, int Z, const Tiling& tl) {
   TileGrid g;
   g.nSegX = (int)ceilDiv(X - 1, kWave);
-  g.Ty = tl.Ty;
+  g.waves = (int)std::min<long long>(tl.ZW, g.nSegX);
+  g.nXT = (int)ceilDiv(g.nSegX, g.waves);
+  g.TZ = tl.TP;
   g.nYR = (int)ceilDiv(Y - 1, tl.Ty);
-  const long long nZR = ceilDiv(Z - 1, tl.TP);
-  g.nZG = (int)ceilDiv(nZR, tl.ZW);
-  g.nTiles = g.nSegX * g.nYR * g.nZG;
+  g.nZR = (int)ceilDiv(Z - 1, tl.TP);
+  g.nTiles = g.nXT * g.nYR * g.nZR;
   g.chunk = (int)ceilDiv(g.nTiles, 8);
   return g;
 }
 
+bool tilingSupported(const Tiling& tl) {
+  return (tl.Ty == 2 || tl.Ty == 4 || tl.Ty == 8) && tl.TP >= 1 && tl.ZW >= 1 && tl.ZW <= 16;
+}
+
+// Workgroups as wide as the row (up to 16 waves) and the longest z-range
+// that still gives about 32 waves per CU. Row pairs per wave: 4 for f32,
+// 8 for f64 (measured on MI355X at 1024^3: f32 0.82 ms with 4 vs 0.90 ms
+// with 8, whose scalar registers cap a 16-wave workgroup at one per CU;
+// f64 1.39 ms with 8 vs 1.49 ms with 4).
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
-  (void)elemBytes;
   Tiling tl;
-  tl.TP = 8;
-  tl.ZW = 8;
-  tl.Ty = 64;
-  // Shrink the y-range until there are enough tiles to fill 256 CUs.
-  while (tl.Ty > 4) {
+  tl.Ty = elemBytes >= 8 ? 8 : 4;
+  tl.ZW = 16;
+  tl.TP = elemBytes >= 8 ? 128 : 256;
+  const long long targetWaves = 256LL * 32;
+  while (tl.TP > 8) {
     const TileGrid g = makeGrid(X, Y, Z, tl);
-    if (g.nTiles >= 2048) break;
-    tl.Ty >>= 1;
+    if ((long long)g.nTiles * g.waves >= targetWaves) break;
+    tl.TP >>= 1;
   }
   return tl;
 }
@@ -228,35 +257,47 @@ hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint
   return hipErrorInvalidValue;
 }
 
-template <typename T, int M, int TP, int ZW>
-static hipError_t launchFusedT(const FieldView& fv, T thr, unsigned long long* bpfd, const Tiling& tl,
+template <typename T, int M, int TY>
+static hipError_t launchFusedT(const FieldView& fv, T thr, unsigned long long* bpfd, const TileGrid& g,
                                hipStream_t s) {
-  const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, tl);
-  if (g.nTiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL((kFusedBPFD<T, M, TP, ZW>), dim3((unsigned)g.chunk * 8u), dim3(ZW * kWave), 0, s,
+  hipLaunchKernelGGL((kFusedBPFD<T, M, TY>), dim3((unsigned)g.chunk * 8u), dim3(g.waves * kWave), 0, s,
                      (const T*)fv.p, thr, fv.X, fv.Y, fv.Z, fv.sy, fv.sz, g, bpfd);
   return hipGetLastError();
+}
+
+template <typename T, int TY>
+static hipError_t dispatchFusedM(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd,
+                                 const TileGrid& g, hipStream_t s) {
+  switch (m & 7u) {
+    case 0: return launchFusedT<T, 0, TY>(fv, thr, bpfd, g, s);
+    case 1: return launchFusedT<T, 1, TY>(fv, thr, bpfd, g, s);
+    case 2: return launchFusedT<T, 2, TY>(fv, thr, bpfd, g, s);
+    case 3: return launchFusedT<T, 3, TY>(fv, thr, bpfd, g, s);
+    case 4: return launchFusedT<T, 4, TY>(fv, thr, bpfd, g, s);
+    case 5: return launchFusedT<T, 5, TY>(fv, thr, bpfd, g, s);
+    case 6: return launchFusedT<T, 6, TY>(fv, thr, bpfd, g, s);
+    default: return launchFusedT<T, 7, TY>(fv, thr, bpfd, g, s);
+  }
 }
 
 template <typename T>
 static hipError_t dispatchFused(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
                                 hipStream_t s) {
-  switch (m & 7u) {
-    case 0: return launchFusedT<T, 0, 8, 8>(fv, thr, bpfd, tl, s);
-    case 1: return launchFusedT<T, 1, 8, 8>(fv, thr, bpfd, tl, s);
-    case 2: return launchFusedT<T, 2, 8, 8>(fv, thr, bpfd, tl, s);
-    case 3: return launchFusedT<T, 3, 8, 8>(fv, thr, bpfd, tl, s);
-    case 4: return launchFusedT<T, 4, 8, 8>(fv, thr, bpfd, tl, s);
-    case 5: return launchFusedT<T, 5, 8, 8>(fv, thr, bpfd, tl, s);
-    case 6: return launchFusedT<T, 6, 8, 8>(fv, thr, bpfd, tl, s);
-    default: return launchFusedT<T, 7, 8, 8>(fv, thr, bpfd, tl, s);
+  const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, tl);
+  if (g.nTiles <= 0) return hipSuccess;
+  switch (tl.Ty) {
+    case 2: return dispatchFusedM<T, 2>(fv, thr, m, bpfd, g, s);
+    case 8: return dispatchFusedM<T, 8>(fv, thr, m, bpfd, g, s);
+    default: return dispatchFusedM<T, 4>(fv, thr, m, bpfd, g, s);
   }
 }
 
 hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, const Tiling& tl,
                        hipStream_t s) {
   if (fv.X < 2 || fv.Y < 2 || fv.Z < 2) return hipSuccess;  // no cells
-  if (tl.TP != 8 || tl.ZW != 8 || tl.Ty < 1) return hipErrorInvalidValue;
+  if (!tilingSupported(tl)) return hipErrorInvalidValue;
+  // f32 fields compare against the threshold rounded to float
+  // (src/binMapCUDA.cuh:138-139).
   if (fv.enc == ENC_F32) return dispatchFused<float>(fv, (float)m.t[0], m.m, bpfd, tl, s);
   if (fv.enc == ENC_F64) return dispatchFused<double>(fv, m.t[0], m.m, bpfd, tl, s);
   return hipErrorInvalidValue;
@@ -265,10 +306,14 @@ hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long lo
 hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
                       unsigned long long* bpfd, const Tiling& tl, hipStream_t s) {
   if (X < 2 || Y < 2 || Z < 2) return hipSuccess;
-  if (tl.TP != 8 || tl.ZW != 8 || tl.Ty < 1) return hipErrorInvalidValue;
+  if (!tilingSupported(tl)) return hipErrorInvalidValue;
   const TileGrid g = makeGrid(X, Y, Z, tl);
-  hipLaunchKernelGGL((kBPFD<8, 8>), dim3((unsigned)g.chunk * 8u), dim3(8 * kWave), 0, s, pW, X, Y, Z, rowWS,
-                     planeWS, g, bpfd);
+  const dim3 grid((unsigned)g.chunk * 8u), block(g.waves * kWave);
+  switch (tl.Ty) {
+    case 2: hipLaunchKernelGGL((kBPFD<2>), grid, block, 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd); break;
+    case 8: hipLaunchKernelGGL((kBPFD<8>), grid, block, 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd); break;
+    default: hipLaunchKernelGGL((kBPFD<4>), grid, block, 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd); break;
+  }
   return hipGetLastError();
 }
 

@@ -80,12 +80,36 @@ def test_f32_threshold_rounding(lib, orc, cuda):
     assert np.all(words == 0)
 
 
-@pytest.mark.parametrize("ty", [1, 2, 5, 13, 64, 200])
-def test_fused_tilings(lib, orc, cuda, ty):
-    f = synth.grf_field((80, 90, 100), sigma=2.0, seed=3, device=cuda, dtype=torch.float32)
-    _, h_ref = oracle_all(orc, host_np(f), ">=", 0.0)
-    h, _ = dev_fused(lib, f, ">=", 0.0, tiling=(8, 8, ty))
+# (planePairs, waves, rowPairs): narrow workgroups split each row over
+# several tiles, so the bit right of a tile's last segment is read from
+# memory instead of the neighbour's LDS slot.
+TILINGS = [(1, 16, 4), (3, 16, 2), (8, 2, 4), (64, 3, 8), (5, 1, 2), (128, 16, 8), (7, 4, 4), (2, 5, 8)]
+
+
+@pytest.mark.parametrize("tiling", TILINGS)
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_fused_and_pbm_histogram_tilings(lib, orc, cuda, tiling, dtype):
+    print(f"tiling {tiling} {dtype}", flush=True)
+    f = synth.grf_field((37, 41, 300), sigma=2.0, seed=3, device=cuda, dtype=dtype)
+    w_ref, h_ref = oracle_all(orc, host_np(f), ">=", 0.0)
+    h, _ = dev_fused(lib, f, ">=", 0.0, tiling=tiling)
     np.testing.assert_array_equal(h, h_ref)
+    _, dw, org, _ = dev_pbm(lib, f, ">=", 0.0)
+    hb = torch.zeros(256, dtype=torch.int64, device=cuda)
+    t = M.MkfTiling(*tiling)
+    M.check(lib.mkfBPFDAsync(hb.data_ptr(), C.byref(org), dw.data_ptr(), C.byref(t), None))
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(hb.cpu().numpy().view(np.uint64), h_ref)
+
+
+def test_bad_tiling_is_rejected(lib, cuda):
+    f = torch.rand((8, 8, 64), device=cuda)
+    bm = M.bin_map(">=", 0.5)
+    h = torch.zeros(256, dtype=torch.int64, device=cuda)
+    fi = M.FieldInfo(f.data_ptr(), (64, 8, 8), M.ENC_F32)
+    for bad in [(0, 16, 4), (8, 17, 4), (8, 16, 3), (8, 0, 4)]:
+        t = M.MkfTiling(*bad)
+        assert lib.mkfFusedBPFD(None, h.data_ptr(), None, fi.ref, C.byref(bm), C.byref(t), None) == -1
 
 
 def test_fused_accumulates(lib, orc, cuda):
