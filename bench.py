@@ -46,7 +46,11 @@ def main():
     ap.add_argument("--threshold", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle check")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
-    ap.add_argument("--tiling", type=str, default="", help="planePairs,waves,rowPairs (default: library choice)")
+    ap.add_argument("--tiling", type=str, default="",
+                    help="planePairs,waves,rowPairs[,voxelsPerLane] (default: library choice)")
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="for counter runs: 3 fused launches, then 3 copies of a 1 GiB buffer (the calibration "
+                         "kernel for FETCH_SIZE / WRITE_SIZE); prints no JSON")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -90,6 +94,21 @@ def main():
         rc = lib.mkfFusedBPFD(None, hist.data_ptr(), None, fi.ref, C.byref(bm), C.byref(tiling), sptr)
         if rc != 0:
             raise RuntimeError(f"mkfFusedBPFD failed: {M.last_error()}")
+
+    if args.pmc_workload:
+        for _ in range(3):
+            launch()
+        torch.cuda.synchronize()
+        del field
+        torch.cuda.empty_cache()
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] counter workload done: 3 fused launches, 3 copies of 1 GiB")
+        return
 
     # Warm-up: at least W launches and at least 0.5 s of launches, so the
     # GPU has left its idle clocks before the timed region.

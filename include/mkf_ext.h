@@ -23,12 +23,15 @@ extern "C" {
 
 /* Work decomposition of the histogram kernels; NULL selects the built-in
  * choice for the field's shape. A workgroup of `waves` (1..16) waves sits
- * side by side along x, one 64-voxel segment per wave, holds rowPairs
- * (2, 4 or 8) row pairs and marches over planePairs (>= 1) plane pairs. */
+ * side by side along x, each lane holding voxelsPerLane consecutive voxels
+ * (1, or 4 for float / 2 for double fields; 0 = the library's choice; the
+ * PBM histogram always uses 1), holds rowPairs (2, 4 or 8) row pairs and
+ * marches over planePairs (>= 1) plane pairs. */
 typedef struct {
   int32_t planePairs;
   int32_t waves;
   int32_t rowPairs;
+  int32_t voxelsPerLane;
 } MkfTiling;
 
 /* Text of the last error raised by any entry point in this thread. */

@@ -64,7 +64,8 @@ class Context(C.Structure):
 
 
 class MkfTiling(C.Structure):
-    _fields_ = [("planePairs", C.c_int32), ("waves", C.c_int32), ("rowPairs", C.c_int32)]
+    _fields_ = [("planePairs", C.c_int32), ("waves", C.c_int32), ("rowPairs", C.c_int32),
+                ("voxelsPerLane", C.c_int32)]
 
 
 _P = C.c_void_p

@@ -101,13 +101,18 @@ int resolveField(const BMFieldInfo* pI, FieldView* fv, FieldDef def[3]) {
 }
 
 Tiling toTiling(const MkfTiling* pT, int X, int Y, int Z, int elemBytes) {
-  if (pT) return Tiling{pT->planePairs, pT->waves, pT->rowPairs};
+  if (pT) {
+    Tiling tl{pT->planePairs, pT->waves, pT->rowPairs, pT->voxelsPerLane};
+    if (tl.V == 0) tl.V = chooseTiling(X, Y, Z, elemBytes).V;
+    return tl;
+  }
   return chooseTiling(X, Y, Z, elemBytes);
 }
 
-int checkTiling(const Tiling& tl) {
-  if (!tilingSupported(tl))
-    return fail(MKF_ERR_ARG, "unsupported tiling planePairs=%d waves=%d rowPairs=%d", tl.TP, tl.ZW, tl.Ty);
+int checkTiling(const Tiling& tl, int elemBytes) {
+  if (!tilingSupported(tl, elemBytes))
+    return fail(MKF_ERR_ARG, "unsupported tiling planePairs=%d waves=%d rowPairs=%d voxelsPerLane=%d", tl.TP, tl.ZW,
+                tl.Ty, tl.V);
   return MKF_OK;
 }
 
@@ -119,7 +124,7 @@ const char* mkfLastError(void) { return gErr; }
 
 MkfTiling mkfDefaultTiling(const FieldDef def[3], int elemBytes) {
   const Tiling tl = chooseTiling(def[0], def[1], def[2], elemBytes);
-  return MkfTiling{tl.TP, tl.ZW, tl.Ty};
+  return MkfTiling{tl.TP, tl.ZW, tl.Ty, tl.V};
 }
 
 int mkfPBMAsync(BMPackWord* pW, BMOrg* pO, const BMFieldInfo* pI, const BinMapF64* pM, void* stream) {
@@ -138,8 +143,9 @@ int mkfPBMAsync(BMPackWord* pW, BMOrg* pO, const BMFieldInfo* pI, const BinMapF6
 int mkfBPFDAsync(size_t* pBPFD, const BMOrg* pO, const BMPackWord* pW, const MkfTiling* pT, void* stream) {
   if (!pBPFD || !pO || !pW) return fail(MKF_ERR_ARG, "NULL argument");
   const int X = (int)pO->rowElem, Y = (int)pO->rowPairs + 1, Z = (int)pO->planePairs + 1;
-  const Tiling tl = toTiling(pT, X, Y, Z, 4);
-  int rc = checkTiling(tl);
+  Tiling tl = toTiling(pT, X, Y, Z, 4);
+  tl.V = 1;  // one PBM bit per lane
+  int rc = checkTiling(tl, 4);
   if (rc) return rc;
   const hipError_t e = launchBPFD(pW, X, Y, Z, pO->rowWS, pO->planeWS, (unsigned long long*)pBPFD, tl,
                                   (hipStream_t)stream);
@@ -155,7 +161,7 @@ int mkfFusedBPFD(KernInfo* pK, size_t* pBPFD, BMOrg* pO, const BMFieldInfo* pI, 
   if (rc) return rc;
   if (pO) setBMO(pO, def, 0);
   const Tiling tl = toTiling(pT, fv.X, fv.Y, fv.Z, fv.enc & ENC_MASK_NUM);
-  rc = checkTiling(tl);
+  rc = checkTiling(tl, fv.enc & ENC_MASK_NUM);
   if (rc) return rc;
   const hipStream_t s = (hipStream_t)stream;
   if (pK) {

@@ -79,51 +79,115 @@ __device__ __forceinline__ T uniformLoad(const T* p) {
   return *(ConstPtr)((((uint64_t)hi) << 32) | lo);
 }
 
-// Pair value for lane l from the segment masks: bit x and bit x + 1.
-__device__ __forceinline__ uint32_t pairBits(uint64_t R, uint64_t E) {
-  return (laneBit(R) ? 1u : 0u) | (laneBit(E) ? 2u : 0u);
+// Byte-packed pair words. A lane holds V consecutive voxels (V = 1, 2 or 4);
+// byte k of its pair word P holds (bit of voxel k, bit of voxel k+1) in bits
+// 8k and 8k+1. Then, for rows r, r+1 and planes z-1, z, all V cell codes of a
+// lane come from two word operations:
+//   Hc = P(r) | P(r+1) << 2          (one plane, two rows: bits 0..3 per byte)
+//   C  = Hc(z-1) | Hc(z) << 4        (byte k = code of the lane's cell k)
+// which is the reference's code bit order (src/bpfdCUDA.cuh:45-61).
+template <int V>
+__device__ __forceinline__ uint32_t packPairs(const uint32_t (&b)[V], uint32_t right) {
+  uint32_t P = 0;
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    P |= b[k] << (8 * k);
+    if (k > 0) P |= b[k] << (8 * (k - 1) + 1);
+  }
+  return P | (right << (8 * (V - 1) + 1));
 }
 
-// Rows of a scalar field thresholded on the fly (fused path). Addresses are
-// a wave-uniform row pointer (scalar registers) plus a 32-bit lane offset,
-// so loads take the global_load saddr form. The wave's row mask R comes from
-// the compare ballots; the bit right of the segment (the next segment's
-// first voxel) is read by the wave itself only when that segment belongs to
-// another workgroup, else it is exchanged through LDS by the walker.
-template <typename T, int M>
+// Bit position of the neighbour-supplied bit in a pair word.
+template <int V>
+constexpr int kRightBit = 8 * (V - 1) + 1;
+
+template <typename T, int V>
+struct VecLoad;
+template <typename T>
+struct VecLoad<T, 1> {
+  typedef T type;
+};
+template <>
+struct VecLoad<float, 2> {
+  typedef float type __attribute__((ext_vector_type(2), aligned(4)));
+};
+template <>
+struct VecLoad<float, 4> {
+  typedef float type __attribute__((ext_vector_type(4), aligned(4)));
+};
+template <>
+struct VecLoad<double, 2> {
+  typedef double type __attribute__((ext_vector_type(2), aligned(8)));
+};
+
+// Rows of a scalar field thresholded on the fly (fused path). Lane l holds
+// voxels x0 + V*l + k. Addresses are a wave-uniform row pointer plus a lane
+// offset; a wave whose voxels all exist reads them with one vector load per
+// row (4 floats or 2 doubles = 16 bytes), a partial wave clamps each voxel.
+// The bit right of the wave's last voxel comes from the next wave through LDS
+// (walker), or from memory when that wave belongs to another workgroup.
+template <typename T, int M, int V>
 struct FieldRows {
   const T* fw;       // field + xb, xb = min(x0, X - 1) (uniform)
   T thr;
   long long sy, sz;  // element strides of y and z (x stride is 1)
-  uint32_t lx;       // lane offset from xb, clamped into the row
-  int hxo;           // offset from xb of the next segment's first voxel, clamped
-  uint64_t tail;     // voxels of this segment that exist
+  uint32_t lx;       // lane offset of voxel 0 from xb, clamped into the row
+  int last;          // X - 1 - xb: clamp for partial waves
+  bool full;         // every voxel of every lane exists
+  int hxo;           // offset from xb of the next wave's first voxel, clamped
+  uint32_t keep;     // pair-word bytes of cells that exist (per lane)
 
   struct Raw {
-    T v;
+    T v[V];
   };
 
-  // Uniform row pointer + zero-extended 32-bit lane offset: the saddr form.
   __device__ __forceinline__ Raw load(long long off) const {
     const T* row = fw + off;
-    return Raw{row[lx]};
+    Raw r;
+    if (V == 1 || full) {
+      const typename VecLoad<T, V>::type x = *(const typename VecLoad<T, V>::type*)(row + lx);
+      if constexpr (V == 1) {
+        r.v[0] = x;
+      } else {
+#pragma unroll
+        for (int k = 0; k < V; ++k) r.v[k] = x[k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < V; ++k) r.v[k] = row[min((int)lx + k, last)];
+    }
+    return r;
   }
-  __device__ __forceinline__ uint64_t mask(const Raw& r) const { return relMask<M>(r.v, thr) & tail; }
-  // Bit of the voxel right of the segment, read from memory (wave-uniform).
+  // Pair word without the neighbour bit of lane 63; R0 receives the wave
+  // mask of voxel 0 (the right neighbours of the lanes' last voxels). Bits of
+  // voxels that do not exist (clamped loads) only reach cells that do not
+  // exist, whose bytes `keep` clears.
+  __device__ __forceinline__ uint32_t pairs(const Raw& r, uint64_t& R0) const {
+    uint32_t b[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const uint64_t mk = relMask<M>(r.v[k], thr);
+      if (k == 0) R0 = mk;
+      b[k] = laneBit(mk) ? 1u : 0u;
+    }
+    const uint32_t right = laneBit(R0 >> 1) ? 1u : 0u;
+    return packPairs<V>(b, right) & keep;
+  }
+  // Bit of the voxel right of the wave, read from memory (wave-uniform).
   __device__ __forceinline__ uint32_t haloBit(long long off) const {
     return (uint32_t)(relMask<M>(uniformLoad(fw + (off + hxo)), thr) & 1ull);
   }
 };
 
-// Rows of an existing packed binary map (BPFD-only path): the segment's two
-// words are read by the scalar unit. Offsets are in words (sy = rowWS,
-// sz = planeWS).
+// Rows of an existing packed binary map (BPFD-only path, one bit per lane):
+// the wave's two words are read by the scalar unit. Offsets are in words
+// (sy = rowWS, sz = planeWS).
 struct PbmRows {
   const uint32_t* ws;  // PBM + first word of the segment, clamped into the row
   long long sy, sz;
   uint32_t o1;         // offset of the segment's second word, 0 if outside the row
   uint32_t hw;         // offset of the word holding the next segment's first bit
-  uint64_t tail;       // bits of this segment inside the row
+  uint32_t keep;       // pair-word bits of cells that exist (per lane)
 
   struct Raw {
     uint32_t w0, w1;
@@ -133,47 +197,46 @@ struct PbmRows {
     const uint32_t* p = ws + off;
     return Raw{uniformLoad(p), uniformLoad(p + o1)};
   }
-  __device__ __forceinline__ uint64_t mask(const Raw& r) const {
-    return ((((uint64_t)r.w1) << 32) | r.w0) & tail;
+  // Bits beyond the row (tail bits of the last word, or a clamped word) only
+  // reach cells that do not exist, whose bits `keep` clears.
+  __device__ __forceinline__ uint32_t pairs(const Raw& r, uint64_t& R0) const {
+    R0 = (((uint64_t)r.w1) << 32) | r.w0;
+    const uint32_t b[1] = {laneBit(R0) ? 1u : 0u};
+    return packPairs<1>(b, laneBit(R0 >> 1) ? 1u : 0u) & keep;
   }
   __device__ __forceinline__ uint32_t haloBit(long long off) const { return uniformLoad(ws + (off + hw)) & 1u; }
 };
 
-// Per-wave tally: all-one cells and mixed cells are counted from ballots in
-// scalar registers; only mixed cells touch the wave's LDS sub-histogram. The
-// all-zero count is what remains of the wave's cell count.
+// Per-wave tally. Cells that do not exist have code 0 (their pair bits are
+// cleared), so no per-cell validity test is needed. All-one cells are counted
+// from ballots in scalar registers; mixed cells go to the wave's LDS
+// sub-histogram; all-zero cells are what remains of the cell count.
 struct Tally {
-  uint32_t c255 = 0, cmix = 0;
+  uint32_t c255 = 0;
 
-  __device__ __forceinline__ void add(uint32_t code, uint64_t vmask, uint32_t* hist) {
-    c255 += __builtin_popcountll(ballot(code == 255u) & vmask);
-    const uint64_t mixed = ballot((code - 1u) < 254u) & vmask;
-    cmix += __builtin_popcountll(mixed);
-    if (laneBit(mixed)) __hip_atomic_fetch_add(hist + code, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  __device__ __forceinline__ void add(uint32_t code, uint32_t* hist) {
+    c255 += __builtin_popcountll(ballot(code == 255u));
+    if ((code - 1u) < 254u) __hip_atomic_fetch_add(hist + code, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   }
 };
 
-// How a wave obtains the bit right of its segment.
+// How a wave obtains the bit right of its last voxel.
 enum HaloFrom : int { kHaloNone = 0, kHaloLds = 1, kHaloMem = 2 };
 
 // Row-stack walker. The workgroup's waves sit side by side along x; each
 // holds rows y0 .. y0+nY (nY <= TY) and marches z from z0 to z0+nZ, so one
 // z-step of the workgroup reads nY+1 consecutive rows of one plane. At every
-// step each wave publishes the low word of its row masks in LDS (double
-// buffered by z parity), one barrier later it reads its right neighbour's
-// word for the bit x+1 of its last lane, forms lane l's pair value
-// (bit x, bit x+1) and the cell codes between this plane and the previous:
-//   hc[r] = q(y0+r) | q(y0+r+1) << 2;  code = hc_prev[r] | hc[r] << 4.
-// Every wave of the workgroup runs the same number of steps (barriers).
-template <int TY, class Src>
+// step each wave publishes the low word of its voxel-0 masks in LDS (double
+// buffered by z parity); one barrier later lane 63 takes bit 0 of its right
+// neighbour's word as the right bit of its last voxel. Every wave of the
+// workgroup runs the same number of steps (barriers).
+template <int TY, int V, class Src>
 __device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1][16], int w, HaloFrom halo,
-                                          int y0, int nY, int z0, int nZ, uint64_t vmask, uint32_t* hist,
-                                          Tally& tally) {
+                                          int y0, int nY, int z0, int nZ, uint32_t* hist, Tally& tally) {
   const int lane = threadIdx.x & (kWave - 1);
-  // The neighbour slot read when the halo comes from LDS (clamped so the
-  // read stays inside the array for the workgroup's last wave).
-  const int nb = min(w + 1, 15);
+  const int nb = min(w + 1, 15);  // neighbour slot, clamped inside the array
   const uint32_t ldsKeep = (halo == kHaloLds) ? 1u : 0u;
+  const uint32_t lane63 = (lane == kWave - 1) ? 1u : 0u;
   long long yo[TY + 1];
 #pragma unroll
   for (int r = 0; r <= TY; ++r) yo[r] = (long long)(y0 + min(r, nY)) * src.sy;
@@ -183,42 +246,46 @@ __device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1
 #pragma unroll
     for (int r = 0; r <= TY; ++r) cur[r] = src.load(yo[r] + zo);
   }
-  uint32_t hp[TY];
+  uint32_t Hp[TY];
   for (int z = z0; z <= z0 + nZ; ++z) {
     const long long zon = (long long)min(z + 1, z0 + nZ) * src.sz;
 #pragma unroll
     for (int r = 0; r <= TY; ++r) nxt[r] = src.load(yo[r] + zon);
     const int par = z & 1;
-    uint64_t R[TY + 1];
+    uint32_t P[TY + 1];
 #pragma unroll
     for (int r = 0; r <= TY; ++r) {
-      R[r] = src.mask(cur[r]);
-      if (lane == 0) xch[par][r][w] = (uint32_t)R[r];
+      uint64_t R0;
+      P[r] = src.pairs(cur[r], R0);
+      if (lane == 0) xch[par][r][w] = (uint32_t)R0;
     }
     __syncthreads();
-    uint32_t hb[TY + 1];
     if (halo == kHaloMem) {
       const long long zo = (long long)z * src.sz;
 #pragma unroll
-      for (int r = 0; r <= TY; ++r) hb[r] = src.haloBit(yo[r] + zo);
+      for (int r = 0; r <= TY; ++r) P[r] |= (src.haloBit(yo[r] + zo) & lane63) << kRightBit<V>;
     } else {
 #pragma unroll
-      for (int r = 0; r <= TY; ++r) hb[r] = __builtin_amdgcn_readfirstlane(xch[par][r][nb]) & ldsKeep;
+      for (int r = 0; r <= TY; ++r) {
+        const uint32_t hb = __builtin_amdgcn_readfirstlane(xch[par][r][nb]) & ldsKeep;
+        P[r] |= (hb & lane63) << kRightBit<V>;
+      }
     }
-    uint32_t q[TY + 1];
+    uint32_t Hc[TY];
 #pragma unroll
-    for (int r = 0; r <= TY; ++r) q[r] = pairBits(R[r], (R[r] >> 1) | ((uint64_t)hb[r] << 63));
-    uint32_t hc[TY];
-#pragma unroll
-    for (int r = 0; r < TY; ++r) hc[r] = q[r] | (q[r + 1] << 2);
+    for (int r = 0; r < TY; ++r) Hc[r] = P[r] | (P[r + 1] << 2);
     if (z > z0) {
 #pragma unroll
       for (int r = 0; r < TY; ++r) {
-        if (r < nY) tally.add(hp[r] | (hc[r] << 4), vmask, hist);
+        if (r < nY) {
+          const uint32_t C = Hp[r] | (Hc[r] << 4);
+#pragma unroll
+          for (int k = 0; k < V; ++k) tally.add((C >> (8 * k)) & 255u, hist);
+        }
       }
     }
 #pragma unroll
-    for (int r = 0; r < TY; ++r) hp[r] = hc[r];
+    for (int r = 0; r < TY; ++r) Hp[r] = Hc[r];
 #pragma unroll
     for (int r = 0; r <= TY; ++r) cur[r] = nxt[r];
   }

@@ -24,10 +24,11 @@ struct Tiling {
   int TP;  // plane pairs per tile (>= 1)
   int ZW;  // waves per workgroup along x (1..16)
   int Ty;  // row pairs per wave (2, 4 or 8)
+  int V;   // voxels per lane: 1, or 16 bytes' worth (4 floats / 2 doubles)
 };
 
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes);
-bool tilingSupported(const Tiling& tl);
+bool tilingSupported(const Tiling& tl, int elemBytes);
 
 // Thresholds fv into the PBM pW (rowWS / planeWS word strides).
 hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint32_t rowWS, uint32_t planeWS,

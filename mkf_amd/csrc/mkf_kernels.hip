@@ -76,17 +76,27 @@ __device__ __forceinline__ bool tileOf(const TileGrid& g, int& xt, int& yr, int&
   return true;
 }
 
+// Sum over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t waveSum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 // Adds the per-wave sub-histograms of the workgroup to the result with one
-// 64-bit atomic per nonzero bin. Bin 0 is each wave's cell count minus its
-// all-one and mixed cells.
+// 64-bit atomic per nonzero bin. A wave's bin 0 is its cell count minus its
+// all-one cells minus its mixed cells (the sum of its sub-histogram).
 __device__ __forceinline__ void flushHistogram(uint32_t (*hist)[256], const Tally& tally, uint32_t cells,
                                                unsigned long long* __restrict__ bpfd) {
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
+  uint32_t mixed = 0;
+  for (int i = lane; i < 256; i += kWave) mixed += hist[w][i];
+  mixed = waveSum(mixed);
   if (lane == 0) {
-    hist[w][0] += cells - tally.c255 - tally.cmix;
-    hist[w][255] += tally.c255;
+    hist[w][0] = cells - tally.c255 - mixed;
+    hist[w][255] = tally.c255;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 256; i += blockDim.x) {
@@ -96,28 +106,27 @@ __device__ __forceinline__ void flushHistogram(uint32_t (*hist)[256], const Tall
   }
 }
 
-// Geometry of one wave inside its tile.
+// Geometry of one wave inside its tile: the wave holds V voxels per lane, so
+// it covers W = 64 V voxels starting at x0.
 struct WaveGeom {
-  int w, seg, x0, nCells, y0, nY, z0, nZ;
+  int w, seg, x0, W, y0, nY, z0, nZ;
   HaloFrom halo;
-  uint64_t vmask;
 };
 
-__device__ __forceinline__ WaveGeom waveGeom(const TileGrid& g, int xt, int yr, int zr, int TY, int X, int Y,
-                                             int Z) {
+__device__ __forceinline__ WaveGeom waveGeom(const TileGrid& g, int xt, int yr, int zr, int TY, int V, int X,
+                                             int Y, int Z) {
   WaveGeom q;
   q.w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   q.seg = xt * g.waves + q.w;
-  q.x0 = q.seg * kWave;
-  q.nCells = max(0, min(kWave, X - 1 - q.x0));
-  q.vmask = lowBits(q.nCells);
+  q.W = kWave * V;
+  q.x0 = q.seg * q.W;
   q.y0 = yr * TY;
   q.nY = min(TY, Y - 1 - q.y0);
   q.z0 = zr * g.TZ;
   q.nZ = min(g.TZ, Z - 1 - q.z0);
-  // The right neighbour's segment holds the bit x0+64; it is in this
-  // workgroup unless this wave is the last of its tile.
-  if (q.x0 + kWave >= X) {
+  // The next wave's first voxel is x0 + W; that wave is in this workgroup
+  // unless this wave is the last of its tile.
+  if (q.x0 + q.W >= X) {
     q.halo = kHaloNone;
   } else if (q.w + 1 < g.waves) {
     q.halo = kHaloLds;
@@ -127,7 +136,23 @@ __device__ __forceinline__ WaveGeom waveGeom(const TileGrid& g, int xt, int yr, 
   return q;
 }
 
-template <typename T, int M, int TY>
+// Lane mask of pair-word bytes whose cell exists, and the wave's cell count
+// per plane pair and row pair.
+template <int V>
+__device__ __forceinline__ uint32_t cellKeep(int x0, int X, uint32_t& cellsPerRow) {
+  const int lane = threadIdx.x & (kWave - 1);
+  uint32_t keep = 0, n = 0;
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    const bool ok = x0 + V * lane + k < X - 1;
+    keep |= ok ? (0xFFu << (8 * k)) : 0u;
+    n += ok ? 1u : 0u;
+  }
+  cellsPerRow = waveSum(n);
+  return keep;
+}
+
+template <typename T, int M, int V, int TY>
 __global__ __launch_bounds__(1024) void kFusedBPFD(const T* __restrict__ f, T thr, int X, int Y, int Z,
                                                     long long sy, long long sz, TileGrid g,
                                                     unsigned long long* __restrict__ bpfd) {
@@ -136,21 +161,24 @@ __global__ __launch_bounds__(1024) void kFusedBPFD(const T* __restrict__ f, T th
   int xt, yr, zr;
   if (!tileOf(g, xt, yr, zr)) return;
   const int lane = threadIdx.x & (kWave - 1);
-  const WaveGeom q = waveGeom(g, xt, yr, zr, TY, X, Y, Z);
+  const WaveGeom q = waveGeom(g, xt, yr, zr, TY, V, X, Y, Z);
   for (int i = lane; i < 256; i += kWave) hist[q.w][i] = 0;
 
   const int xb = min(q.x0, X - 1);
-  FieldRows<T, M> src;
+  FieldRows<T, M, V> src;
   src.fw = f + xb;
   src.thr = thr;
   src.sy = sy;
   src.sz = sz;
-  src.lx = (uint32_t)(min(q.x0 + lane, X - 1) - xb);
-  src.hxo = min(q.x0 + kWave, X - 1) - xb;
-  src.tail = q.x0 < X ? lowBits(X - q.x0) : 0ull;
+  src.lx = (uint32_t)(min(q.x0 + V * lane, X - 1) - xb);
+  src.last = X - 1 - xb;
+  src.full = q.x0 + q.W <= X;
+  src.hxo = min(q.x0 + q.W, X - 1) - xb;
+  uint32_t cellsPerRow;
+  src.keep = cellKeep<V>(q.x0, X, cellsPerRow);
   Tally tally;
-  walkStack<TY>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, q.vmask, hist[q.w], tally);
-  flushHistogram(hist, tally, (uint32_t)q.nCells * q.nY * q.nZ, bpfd);
+  walkStack<TY, V>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist[q.w], tally);
+  flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
 }
 
 template <int TY>
@@ -162,7 +190,7 @@ __global__ __launch_bounds__(1024) void kBPFD(const uint32_t* __restrict__ pW, i
   int xt, yr, zr;
   if (!tileOf(g, xt, yr, zr)) return;
   const int lane = threadIdx.x & (kWave - 1);
-  const WaveGeom q = waveGeom(g, xt, yr, zr, TY, X, Y, Z);
+  const WaveGeom q = waveGeom(g, xt, yr, zr, TY, 1, X, Y, Z);
   for (int i = lane; i < 256; i += kWave) hist[q.w][i] = 0;
 
   PbmRows src;
@@ -173,10 +201,11 @@ __global__ __launch_bounds__(1024) void kBPFD(const uint32_t* __restrict__ pW, i
   src.sz = planeWS;
   src.o1 = (2u * q.seg + 1u <= lastWord) ? 1u : 0u;
   src.hw = min(2u * q.seg + 2u, lastWord) - wi0;
-  src.tail = q.x0 < X ? lowBits(X - q.x0) : 0ull;
+  uint32_t cellsPerRow;
+  src.keep = cellKeep<1>(q.x0, X, cellsPerRow);
   Tally tally;
-  walkStack<TY>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, q.vmask, hist[q.w], tally);
-  flushHistogram(hist, tally, (uint32_t)q.nCells * q.nY * q.nZ, bpfd);
+  walkStack<TY, 1>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist[q.w], tally);
+  flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
 }
 
 // ---------------------------------------------------------------------------
@@ -186,7 +215,7 @@ static inline long long ceilDiv(long long a, long long b) { return (a + b - 1) /
 
 static TileGrid makeGrid(int X, int Y, int Z, const Tiling& tl) {
   TileGrid g;
-  g.nSegX = (int)ceilDiv(X - 1, kWave);
+  g.nSegX = (int)ceilDiv(X - 1, (long long)kWave * tl.V);
   g.waves = (int)std::min<long long>(tl.ZW, g.nSegX);
   g.nXT = (int)ceilDiv(g.nSegX, g.waves);
   g.TZ = tl.TP;
@@ -197,24 +226,35 @@ static TileGrid makeGrid(int X, int Y, int Z, const Tiling& tl) {
   return g;
 }
 
-bool tilingSupported(const Tiling& tl) {
-  return (tl.Ty == 2 || tl.Ty == 4 || tl.Ty == 8) && tl.TP >= 1 && tl.ZW >= 1 && tl.ZW <= 16;
+bool tilingSupported(const Tiling& tl, int elemBytes) {
+  const bool vOk = tl.V == 1 || (elemBytes == 4 && tl.V == 4) || (elemBytes == 8 && tl.V == 2);
+  return vOk && (tl.Ty == 2 || tl.Ty == 4 || tl.Ty == 8) && tl.TP >= 1 && tl.ZW >= 1 && tl.ZW <= 16;
 }
 
 // Workgroups as wide as the row (up to 16 waves) and the longest z-range
-// that still gives about 32 waves per CU. Row pairs per wave: 4 for f32,
-// 8 for f64 (measured on MI355X at 1024^3: f32 0.82 ms with 4 vs 0.90 ms
-// with 8, whose scalar registers cap a 16-wave workgroup at one per CU;
-// f64 1.39 ms with 8 vs 1.49 ms with 4).
+// that still gives every CU about 64 lanes' worth of voxels x 16 waves.
+// Measured on MI355X at 1024^3 (HIP events, warm clocks):
+//   f32: 4 voxels/lane, 4 row pairs, 256 plane pairs  0.723 ms (5.94 TB/s);
+//        1 voxel/lane, 4 row pairs, 128 plane pairs   0.730 ms;
+//        4 voxels/lane, 8 row pairs                   0.76-0.77 ms.
+//   f64: 1 voxel/lane, 8 row pairs, 128 plane pairs   1.342 ms (6.40 TB/s);
+//        2 voxels/lane, 8 row pairs                   1.41 ms.
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   Tiling tl;
-  tl.Ty = elemBytes >= 8 ? 8 : 4;
+  if (elemBytes >= 8) {
+    tl.V = 1;
+    tl.Ty = 8;
+    tl.TP = 128;
+  } else {
+    tl.V = 4;
+    tl.Ty = 4;
+    tl.TP = 256;
+  }
   tl.ZW = 16;
-  tl.TP = elemBytes >= 8 ? 128 : 256;
-  const long long targetWaves = 256LL * 32;
+  const long long targetLaneWork = 256LL * 16 * 4;  // waves x voxels per lane
   while (tl.TP > 8) {
     const TileGrid g = makeGrid(X, Y, Z, tl);
-    if ((long long)g.nTiles * g.waves >= targetWaves) break;
+    if ((long long)g.nTiles * g.waves * tl.V >= targetLaneWork) break;
     tl.TP >>= 1;
   }
   return tl;
@@ -257,56 +297,66 @@ hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint
   return hipErrorInvalidValue;
 }
 
-template <typename T, int M, int TY>
+template <typename T, int M, int V, int TY>
 static hipError_t launchFusedT(const FieldView& fv, T thr, unsigned long long* bpfd, const TileGrid& g,
                                hipStream_t s) {
-  hipLaunchKernelGGL((kFusedBPFD<T, M, TY>), dim3((unsigned)g.chunk * 8u), dim3(g.waves * kWave), 0, s,
+  hipLaunchKernelGGL((kFusedBPFD<T, M, V, TY>), dim3((unsigned)g.chunk * 8u), dim3(g.waves * kWave), 0, s,
                      (const T*)fv.p, thr, fv.X, fv.Y, fv.Z, fv.sy, fv.sz, g, bpfd);
   return hipGetLastError();
 }
 
-template <typename T, int TY>
+template <typename T, int V, int TY>
 static hipError_t dispatchFusedM(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd,
                                  const TileGrid& g, hipStream_t s) {
   switch (m & 7u) {
-    case 0: return launchFusedT<T, 0, TY>(fv, thr, bpfd, g, s);
-    case 1: return launchFusedT<T, 1, TY>(fv, thr, bpfd, g, s);
-    case 2: return launchFusedT<T, 2, TY>(fv, thr, bpfd, g, s);
-    case 3: return launchFusedT<T, 3, TY>(fv, thr, bpfd, g, s);
-    case 4: return launchFusedT<T, 4, TY>(fv, thr, bpfd, g, s);
-    case 5: return launchFusedT<T, 5, TY>(fv, thr, bpfd, g, s);
-    case 6: return launchFusedT<T, 6, TY>(fv, thr, bpfd, g, s);
-    default: return launchFusedT<T, 7, TY>(fv, thr, bpfd, g, s);
+    case 0: return launchFusedT<T, 0, V, TY>(fv, thr, bpfd, g, s);
+    case 1: return launchFusedT<T, 1, V, TY>(fv, thr, bpfd, g, s);
+    case 2: return launchFusedT<T, 2, V, TY>(fv, thr, bpfd, g, s);
+    case 3: return launchFusedT<T, 3, V, TY>(fv, thr, bpfd, g, s);
+    case 4: return launchFusedT<T, 4, V, TY>(fv, thr, bpfd, g, s);
+    case 5: return launchFusedT<T, 5, V, TY>(fv, thr, bpfd, g, s);
+    case 6: return launchFusedT<T, 6, V, TY>(fv, thr, bpfd, g, s);
+    default: return launchFusedT<T, 7, V, TY>(fv, thr, bpfd, g, s);
   }
 }
 
-template <typename T>
+template <typename T, int V>
+static hipError_t dispatchFusedTY(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd,
+                                  const TileGrid& g, int Ty, hipStream_t s) {
+  switch (Ty) {
+    case 2: return dispatchFusedM<T, V, 2>(fv, thr, m, bpfd, g, s);
+    case 4: return dispatchFusedM<T, V, 4>(fv, thr, m, bpfd, g, s);
+    default: return dispatchFusedM<T, V, 8>(fv, thr, m, bpfd, g, s);
+  }
+}
+
+template <typename T, int VW>
 static hipError_t dispatchFused(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
                                 hipStream_t s) {
   const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, tl);
   if (g.nTiles <= 0) return hipSuccess;
-  switch (tl.Ty) {
-    case 2: return dispatchFusedM<T, 2>(fv, thr, m, bpfd, g, s);
-    case 8: return dispatchFusedM<T, 8>(fv, thr, m, bpfd, g, s);
-    default: return dispatchFusedM<T, 4>(fv, thr, m, bpfd, g, s);
-  }
+  if (tl.V == 1) return dispatchFusedTY<T, 1>(fv, thr, m, bpfd, g, tl.Ty, s);
+  return dispatchFusedTY<T, VW>(fv, thr, m, bpfd, g, tl.Ty, s);
 }
 
 hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, const Tiling& tl,
                        hipStream_t s) {
   if (fv.X < 2 || fv.Y < 2 || fv.Z < 2) return hipSuccess;  // no cells
-  if (!tilingSupported(tl)) return hipErrorInvalidValue;
+  const int eb = fv.enc & ENC_MASK_NUM;
+  if (!tilingSupported(tl, eb)) return hipErrorInvalidValue;
   // f32 fields compare against the threshold rounded to float
   // (src/binMapCUDA.cuh:138-139).
-  if (fv.enc == ENC_F32) return dispatchFused<float>(fv, (float)m.t[0], m.m, bpfd, tl, s);
-  if (fv.enc == ENC_F64) return dispatchFused<double>(fv, m.t[0], m.m, bpfd, tl, s);
+  if (fv.enc == ENC_F32) return dispatchFused<float, 4>(fv, (float)m.t[0], m.m, bpfd, tl, s);
+  if (fv.enc == ENC_F64) return dispatchFused<double, 2>(fv, m.t[0], m.m, bpfd, tl, s);
   return hipErrorInvalidValue;
 }
 
 hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
-                      unsigned long long* bpfd, const Tiling& tl, hipStream_t s) {
+                      unsigned long long* bpfd, const Tiling& tl0, hipStream_t s) {
   if (X < 2 || Y < 2 || Z < 2) return hipSuccess;
-  if (!tilingSupported(tl)) return hipErrorInvalidValue;
+  Tiling tl = tl0;
+  tl.V = 1;  // one PBM bit per lane
+  if (!tilingSupported(tl, 4)) return hipErrorInvalidValue;
   const TileGrid g = makeGrid(X, Y, Z, tl);
   const dim3 grid((unsigned)g.chunk * 8u), block(g.waves * kWave);
   switch (tl.Ty) {

@@ -87,10 +87,14 @@ TILINGS = [(1, 16, 4), (3, 16, 2), (8, 2, 4), (64, 3, 8), (5, 1, 2), (128, 16, 8
 
 
 @pytest.mark.parametrize("tiling", TILINGS)
+@pytest.mark.parametrize("vpl", [1, 0])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-def test_fused_and_pbm_histogram_tilings(lib, orc, cuda, tiling, dtype):
-    print(f"tiling {tiling} {dtype}", flush=True)
-    f = synth.grf_field((37, 41, 300), sigma=2.0, seed=3, device=cuda, dtype=dtype)
+@pytest.mark.parametrize("X", [300, 1030])
+def test_fused_and_pbm_histogram_tilings(lib, orc, cuda, tiling, vpl, dtype, X):
+    """voxelsPerLane 0 selects 16 bytes per lane (4 floats / 2 doubles)."""
+    tiling = tiling + (vpl,)
+    print(f"tiling {tiling} {dtype} X={X}", flush=True)
+    f = synth.grf_field((37, 41, X), sigma=2.0, seed=3, device=cuda, dtype=dtype)
     w_ref, h_ref = oracle_all(orc, host_np(f), ">=", 0.0)
     h, _ = dev_fused(lib, f, ">=", 0.0, tiling=tiling)
     np.testing.assert_array_equal(h, h_ref)
@@ -107,7 +111,7 @@ def test_bad_tiling_is_rejected(lib, cuda):
     bm = M.bin_map(">=", 0.5)
     h = torch.zeros(256, dtype=torch.int64, device=cuda)
     fi = M.FieldInfo(f.data_ptr(), (64, 8, 8), M.ENC_F32)
-    for bad in [(0, 16, 4), (8, 17, 4), (8, 16, 3), (8, 0, 4)]:
+    for bad in [(0, 16, 4, 1), (8, 17, 4, 1), (8, 16, 3, 1), (8, 0, 4, 1), (8, 16, 4, 3), (8, 16, 4, 2)]:
         t = M.MkfTiling(*bad)
         assert lib.mkfFusedBPFD(None, h.data_ptr(), None, fi.ref, C.byref(bm), C.byref(t), None) == -1
 
