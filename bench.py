@@ -34,6 +34,145 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def bench_batch(args, dev, stream):
+    """The uneven batch: 64 GRF fields (each axis dim uniform in [32, 384],
+    half float32 / half float64, sigma uniform in [1, 6]) x 8 levels >= t,
+    t = linspace(-1.5, 1.5, 8) passed as float; one graph replay of the
+    persistent batch kernel per step, against plain per-field launches."""
+    import numpy as np
+    import torch
+
+    import mkf_amd as M
+    from mkf_amd import synth
+    from mkf_amd.batch import Batch
+
+    lib = M.load()
+    sptr = C.c_void_p(stream.cuda_stream)
+    specs = synth.batch_mix(seed=args.batch_seed)
+    t0 = time.perf_counter()
+    fields = synth.make_fields(specs, dev)
+    torch.cuda.synchronize()
+    log(f"[batch] {len(fields)} fields generated in {time.perf_counter() - t0:.1f}s")
+    levels = [float(v) for v in np.linspace(-1.5, 1.5, 8)]
+    L = len(levels)
+    lv = [[(">=", t) for t in levels] for _ in fields]
+    nvox = sum(f.numel() for f in fields)
+    nbytes = sum(f.numel() * f.element_size() for f in fields)
+    work = nvox * L
+
+    B = Batch(fields, lv, zero=True, stream=stream)
+    cands = [(pp, sg) for pp in (16, 32, 64, 128) for sg in (8, 2, 1)]
+    t0 = time.perf_counter()
+    best = B.autotune(cands, reps=3)
+    info = B.info()
+    log(f"[batch] autotune over {len(cands)} shapes in {time.perf_counter() - t0:.2f}s -> planePairs {best[0]}, "
+        f"segments {best[1]}, {info['items']} items, grid {info['grid']}")
+
+    def timed(fn, reps, min_s=0.5):
+        t_w = time.perf_counter()
+        n = 0
+        while n < 2 or time.perf_counter() - t_w < min_s:
+            fn()
+            n += 1
+            if n % 8 == 0:
+                torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        t1 = time.perf_counter()
+        for a, b in ev:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t1) / reps
+        ms = [a.elapsed_time(b) for a, b in ev]
+        return wall * 1e3, sum(ms) / len(ms), min(ms)
+
+    steps = max(5, args.steps)
+    b_wall, b_ms, b_min = timed(lambda: B.run(stream), steps)
+    batch_hist = B.out.cpu().numpy().view(np.uint64).copy()
+    d_wall, d_ms, _ = timed(lambda: B.run(stream, graph=False), steps)
+    log(f"[batch] graph replay {b_ms:.3f} ms (min {b_min:.3f}), direct launch {d_ms:.3f} ms; "
+        f"{work / (b_wall * 1e-3) / 1e9:.0f} Gvox-levels/s, {nbytes / (b_ms * 1e-3) / 1e9:.0f} GB/s")
+
+    # Plain 1: one mkfFusedBPFD launch per field and level (the field is read
+    # once per level), asynchronous on the same stream.
+    fused_out = torch.zeros((len(fields) * L, 256), dtype=torch.int64, device=dev)
+    infos = [M.FieldInfo(f.data_ptr(), tuple(reversed(f.shape)), M.ENC_F32 if f.dtype == torch.float32 else M.ENC_F64)
+             for f in fields]
+    maps = [M.bin_map(">=", t) for t in levels]
+
+    def plain_fused():
+        fused_out.zero_()
+        for i, fi in enumerate(infos):
+            for j, bm in enumerate(maps):
+                rc = lib.mkfFusedBPFD(None, fused_out[i * L + j].data_ptr(), None, fi.ref, C.byref(bm), None, sptr)
+                if rc:
+                    raise RuntimeError(M.last_error())
+
+    f_wall, f_ms, _ = timed(plain_fused, max(3, steps // 2))
+    fused_same = bool(np.array_equal(fused_out.cpu().numpy().view(np.uint64), batch_hist))
+    log(f"[batch] plain fused launches: {f_wall:.3f} ms wall ({f_ms:.3f} ms GPU), same histograms: {fused_same}")
+
+    # Plain 2: the reference's drop-in sequence per field and level:
+    # binMapCUDA into a PBM buffer, then mkfCUDAGetBPFD (both blocking).
+    drop_out = torch.zeros((len(fields) * L, 256), dtype=torch.int64, device=dev)
+    maxw = max(((f.shape[2] + 31) // 32) * f.shape[1] * f.shape[0] for f in fields)
+    pbm = torch.empty(maxw, dtype=torch.int32, device=dev)
+
+    def plain_dropin():
+        drop_out.zero_()
+        torch.cuda.synchronize()
+        for i, fi in enumerate(infos):
+            for j, bm in enumerate(maps):
+                org = M.BMOrg()
+                if not lib.binMapCUDA(None, pbm.data_ptr(), C.byref(org), fi.ref, C.byref(bm)):
+                    raise RuntimeError(M.last_error())
+                lib.mkfCUDAGetBPFD(None, drop_out[i * L + j].data_ptr(), C.byref(org), pbm.data_ptr(), 3)
+
+    plain_dropin()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    reps_d = 2
+    for _ in range(reps_d):
+        plain_dropin()
+    torch.cuda.synchronize()
+    dr_wall = (time.perf_counter() - t1) * 1e3 / reps_d
+    drop_same = bool(np.array_equal(drop_out.cpu().numpy().view(np.uint64), batch_hist))
+    log(f"[batch] plain drop-in calls: {dr_wall:.3f} ms wall, same histograms: {drop_same}")
+
+    res = {
+        "workload": f"{len(fields)} GRF fields, dims U[32,384] per axis, 32 f32 + 32 f64, sigma U[1,6], "
+                    f"{L} levels >= linspace(-1.5,1.5) as float; batch seed {args.batch_seed}",
+        "voxels": nvox, "field_bytes": nbytes, "levels": L,
+        "value": round(work / (b_wall * 1e-3) / 1e9, 2), "unit": "Gvox-levels/s",
+        "ms_per_batch": round(b_wall, 4), "kernel_ms_mean": round(b_ms, 4), "kernel_ms_min": round(b_min, 4),
+        "direct_launch_ms": round(d_ms, 4),
+        "hbm_GBps": round(nbytes / (b_ms * 1e-3) / 1e9, 1),
+        "hbm_frac": round(nbytes / (b_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "shape": {"planePairs": best[0], "segments": best[1]}, "items": info["items"], "grid": info["grid"],
+        "plain_fused_ms": round(f_wall, 4), "plain_dropin_ms": round(dr_wall, 4),
+        "speedup_vs_plain_fused": round(f_wall / b_wall, 2), "speedup_vs_plain_dropin": round(dr_wall / b_wall, 2),
+        "same_as_plain_fused": fused_same, "same_as_plain_dropin": drop_same,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        orc.set_threads(args.cpu_threads or min(16, os.cpu_count() or 1))
+        t1 = time.perf_counter()
+        ok = True
+        for i, f in enumerate(fields):
+            f_np = f.cpu().numpy()
+            for j, bm in enumerate(maps):
+                h, _ = orc.bpfd_field(f_np, bm.t[0], bm.m)
+                ok &= bool(np.array_equal(h, batch_hist[i * L + j]))
+        res["parity"] = "bit-exact vs oracle (all 512 histograms)" if ok else "MISMATCH vs oracle"
+        log(f"[batch] oracle check of every histogram in {time.perf_counter() - t1:.1f}s: {ok}")
+    B.close()
+    del fields, fused_out, drop_out, pbm
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,6 +187,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--tiling", type=str, default="",
                     help="planePairs,waves,rowPairs[,voxelsPerLane] (default: library choice)")
+    ap.add_argument("--mode", choices=["all", "fused", "batch"], default="all",
+                    help="fused: the 1024^3 kernel only; batch: the uneven batch only; all: both (default)")
+    ap.add_argument("--batch-seed", type=int, default=4)
     ap.add_argument("--pmc-workload", action="store_true",
                     help="for counter runs: 3 fused launches, then 3 copies of a 1 GiB buffer (the calibration "
                          "kernel for FETCH_SIZE / WRITE_SIZE); prints no JSON")
@@ -67,6 +209,15 @@ def main():
 
     lib = M.load()
     dev = torch.device("cuda:0")
+    if args.mode == "batch":
+        res = bench_batch(args, dev, torch.cuda.current_stream())
+        out = {"metric": "Gvox-levels/s batched uneven mix (64 fields x 8 levels)", "value": res["value"],
+               "unit": res["unit"], "n_gpus": 1, "steps": max(5, args.steps), "warmup": args.warmup,
+               "ms_per_step": res["ms_per_batch"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic: seeded GRF mix",
+               "config": {"workload": res["workload"]}, "batched": res}
+        print(json.dumps(out), flush=True)
+        return
     S = args.size
     log(f"[bench] generating {S}^3 GRF sigma={args.sigma} seed={args.seed} ({args.dtype})")
     t0 = time.perf_counter()
@@ -177,6 +328,39 @@ def main():
         "cpu_baseline": None,
     }
 
+    # The reference's two-kernel drop-in path on the same field: binMapCUDA
+    # (PBM into a caller buffer) then mkfCUDAGetBPFD (histogram of the PBM),
+    # timed with HIP events per kernel on the launch stream.
+    org = M.BMOrg()
+    nwords = lib.setBMO(C.byref(org), M.dims((S, S, S)), b"\x00")
+    pbm = torch.empty(nwords, dtype=torch.int32, device=dev)
+    h2 = torch.zeros(256, dtype=torch.int64, device=dev)
+    t1 = lib.mkfDefaultTiling(M.dims((S, S, S)), 4)
+    pbm_ms, bpfd_ms = [], []
+    for i in range(max(3, K // 4) + 2):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        h2.zero_()
+        e[0].record(stream)
+        M.check(lib.mkfPBMAsync(pbm.data_ptr(), C.byref(org), fi.ref, C.byref(bm), sptr), "mkfPBMAsync")
+        e[1].record(stream)
+        M.check(lib.mkfBPFDAsync(h2.data_ptr(), C.byref(org), pbm.data_ptr(), C.byref(t1), sptr), "mkfBPFDAsync")
+        e[2].record(stream)
+        torch.cuda.synchronize()
+        if i >= 2:
+            pbm_ms.append(e[0].elapsed_time(e[1]))
+            bpfd_ms.append(e[1].elapsed_time(e[2]))
+    dropin_ok = bool(np.array_equal(h2.cpu().numpy().view(np.uint64), gpu_hist))
+    pbm_bytes = nvox * eb + nwords * 4
+    result["dropin"] = {
+        "pbm_ms": round(sum(pbm_ms) / len(pbm_ms), 4), "bpfd_ms": round(sum(bpfd_ms) / len(bpfd_ms), 4),
+        "pbm_GBps": round(pbm_bytes / (sum(pbm_ms) / len(pbm_ms) * 1e-3) / 1e9, 1),
+        "bpfd_GBps": round(nwords * 4 / (sum(bpfd_ms) / len(bpfd_ms) * 1e-3) / 1e9, 1),
+        "same_histogram_as_fused": dropin_ok,
+    }
+    log(f"[bench] drop-in path: PBM {result['dropin']['pbm_ms']:.3f} ms ({result['dropin']['pbm_GBps']} GB/s), "
+        f"BPFD {result['dropin']['bpfd_ms']:.3f} ms, same histogram: {dropin_ok}")
+    del pbm
+
     if not args.no_cpu:
         from oracle import oracle as orc
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -197,6 +381,10 @@ def main():
         if not parity:
             print(json.dumps(result))
             sys.exit(1)
+    if args.mode == "all":
+        del field
+        torch.cuda.empty_cache()
+        result["batched"] = bench_batch(args, dev, stream)
     print(json.dumps(result), flush=True)
 
 

@@ -54,6 +54,58 @@ int mkfBPFDAsync(size_t *pBPFD, const BMOrg *pO, const BMPackWord *pW, const Mkf
 /* Tiling the library would choose for a field of dims def and element size. */
 MkfTiling mkfDefaultTiling(const FieldDef def[3], int elemBytes);
 
+/* ---------------------------------------------------------------------------
+ * Batched form: many fields x many threshold levels in one launch
+ * ------------------------------------------------------------------------- */
+#define MKF_BATCH_MAX_LEVELS 16
+#define MKF_BATCH_ZERO_OUTPUT 1 /* flag: clear every output before each run */
+
+/* One field of a batch: a planar device field of dims def, evaluated at
+ * nLevels threshold maps pM[] (host array). Its nLevels consecutive 256-bin
+ * histograms at pBPFD (device, uint64) are ADDED to by each run (or cleared
+ * first with MKF_BATCH_ZERO_OUTPUT). */
+typedef struct {
+  const void *pF;
+  NumEnc enc;
+  FieldDef def[3];
+  int32_t nLevels;
+  const BinMapF64 *pM;
+  size_t *pBPFD;
+} MkfBatchField;
+
+/* Work-item (slab) shape of a batch. A work item is a slab of planePairs
+ * plane pairs by 8 row-stacks-worth of row pairs by one x-tile of
+ * `segments` 64-voxel segments; the 8 waves of a workgroup cover
+ * 8 / segments row stacks of 4 row pairs each. */
+typedef struct {
+  int32_t planePairs; /* slab depth, 1..4096; 0 selects 32 */
+  int32_t segments;   /* largest x-tile: 1, 2, 4 or 8 segments; 0 selects 8. Each field
+                         uses the largest power of two up to this that wastes the fewest
+                         segments at the end of its rows */
+} MkfBatchShape;
+
+typedef struct MkfBatch MkfBatch;
+
+/* Plans a batch: a device work queue of (field, x-tile, y-block, slab) items,
+ * each field read once for all of its levels by a persistent kernel,
+ * captured with the queue reset into a HIP graph. shape NULL selects the
+ * defaults; see mkfBatchAutotune. NULL on error (mkfLastError). */
+MkfBatch *mkfBatchCreate(const MkfBatchField *fields, int nFields, const MkfBatchShape *shape, int flags,
+                         void *stream);
+/* Replays the captured graph on stream (asynchronous). Runs on one device
+ * overlap only if their outputs are distinct. */
+int mkfBatchRun(MkfBatch *b, void *stream);
+/* Launches the same kernels without the graph (for comparison). */
+int mkfBatchRunDirect(MkfBatch *b, void *stream);
+/* Times each candidate shape (one warm-up and reps runs each, into scratch
+ * histograms: the outputs are not touched), keeps the fastest, re-captures
+ * and stores it in *chosen (may be NULL). Returns MKF_OK or an error. */
+int mkfBatchAutotune(MkfBatch *b, const MkfBatchShape *candidates, int nCandidates, int reps,
+                     MkfBatchShape *chosen, void *stream);
+/* Work items, shape and persistent grid size of a plan (pointers may be NULL). */
+int mkfBatchInfo(const MkfBatch *b, int *nItems, MkfBatchShape *shape, int *grid);
+void mkfBatchDestroy(MkfBatch *b);
+
 /* Read-only views of the measure tables used by mkfSelectMeasuresBPFD: the
  * symmetry group of each of the 256 patterns and the Euler weight of each of
  * the 22 groups (src/mkfUtil.c:33-58), for callers that verify them. */

@@ -68,6 +68,18 @@ class MkfTiling(C.Structure):
                 ("voxelsPerLane", C.c_int32)]
 
 
+class MkfBatchField(C.Structure):
+    _fields_ = [("pF", C.c_void_p), ("enc", C.c_uint8), ("def_", C.c_int * 3), ("nLevels", C.c_int32),
+                ("pM", C.POINTER(BinMapF64)), ("pBPFD", C.c_void_p)]
+
+
+class MkfBatchShape(C.Structure):
+    _fields_ = [("planePairs", C.c_int32), ("segments", C.c_int32)]
+
+
+MKF_BATCH_MAX_LEVELS = 16
+MKF_BATCH_ZERO_OUTPUT = 1
+
 _P = C.c_void_p
 _SZP = C.POINTER(C.c_size_t)
 
@@ -103,6 +115,12 @@ _SIGNATURES = {
     "mkfPBMAsync": (C.c_int, [_P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo), C.POINTER(BinMapF64), _P]),
     "mkfBPFDAsync": (C.c_int, [_P, C.POINTER(BMOrg), _P, C.POINTER(MkfTiling), _P]),
     "mkfDefaultTiling": (MkfTiling, [C.POINTER(C.c_int), C.c_int]),
+    "mkfBatchCreate": (_P, [C.POINTER(MkfBatchField), C.c_int, C.POINTER(MkfBatchShape), C.c_int, _P]),
+    "mkfBatchRun": (C.c_int, [_P, _P]),
+    "mkfBatchRunDirect": (C.c_int, [_P, _P]),
+    "mkfBatchAutotune": (C.c_int, [_P, C.POINTER(MkfBatchShape), C.c_int, C.c_int, C.POINTER(MkfBatchShape), _P]),
+    "mkfBatchInfo": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(MkfBatchShape), C.POINTER(C.c_int)]),
+    "mkfBatchDestroy": (None, [_P]),
     "mkfPatternGroupTable": (C.POINTER(C.c_uint8), []),
     "mkfGroupWeightKTable": (C.POINTER(C.c_int8), []),
 }

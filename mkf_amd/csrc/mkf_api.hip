@@ -33,6 +33,19 @@ int hipFail(hipError_t e, const char* what) {
   return fail(MKF_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+}  // namespace
+
+int mkf::setError(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(gErr, sizeof(gErr), fmt, ap);
+  va_end(ap);
+  if (getenv("MKF_VERBOSE")) fprintf(stderr, "[mkf] %s\n", gErr);
+  return code;
+}
+
+namespace {
+
 // Library-owned buffers of the blocking entry points (the reference keeps the
 // same statics: gpDevW, src/binMapCUDA.cu:142; gHD, src/mkfCUDA.cu:69). The
 // PBM buffer is regrown when a larger field arrives (the reference never

@@ -1,0 +1,590 @@
+// mkf_batch.hip - batched multi-level histograms: many fields of mixed dims
+// and element types, each evaluated at up to kMaxLevels threshold levels, in
+// one persistent launch over a device work queue, captured in a HIP graph.
+//
+// Every field is read from HBM once for all of its levels: a wave loads one
+// stack of rows per z-step and derives the pair bits of every level from the
+// same registers. Work items are (field, x-tile, y-block, slab) tiles sorted
+// by field; persistent workgroups take items from an atomic counter and add
+// their LDS histograms to the field's outputs when the next item belongs to
+// another field (or at the end).
+//
+// A workgroup of kBatchWaves waves holds 8 / segs row stacks side by side
+// (wave w: segment w % segs of the x-tile, stack w / segs), so narrow fields
+// still use every wave. Neighbouring segments of one stack exchange the
+// voxel right of each row through LDS; the last segment of an x-tile reads
+// it from memory.
+//
+// Per level and row the wave forms the row's relation mask with one or two
+// ballots, clears invalid lanes in scalar registers, and packs the 2x2 pair
+// nibbles of its 4 row pairs into the 4 bytes of one word H. The cell codes
+// of plane pair (z-1, z) are the bytes of Hprev | H << 4; all-one cells are
+// counted from ballots, mixed cells go to the workgroup's LDS histogram, and
+// all-zero cells are what remains of the cell count.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "mkf_device.hpp"
+#include "mkf_internal.h"
+
+namespace mkf {
+
+constexpr int kBatchWaves = 8;
+constexpr int kBatchTY = 4;
+constexpr int kMaxLevels = MKF_BATCH_MAX_LEVELS;
+// Slab depth limit: one item adds at most 8 x 64 x 4 x 4096 cells, so the
+// 32-bit LDS counters, flushed once they pass 2^31, cannot wrap.
+constexpr int kMaxPlanePairs = 4096;
+
+// Device descriptor of one field of a batch.
+struct BatchField {
+  const void* p;
+  unsigned long long* out;  // nLevels x 256 u64, added to
+  long long sy, sz;
+  int enc, X, Y, Z;
+  int nLevels;
+  int segs, stacks, TZ;  // segments per x-tile, row stacks per workgroup, plane pairs per slab
+  int nXT, nYB, nZR;     // x-tiles, y-blocks, slabs
+  double t[kMaxLevels];  // f32 fields: the threshold rounded to float
+  uint32_t m[kMaxLevels];
+};
+
+struct BatchItem {
+  int field, xt, yb, zr;
+};
+
+// Shared state of a batch workgroup. hpk holds, per wave, level and lane,
+// the previous plane's H word (four row-pair nibbles, one per byte).
+struct BatchShared {
+  uint32_t hist[kMaxLevels][256];
+  uint32_t hpk[kBatchWaves][kMaxLevels][kWave];
+  double xv[2][kBatchTY + 1][kBatchWaves];  // voxel at lane 0 of each row, per wave
+  uint32_t msum[kMaxLevels];
+  uint32_t cells;
+  int item;
+};
+
+// Adds the workgroup's histograms of field fd to its outputs and clears them.
+// Bin 0 of a level is the workgroup's cells minus its all-one and mixed cells.
+__device__ void batchFlush(BatchShared& sh, const BatchField& fd) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nL = fd.nLevels;
+  __syncthreads();
+  for (int l = 0; l < nL; ++l) {
+    uint32_t part = 0;
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) part += (i >= 1 && i <= 254) ? sh.hist[l][i] : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+    if (lane == 0 && part) atomicAdd(&sh.msum[l], part);
+  }
+  __syncthreads();
+  for (int l = 0; l < nL; ++l) {
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+      const uint32_t v = (i == 0) ? sh.cells - sh.msum[l] - sh.hist[l][255] : sh.hist[l][i];
+      if (v) atomicAdd(fd.out + (size_t)l * 256 + i, (unsigned long long)v);
+    }
+  }
+  __syncthreads();
+  for (int l = 0; l < nL; ++l)
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) sh.hist[l][i] = 0;
+  if (threadIdx.x < kMaxLevels) sh.msum[threadIdx.x] = 0;
+  if (threadIdx.x == 0) sh.cells = 0;
+  __syncthreads();
+}
+
+// Pair bits of one row at relation M: bit 0 from the lane's voxel, bit 1
+// from its right neighbour. Relations that are the complement of a one- or
+// two-ballot mask (">=", "<=", "=", all) select inverted constants rather than
+// inverting the mask. NaN compares false both ways and so takes the EQ bit.
+template <int M, typename T>
+__device__ __forceinline__ uint32_t pairBits(T f, T fr, T t) {
+  constexpr bool inv = (M == 2 || M == 3 || M == 6 || M == 7);
+  uint64_t a = 0, b = 0;
+  if constexpr (M == 1 || M == 6) {
+    a = ballot(f < t);
+    b = ballot(fr < t);
+  } else if constexpr (M == 3 || M == 4) {
+    a = ballot(f > t);
+    b = ballot(fr > t);
+  } else if constexpr (M == 2 || M == 5) {
+    a = ballot(f < t) | ballot(f > t);
+    b = ballot(fr < t) | ballot(fr > t);
+  }
+  const uint32_t pa = laneBit(a) ? (inv ? 0u : 1u) : (inv ? 1u : 0u);
+  const uint32_t pb = laneBit(b) ? (inv ? 0u : 2u) : (inv ? 2u : 0u);
+  return pa | pb;
+}
+
+// One level of one z-step: the pair nibbles of the wave's 5 rows at level
+// relation M, then (from the second plane on) the tally of its row pairs'
+// cells into histL. rgt[r] is the voxel right of each lane's voxel (lane 63:
+// the halo voxel). vr[r] holds the lanes whose cell in row pair r exists
+// (none for row pairs beyond the field), so nothing else is masked.
+template <int M, typename T>
+__device__ __forceinline__ uint32_t levelStep(const T (&cur)[kBatchTY + 1], const T (&rgt)[kBatchTY + 1], T thr,
+                                              const uint64_t (&vr)[kBatchTY], bool count, uint32_t hp,
+                                              uint32_t* histL) {
+  uint32_t P[kBatchTY + 1];
+#pragma unroll
+  for (int r = 0; r <= kBatchTY; ++r) P[r] = pairBits<M>(cur[r], rgt[r], thr);
+  const uint32_t lo = P[0] | (P[1] << 8) | (P[2] << 16) | (P[3] << 24);
+  const uint32_t hi = (lo >> 8) | (P[4] << 24);
+  const uint32_t H = lo | (hi << 2);
+  if (count) {
+    const uint32_t C = hp | (H << 4);
+    uint32_t ones = 0;
+#pragma unroll
+    for (int r = 0; r < kBatchTY; ++r) {
+      const uint32_t code = (C >> (8 * r)) & 255u;
+      ones += (uint32_t)__builtin_popcountll(ballot(code == 255u) & vr[r]);
+      if (laneBit(vr[r]) && code - 1u < 254u)
+        __hip_atomic_fetch_add(histL + code, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && ones) atomicAdd(histL + 255, ones);
+  }
+  return H;
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t levelDispatch(uint32_t m, const T (&cur)[kBatchTY + 1],
+                                                  const T (&rgt)[kBatchTY + 1], T thr, const uint64_t (&vr)[kBatchTY],
+                                                  bool count, uint32_t hp, uint32_t* histL) {
+  switch (m) {
+    case 1: return levelStep<1>(cur, rgt, thr, vr, count, hp, histL);
+    case 2: return levelStep<2>(cur, rgt, thr, vr, count, hp, histL);
+    case 3: return levelStep<3>(cur, rgt, thr, vr, count, hp, histL);
+    case 4: return levelStep<4>(cur, rgt, thr, vr, count, hp, histL);
+    case 5: return levelStep<5>(cur, rgt, thr, vr, count, hp, histL);
+    case 6: return levelStep<6>(cur, rgt, thr, vr, count, hp, histL);
+    case 7: return levelStep<7>(cur, rgt, thr, vr, count, hp, histL);
+    default: return levelStep<0>(cur, rgt, thr, vr, count, hp, histL);
+  }
+}
+
+// One work item: the wave's stack of rows over the item's slab, all levels.
+template <typename T>
+__device__ void batchItem(BatchShared& sh, const BatchField& fd, const BatchItem it) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int X = fd.X, Y = fd.Y;
+  const int nL = fd.nLevels;
+  const int segs = fd.segs;
+  const int seg = w % segs;
+  const int stack = w / segs;
+  const bool active = stack < fd.stacks;
+  const int x0 = (it.xt * segs + seg) * kWave;
+  const int y0 = (it.yb * fd.stacks + (active ? stack : 0)) * kBatchTY;
+  const int nY = active ? max(0, min(kBatchTY, Y - 1 - y0)) : 0;  // 0: the stack counts nothing
+  const int yc = min(y0, Y - 1);
+  const int z0 = it.zr * fd.TZ;
+  const int nZ = min(fd.TZ, fd.Z - 1 - z0);
+  HaloFrom halo = kHaloNone;
+  if (nY > 0 && x0 + kWave < X) halo = (seg + 1 < segs) ? kHaloLds : kHaloMem;
+
+  const int xb = min(x0, X - 1);
+  const T* fw = (const T*)fd.p + xb;
+  const long long sy = fd.sy, sz = fd.sz;
+  const uint32_t lx = (uint32_t)(min(x0 + lane, X - 1) - xb);
+  const int hxo = min(x0 + kWave, X - 1) - xb;
+  const uint64_t valid = ballot(x0 + lane < X - 1);
+  if (lane == 0 && nY > 0) atomicAdd(&sh.cells, (uint32_t)__builtin_popcountll(valid) * (uint32_t)(nY * nZ));
+  uint64_t vr[kBatchTY];
+#pragma unroll
+  for (int r = 0; r < kBatchTY; ++r) vr[r] = r < nY ? valid : 0ull;
+
+  // Lane l holds level l's threshold and relation; the level loop reads them
+  // with readlane.
+  const int ll = min(lane, nL - 1);
+  const T tl = (T)fd.t[ll];
+  const uint32_t ml = fd.m[ll];
+  const bool last = lane == kWave - 1;
+
+  const int nb = min(w + 1, kBatchWaves - 1);
+  long long yo[kBatchTY + 1];
+#pragma unroll
+  for (int r = 0; r <= kBatchTY; ++r) yo[r] = (long long)(yc + min(r, nY)) * sy;
+  T cur[kBatchTY + 1], nxt[kBatchTY + 1];
+  T hc[kBatchTY + 1], hn[kBatchTY + 1];
+#pragma unroll
+  for (int r = 0; r <= kBatchTY; ++r) {
+    cur[r] = globalLoad(fw, yo[r] + (long long)z0 * sz + lx);
+    hc[r] = (halo == kHaloMem) ? uniformLoad(fw + (yo[r] + (long long)z0 * sz + hxo)) : T(0);
+  }
+  uint32_t* hpk = &sh.hpk[w][0][lane];
+  for (int z = z0; z <= z0 + nZ; ++z) {
+    const long long zon = (long long)min(z + 1, z0 + nZ) * sz;
+#pragma unroll
+    for (int r = 0; r <= kBatchTY; ++r) {
+      nxt[r] = globalLoad(fw, yo[r] + zon + lx);
+      hn[r] = (halo == kHaloMem) ? uniformLoad(fw + (yo[r] + zon + hxo)) : T(0);
+    }
+    const int par = z & 1;
+    if (segs > 1) {
+      if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r <= kBatchTY; ++r) sh.xv[par][r][w] = (double)cur[r];
+      }
+      __syncthreads();
+    }
+    // Right neighbours: lane l+1's voxel, and the halo voxel for lane 63.
+    T rgt[kBatchTY + 1];
+#pragma unroll
+    for (int r = 0; r <= kBatchTY; ++r) {
+      const T hv = (halo == kHaloLds) ? (T)sh.xv[par][r][nb] : hc[r];
+      const T n1 = laneNext(cur[r]);
+      rgt[r] = last ? hv : n1;
+    }
+    const bool count = z > z0;
+    uint32_t hpNext = hpk[0];
+    for (int l = 0; l < nL; ++l) {
+      const uint32_t hp = hpNext;
+      if (l + 1 < nL) hpNext = hpk[(l + 1) * kWave];
+      const T thr = readLane(tl, l);
+      const uint32_t m = readLane(ml, l);
+      hpk[l * kWave] = levelDispatch<T>(m, cur, rgt, thr, vr, count, hp, sh.hist[l]);
+    }
+#pragma unroll
+    for (int r = 0; r <= kBatchTY; ++r) {
+      cur[r] = nxt[r];
+      hc[r] = hn[r];
+    }
+  }
+}
+
+__global__ void kBatchReset(unsigned int* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+}
+
+__global__ void kZeroList(unsigned long long* const* outs, const int* counts, int n) {
+  for (int f = blockIdx.x; f < n; f += gridDim.x)
+    for (int i = threadIdx.x; i < counts[f]; i += blockDim.x) outs[f][i] = 0ull;
+}
+
+__global__ __launch_bounds__(kBatchWaves* kWave) void kBatch(const BatchField* __restrict__ fields,
+                                                              const BatchItem* __restrict__ items, int nItems,
+                                                              unsigned int* __restrict__ counter) {
+  __shared__ BatchShared sh;
+  for (int l = 0; l < kMaxLevels; ++l)
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) sh.hist[l][i] = 0;
+  if (threadIdx.x < kMaxLevels) sh.msum[threadIdx.x] = 0;
+  if (threadIdx.x == 0) sh.cells = 0;
+  int curField = -1;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) sh.item = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int i = __builtin_amdgcn_readfirstlane(sh.item);
+    const int f = i < nItems ? items[i].field : -1;
+    const bool full = sh.cells > 0x80000000u;
+    if ((f != curField || full) && curField >= 0) batchFlush(sh, fields[curField]);
+    if (i >= nItems) break;
+    curField = f;
+    const BatchField& fd = fields[f];
+    if (fd.enc == ENC_F32) {
+      batchItem<float>(sh, fd, items[i]);
+    } else {
+      batchItem<double>(sh, fd, items[i]);
+    }
+  }
+}
+
+}  // namespace mkf
+
+using namespace mkf;
+
+// ---------------------------------------------------------------------------
+// Host plan
+// ---------------------------------------------------------------------------
+struct MkfBatch {
+  std::vector<BatchField> host;
+  BatchField* dFields = nullptr;
+  BatchItem* dItems = nullptr;
+  size_t itemCap = 0;
+  unsigned int* dCounter = nullptr;
+  unsigned long long** dOuts = nullptr;
+  int* dCounts = nullptr;
+  int nItems = 0;
+  int grid = 0;
+  MkfBatchShape shape{32, 8};
+  int flags = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+};
+
+namespace {
+
+int bfail(int code, const char* msg) { return setError(code, "batch: %s", msg); }
+
+long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
+
+bool shapeOk(const MkfBatchShape& s) {
+  return s.planePairs >= 1 && s.planePairs <= kMaxPlanePairs && (s.segments == 1 || s.segments == 2 || s.segments == 4 || s.segments == 8);
+}
+
+MkfBatchShape normalise(const MkfBatchShape* s) {
+  MkfBatchShape r{32, 8};
+  if (s) {
+    r.planePairs = s->planePairs == 0 ? 32 : s->planePairs;
+    r.segments = s->segments == 0 ? 8 : s->segments;
+  }
+  return r;
+}
+
+// Tile shape of one field: the largest power-of-two x-tile up to the shape's
+// limit that leaves the fewest empty segments at the ends of the rows.
+void shapeField(BatchField& b, const MkfBatchShape& s) {
+  const int nSeg = (int)std::max<long long>(1, cdiv(b.X - 1, kWave));
+  int best = 1;
+  long long bestWaste = 1LL << 40;
+  for (int g = s.segments; g >= 1; g >>= 1) {
+    const long long waste = cdiv(nSeg, g) * g - nSeg;
+    if (waste < bestWaste) {
+      bestWaste = waste;
+      best = g;
+    }
+  }
+  b.segs = best;
+  b.stacks = kBatchWaves / best;
+  b.TZ = s.planePairs;
+  b.nXT = (int)cdiv(nSeg, best);
+  b.nYB = (int)cdiv(cdiv(std::max(b.Y - 1, 1), kBatchTY), b.stacks);
+  b.nZR = (int)cdiv(std::max(b.Z - 1, 1), b.TZ);
+}
+
+// Items sorted by field, so a workgroup flushes when it moves to the next
+// field. Fields without cells (a dim of 1) have no items.
+void buildItems(const std::vector<BatchField>& fs, std::vector<BatchItem>& items) {
+  items.clear();
+  for (int f = 0; f < (int)fs.size(); ++f) {
+    const BatchField& b = fs[f];
+    if (b.X < 2 || b.Y < 2 || b.Z < 2) continue;
+    for (int zr = 0; zr < b.nZR; ++zr)
+      for (int yb = 0; yb < b.nYB; ++yb)
+        for (int xt = 0; xt < b.nXT; ++xt) items.push_back(BatchItem{f, xt, yb, zr});
+  }
+}
+
+int upload(MkfBatch* B, const MkfBatchShape& s, hipStream_t st) {
+  for (auto& b : B->host) shapeField(b, s);
+  std::vector<BatchItem> items;
+  buildItems(B->host, items);
+  if (items.size() > (size_t)0x7fffffff - 1) return bfail(MKF_ERR_UNSUPPORTED, "too many work items");
+  if (items.size() > B->itemCap) {
+    if (B->dItems) hipFree(B->dItems);
+    B->dItems = nullptr;
+    B->itemCap = 0;
+    if (hipMalloc((void**)&B->dItems, sizeof(BatchItem) * items.size()) != hipSuccess)
+      return bfail(MKF_ERR_MEM, "work queue allocation");
+    B->itemCap = items.size();
+  }
+  B->nItems = (int)items.size();
+  B->shape = s;
+  if (hipMemcpyAsync(B->dFields, B->host.data(), sizeof(BatchField) * B->host.size(), hipMemcpyHostToDevice, st) !=
+          hipSuccess ||
+      (!items.empty() && hipMemcpyAsync(B->dItems, items.data(), sizeof(BatchItem) * items.size(),
+                                        hipMemcpyHostToDevice, st) != hipSuccess))
+    return bfail(MKF_ERR_HIP, "upload of the batch description");
+  return hipStreamSynchronize(st) == hipSuccess ? MKF_OK : bfail(MKF_ERR_HIP, "upload sync");
+}
+
+// Records the batch into stream s: queue reset (a kernel, so a captured
+// graph has no memset node), optional output clearing, then the persistent
+// kernel.
+hipError_t record(MkfBatch* B, hipStream_t s) {
+  hipLaunchKernelGGL(kBatchReset, dim3(1), dim3(64), 0, s, B->dCounter);
+  if (B->flags & MKF_BATCH_ZERO_OUTPUT)
+    hipLaunchKernelGGL(kZeroList, dim3(std::min<int>(1024, (int)B->host.size())), dim3(256), 0, s, B->dOuts,
+                       B->dCounts, (int)B->host.size());
+  if (B->nItems > 0)
+    hipLaunchKernelGGL(kBatch, dim3(B->grid), dim3(kBatchWaves * kWave), 0, s, B->dFields, B->dItems, B->nItems,
+                       B->dCounter);
+  return hipGetLastError();
+}
+
+int capture(MkfBatch* B) {
+  if (B->exec) hipGraphExecDestroy(B->exec);
+  if (B->graph) hipGraphDestroy(B->graph);
+  B->exec = nullptr;
+  B->graph = nullptr;
+  if (hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return bfail(MKF_ERR_HIP, "stream capture begin");
+  const hipError_t e = record(B, B->cap);
+  if (hipStreamEndCapture(B->cap, &B->graph) != hipSuccess || e != hipSuccess)
+    return bfail(MKF_ERR_HIP, "stream capture");
+  if (hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0) != hipSuccess)
+    return bfail(MKF_ERR_HIP, "graph instantiation");
+  return MKF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+MkfBatch* mkfBatchCreate(const MkfBatchField* fields, int nFields, const MkfBatchShape* shape, int flags,
+                         void* stream) {
+  if (!fields || nFields <= 0) {
+    bfail(MKF_ERR_ARG, "no fields");
+    return nullptr;
+  }
+  const MkfBatchShape s = normalise(shape);
+  if (!shapeOk(s)) {
+    bfail(MKF_ERR_ARG, "shape: planePairs in 1..4096 and segments in {1, 2, 4, 8}");
+    return nullptr;
+  }
+  std::vector<BatchField> host;
+  std::vector<unsigned long long*> outs;
+  std::vector<int> counts;
+  for (int f = 0; f < nFields; ++f) {
+    const MkfBatchField& src = fields[f];
+    BatchField b;
+    memset(&b, 0, sizeof(b));
+    FieldDef d[3];
+    const int nd = copyNonZeroDef(d, src.def, 3);
+    if (!src.pF || !src.pBPFD || (src.enc != ENC_F32 && src.enc != ENC_F64) || src.nLevels < 1 ||
+        src.nLevels > kMaxLevels || !src.pM || nd <= 0 || src.def[0] < 0 || src.def[1] < 0 || src.def[2] < 0) {
+      setError(MKF_ERR_ARG, "batch: invalid description of field %d", f);
+      return nullptr;
+    }
+    b.p = src.pF;
+    b.out = (unsigned long long*)src.pBPFD;
+    b.enc = src.enc;
+    b.X = d[0];
+    b.Y = d[1];
+    b.Z = d[2];
+    b.sy = d[0];
+    b.sz = (long long)d[0] * d[1];
+    b.nLevels = src.nLevels;
+    for (int l = 0; l < src.nLevels; ++l) {
+      // f32 fields compare against the threshold rounded to float.
+      b.t[l] = (src.enc == ENC_F32) ? (double)(float)src.pM[l].t[0] : src.pM[l].t[0];
+      b.m[l] = src.pM[l].m & 7u;
+    }
+    host.push_back(b);
+    outs.push_back(b.out);
+    counts.push_back(256 * src.nLevels);
+  }
+  MkfBatch* B = new MkfBatch();
+  B->flags = flags;
+  B->host.swap(host);
+  hipStream_t st = (hipStream_t)stream;
+  int dev = 0, cus = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  int perCU = 1;
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kBatch, kBatchWaves * kWave, 0);
+  B->grid = std::max(1, cus * std::max(1, perCU));
+  if (hipMalloc((void**)&B->dFields, sizeof(BatchField) * nFields) != hipSuccess ||
+      hipMalloc((void**)&B->dCounter, sizeof(unsigned int)) != hipSuccess ||
+      hipMalloc((void**)&B->dOuts, sizeof(void*) * nFields) != hipSuccess ||
+      hipMalloc((void**)&B->dCounts, sizeof(int) * nFields) != hipSuccess ||
+      hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking) != hipSuccess) {
+    bfail(MKF_ERR_MEM, "batch allocation");
+    mkfBatchDestroy(B);
+    return nullptr;
+  }
+  int rc = MKF_OK;
+  if (hipMemcpyAsync(B->dOuts, outs.data(), sizeof(void*) * nFields, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(B->dCounts, counts.data(), sizeof(int) * nFields, hipMemcpyHostToDevice, st) != hipSuccess)
+    rc = bfail(MKF_ERR_HIP, "upload of the output table");
+  if (rc == MKF_OK) rc = upload(B, s, st);
+  if (rc == MKF_OK) rc = capture(B);
+  if (rc != MKF_OK) {
+    mkfBatchDestroy(B);
+    return nullptr;
+  }
+  return B;
+}
+
+int mkfBatchRun(MkfBatch* B, void* stream) {
+  if (!B || !B->exec) return bfail(MKF_ERR_ARG, "no batch");
+  const hipError_t e = hipGraphLaunch(B->exec, (hipStream_t)stream);
+  return e == hipSuccess ? MKF_OK : bfail(MKF_ERR_HIP, "graph launch");
+}
+
+int mkfBatchRunDirect(MkfBatch* B, void* stream) {
+  if (!B) return bfail(MKF_ERR_ARG, "no batch");
+  return record(B, (hipStream_t)stream) == hipSuccess ? MKF_OK : bfail(MKF_ERR_HIP, "batch launch");
+}
+
+int mkfBatchAutotune(MkfBatch* B, const MkfBatchShape* candidates, int nCandidates, int reps,
+                     MkfBatchShape* chosen, void* stream) {
+  if (!B || !candidates || nCandidates <= 0) return bfail(MKF_ERR_ARG, "no candidates");
+  hipStream_t st = (hipStream_t)stream;
+  // Point every field at a scratch histogram while tuning.
+  size_t total = 0;
+  for (auto& b : B->host) total += (size_t)b.nLevels * 256;
+  unsigned long long* scratch = nullptr;
+  if (hipMalloc((void**)&scratch, total * 8) != hipSuccess) return bfail(MKF_ERR_MEM, "tuning scratch");
+  std::vector<unsigned long long*> saved;
+  size_t off = 0;
+  for (auto& b : B->host) {
+    saved.push_back(b.out);
+    b.out = scratch + off;
+    off += (size_t)b.nLevels * 256;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  MkfBatchShape best{0, 0};
+  float bestMs = 1e30f;
+  const int saveFlags = B->flags;
+  B->flags &= ~MKF_BATCH_ZERO_OUTPUT;
+  int rc = MKF_OK;
+  for (int c = 0; c < nCandidates && rc == MKF_OK; ++c) {
+    const MkfBatchShape s = normalise(&candidates[c]);
+    if (!shapeOk(s)) continue;
+    if ((rc = upload(B, s, st)) != MKF_OK) break;
+    record(B, st);  // warm-up
+    hipEventRecord(e0, st);
+    for (int r = 0; r < std::max(1, reps); ++r) record(B, st);
+    hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      rc = bfail(MKF_ERR_HIP, "tuning run");
+      break;
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (ms < bestMs) {
+      bestMs = ms;
+      best = s;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  for (size_t f = 0; f < B->host.size(); ++f) B->host[f].out = saved[f];
+  B->flags = saveFlags;
+  hipFree(scratch);
+  if (rc != MKF_OK) return rc;
+  if (best.planePairs <= 0) return bfail(MKF_ERR_ARG, "no usable candidate");
+  if ((rc = upload(B, best, st)) != MKF_OK || (rc = capture(B)) != MKF_OK) return rc;
+  if (chosen) *chosen = best;
+  return MKF_OK;
+}
+
+int mkfBatchInfo(const MkfBatch* B, int* nItems, MkfBatchShape* shape, int* grid) {
+  if (!B) return bfail(MKF_ERR_ARG, "no batch");
+  if (nItems) *nItems = B->nItems;
+  if (shape) *shape = B->shape;
+  if (grid) *grid = B->grid;
+  return MKF_OK;
+}
+
+void mkfBatchDestroy(MkfBatch* B) {
+  if (!B) return;
+  if (B->exec) hipGraphExecDestroy(B->exec);
+  if (B->graph) hipGraphDestroy(B->graph);
+  if (B->cap) hipStreamDestroy(B->cap);
+  if (B->dFields) hipFree(B->dFields);
+  if (B->dItems) hipFree(B->dItems);
+  if (B->dCounter) hipFree(B->dCounter);
+  if (B->dOuts) hipFree(B->dOuts);
+  if (B->dCounts) hipFree(B->dCounts);
+  delete B;
+}
+
+}  // extern "C"

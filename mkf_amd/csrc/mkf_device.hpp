@@ -63,6 +63,29 @@ __device__ __forceinline__ uint64_t relMaskDyn(T f, T t, uint32_t m) {
 // Lane l reads bit l of a wave-uniform 64-bit mask (one v_cndmask).
 __device__ __forceinline__ bool laneBit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 
+// Lane l receives lane l+1's value through DPP (wave_shl:1, one v_mov per
+// dword); lane 63, which has no source, keeps its own value.
+__device__ __forceinline__ uint32_t laneNext(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x130, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float laneNext(float v) { return __uint_as_float(laneNext(__float_as_uint(v))); }
+__device__ __forceinline__ double laneNext(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint64_t lo = laneNext((uint32_t)b), hi = laneNext((uint32_t)(b >> 32));
+  return __longlong_as_double((long long)(lo | (hi << 32)));
+}
+
+// Wave-uniform read of lane `l` of v (v_readlane; l is uniform).
+__device__ __forceinline__ uint32_t readLane(uint32_t v, int l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ float readLane(float v, int l) { return __uint_as_float(readLane(__float_as_uint(v), l)); }
+__device__ __forceinline__ double readLane(double v, int l) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint64_t lo = readLane((uint32_t)b, l), hi = readLane((uint32_t)(b >> 32), l);
+  return __longlong_as_double((long long)(lo | (hi << 32)));
+}
+
 __device__ __forceinline__ uint64_t lowBits(int n) {
   return n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
 }
@@ -77,6 +100,14 @@ __device__ __forceinline__ T uniformLoad(const T* p) {
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
   typedef const __attribute__((address_space(4))) T* ConstPtr;
   return *(ConstPtr)((((uint64_t)hi) << 32) | lo);
+}
+
+// Per-lane load through a pointer known to be in global memory (a pointer
+// read from a descriptor is generic and would otherwise load with flat_load).
+template <typename T>
+__device__ __forceinline__ T globalLoad(const T* p, long long i) {
+  typedef const __attribute__((address_space(1))) T* GlobalPtr;
+  return ((GlobalPtr)p)[i];
 }
 
 // Byte-packed pair words. A lane holds V consecutive voxels (V = 1, 2 or 4);

@@ -27,6 +27,10 @@ struct Tiling {
   int V;   // voxels per lane: 1, or 16 bytes' worth (4 floats / 2 doubles)
 };
 
+// Records the text of an error for mkfLastError() (this thread) and returns
+// code.
+int setError(int code, const char* fmt, ...);
+
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes);
 bool tilingSupported(const Tiling& tl, int elemBytes);
 

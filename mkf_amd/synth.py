@@ -11,6 +11,10 @@ the same bytes go to the GPU kernels and to the CPU checker.
                  variance in float64; float32 is the rounded float64 field.
                  On the GPU torch generates it; on the CPU numpy does (the
                  two give different fields for one seed)
+  grf_field_direct  the same kind of field by direct periodic convolution with
+                 a truncated Gaussian (radius ceil(4 sigma), normalised), one
+                 axis at a time; used for the many odd-sized fields of the
+                 batch mix, where an FFT plan per size would dominate
   sdf_field      signed distance to a union of spheres and axis-aligned boxes
 """
 
@@ -128,3 +132,45 @@ def sdf_field(shape, seed=0, device="cpu", dtype=torch.float32, zchunk=32, **kw)
             d = torch.minimum(d, outside + inside)
         out[z0:z1] = d.to(dtype)
     return out
+
+
+def batch_mix(seed=4, n=64, dim_range=(32, 384), sigma_range=(1.0, 6.0)):
+    """The uneven batch: n fields, each axis dim uniform in dim_range
+    (inclusive), half float32 and half float64 (a seeded permutation), GRF
+    blur sigma uniform in sigma_range. Returns a list of
+    (shape (Z, Y, X), torch dtype, sigma, field seed)."""
+    rng = np.random.default_rng(seed)
+    dims = rng.integers(dim_range[0], dim_range[1] + 1, size=(n, 3))
+    sig = rng.uniform(sigma_range[0], sigma_range[1], size=n)
+    f64 = np.zeros(n, dtype=bool)
+    f64[rng.permutation(n)[: n // 2]] = True
+    return [(tuple(int(v) for v in dims[i]), torch.float64 if f64[i] else torch.float32, float(sig[i]),
+             int(seed) * 1000 + i) for i in range(n)]
+
+
+def grf_field_direct(shape, sigma, seed=0, device="cpu", dtype=torch.float64):
+    """Gaussian random field (Z, Y, X): seeded N(0,1) noise blurred along each
+    axis by periodic convolution with exp(-d^2 / (2 sigma^2)), |d| <= ceil(4
+    sigma), normalised to sum 1; then standardised in float64 (float32 is the
+    rounded float64 field)."""
+    Z, Y, X = (int(s) for s in shape)
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    f = torch.randn((Z, Y, X), generator=g, dtype=torch.float64, device=device)
+    R = int(math.ceil(4.0 * float(sigma)))
+    d = torch.arange(-R, R + 1, dtype=torch.float64)
+    k = torch.exp(-0.5 * (d / float(sigma)) ** 2)
+    k = (k / k.sum()).tolist()
+    for axis in range(3):
+        out = torch.zeros_like(f)
+        for i, w in enumerate(k):
+            out.add_(torch.roll(f, shifts=i - R, dims=axis), alpha=w)
+        f = out
+    f -= f.mean()
+    f /= torch.sqrt((f * f).mean())
+    return f if dtype == torch.float64 else f.to(dtype)
+
+
+def make_fields(specs, device):
+    """Generates the fields of batch_mix specs (grf_field_direct)."""
+    return [grf_field_direct(shape, sigma, s, device=device, dtype=dtype) for shape, dtype, sigma, s in specs]

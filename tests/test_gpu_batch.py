@@ -1,0 +1,198 @@
+"""Batched form: many fields x many threshold levels in one persistent,
+graph-captured launch. Every histogram equals the oracle's for that field and
+level, bit for bit, and equals the single-field fused kernel's."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import mkf_amd as M
+from mkf_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+RELOPS = [">=", "<", "<=", ">", "=", "!=", "", "!"]
+
+
+def make_batch(cuda, specs, seed=0):
+    """specs: list of ((Z, Y, X), dtype, levels[(relop, t)]) -> fields, maps, outputs."""
+    fields, maps, outs, arr = [], [], [], (M.MkfBatchField * len(specs))()
+    for i, (shape, dtype, levels) in enumerate(specs):
+        f = synth.grf_field(shape, sigma=1.5, seed=seed + i, device=cuda, dtype=dtype)
+        fields.append(f)
+        bms = (M.BinMapF64 * len(levels))()
+        for j, (rel, t) in enumerate(levels):
+            bms[j] = M.bin_map(rel, t)
+        maps.append(bms)
+        out = torch.zeros(len(levels) * 256, dtype=torch.int64, device=cuda)
+        outs.append(out)
+        Z, Y, X = shape
+        arr[i].pF = f.data_ptr()
+        arr[i].enc = M.ENC_F32 if dtype == torch.float32 else M.ENC_F64
+        arr[i].def_[0], arr[i].def_[1], arr[i].def_[2] = X, Y, Z
+        arr[i].nLevels = len(levels)
+        arr[i].pM = C.cast(bms, C.POINTER(M.BinMapF64))
+        arr[i].pBPFD = out.data_ptr()
+    return fields, maps, outs, arr
+
+
+def oracle_hists(orc, fields, maps):
+    res = []
+    for f, bms in zip(fields, maps):
+        f_np = f.cpu().numpy()
+        hs = []
+        for bm in bms:
+            words = orc.pbm_c(f_np, bm.t[0], bm.m)
+            X = f_np.shape[2]
+            hs.append(orc.bpfd_chunk(words, X) if X >= 32 else orc.bpfd_direct_c(words, X))
+        res.append(np.stack(hs))
+    return res
+
+
+def got(out, n):
+    return out.cpu().numpy().view(np.uint64).reshape(n, 256)
+
+
+SPECS = [
+    ((9, 11, 2), torch.float32, [(">=", 0.0)]),
+    ((7, 5, 33), torch.float64, [(">=", -0.5), ("<", 0.3), ("!=", 0.0)]),
+    ((12, 10, 65), torch.float32, [(r, 0.1 * k) for k, r in enumerate(RELOPS)]),
+    ((20, 17, 129), torch.float64, [(">=", t) for t in np.linspace(-1.5, 1.5, 8)]),
+    ((6, 40, 300), torch.float32, [("<=", t) for t in np.linspace(-1.0, 1.0, 16)]),
+    ((30, 9, 512), torch.float64, [(">", 0.2), ("=", 0.0)]),
+    ((2, 2, 64), torch.float32, [(">=", 0.0)] * 2),
+    ((41, 3, 100), torch.float32, [(">=", 0.25)]),
+    ((5, 7, 1030), torch.float32, [(">=", -0.3), ("<", 0.3), ("!=", 0.1)]),
+    ((11, 13, 700), torch.float64, [("<=", 0.0)]),
+    ((1, 6, 50), torch.float32, [(">=", 0.0)]),
+]
+
+
+def bshape(pp, segs):
+    s = M.MkfBatchShape()
+    s.planePairs, s.segments = pp, segs
+    return s
+
+
+@pytest.mark.parametrize("depth,segs", [(1, 8), (3, 2), (32, 1), (7, 4), (0, 0)])
+def test_batch_matches_oracle_and_graph_replays(lib, orc, cuda, depth, segs):
+    fields, maps, outs, arr = make_batch(cuda, SPECS)
+    ref = oracle_hists(orc, fields, maps)
+    b = lib.mkfBatchCreate(arr, len(SPECS), C.byref(bshape(depth, segs)), M.MKF_BATCH_ZERO_OUTPUT, None)
+    assert b, M.last_error()
+    try:
+        for rep in range(3):
+            # eager work between replays, on the same stream
+            for o in outs:
+                o.add_(7)
+            M.check(lib.mkfBatchRun(b, None), "mkfBatchRun")
+            torch.cuda.synchronize()
+            for i, (o, bms) in enumerate(zip(outs, maps)):
+                np.testing.assert_array_equal(got(o, len(bms)), ref[i], err_msg=f"replay {rep} field {i}")
+        M.check(lib.mkfBatchRunDirect(b, None), "direct")
+        torch.cuda.synchronize()
+        for i, (o, bms) in enumerate(zip(outs, maps)):
+            np.testing.assert_array_equal(got(o, len(bms)), ref[i])
+    finally:
+        lib.mkfBatchDestroy(b)
+
+
+def test_batch_accumulates_without_zero_flag(lib, orc, cuda):
+    fields, maps, outs, arr = make_batch(cuda, SPECS[1:5], seed=10)
+    ref = oracle_hists(orc, fields, maps)
+    b = lib.mkfBatchCreate(arr, 4, C.byref(bshape(8, 8)), 0, None)
+    assert b
+    try:
+        for _ in range(2):
+            M.check(lib.mkfBatchRun(b, None))
+        torch.cuda.synchronize()
+        for i, (o, bms) in enumerate(zip(outs, maps)):
+            np.testing.assert_array_equal(got(o, len(bms)), 2 * ref[i])
+    finally:
+        lib.mkfBatchDestroy(b)
+
+
+def test_batch_equals_fused_per_field(lib, cuda):
+    specs = [((24, 30, 200), torch.float32, [(">=", t) for t in (-0.5, 0.0, 0.5)]),
+             ((16, 20, 384), torch.float64, [("<", t) for t in (-1.0, 1.0)])]
+    fields, maps, outs, arr = make_batch(cuda, specs, seed=20)
+    b = lib.mkfBatchCreate(arr, 2, None, M.MKF_BATCH_ZERO_OUTPUT, None)
+    assert b
+    try:
+        grid = [(4, 8), (16, 2), (64, 1), (16, 0)]
+        cands = (M.MkfBatchShape * len(grid))(*[bshape(*g) for g in grid])
+        best = M.MkfBatchShape()
+        M.check(lib.mkfBatchAutotune(b, cands, len(grid), 2, C.byref(best), None), "autotune")
+        assert (best.planePairs, best.segments) in [(4, 8), (16, 2), (64, 1), (16, 8)]
+        n = C.c_int(0)
+        info = M.MkfBatchShape()
+        g = C.c_int(0)
+        M.check(lib.mkfBatchInfo(b, C.byref(n), C.byref(info), C.byref(g)))
+        assert (info.planePairs, info.segments) == (best.planePairs, best.segments)
+        assert n.value > 0 and g.value > 0
+        M.check(lib.mkfBatchRun(b, None))
+        torch.cuda.synchronize()
+        for f, bms, o in zip(fields, maps, outs):
+            Z, Y, X = f.shape
+            for j in range(len(bms)):
+                h = torch.zeros(256, dtype=torch.int64, device=cuda)
+                fi = M.FieldInfo(f.data_ptr(), (X, Y, Z), M.ENC_F32 if f.dtype == torch.float32 else M.ENC_F64)
+                M.check(lib.mkfFusedBPFD(None, h.data_ptr(), None, fi.ref, C.byref(bms[j]), None, None))
+                torch.cuda.synchronize()
+                np.testing.assert_array_equal(got(o, len(bms))[j], h.cpu().numpy().view(np.uint64))
+    finally:
+        lib.mkfBatchDestroy(b)
+
+
+def test_batch_rejects_bad_arguments(lib, cuda):
+    _, _, _, arr = make_batch(cuda, [((4, 4, 40), torch.float32, [(">=", 0.0)])])
+    for bad in [(8, 3), (5000, 8), (-1, 8)]:
+        assert not lib.mkfBatchCreate(arr, 1, C.byref(bshape(*bad)), 0, None)
+        assert "shape" in M.last_error()
+    arr[0].nLevels = 17
+    assert not lib.mkfBatchCreate(arr, 1, None, 0, None)
+    assert "field 0" in M.last_error()
+    arr[0].nLevels = 1
+    assert not lib.mkfBatchCreate(arr, 0, None, 0, None)
+    arr[0].enc = 9
+    assert not lib.mkfBatchCreate(arr, 1, None, 0, None)
+
+
+def _special_field(shape, dtype, seed, levels):
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal(shape).astype(np.float64)
+    flat = f.reshape(-1)
+    n = flat.size
+    for vals, frac in (([np.nan], 0.04), ([np.inf, -np.inf], 0.02), ([0.0, -0.0], 0.03)):
+        idx = rng.choice(n, int(frac * n), replace=False)
+        flat[idx] = rng.choice(vals, idx.size)
+    idx = rng.choice(n, int(0.05 * n), replace=False)
+    flat[idx] = rng.choice([t for _, t in levels], idx.size)  # exactly on a threshold
+    if dtype == torch.float32:
+        # thresholds are passed as float, so values equal to the float threshold
+        f = f.astype(np.float32)
+    return torch.from_numpy(f)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_batch_special_values_every_relation(lib, orc, cuda, dtype):
+    levels = [(r, t) for r, t in zip(RELOPS + ["<", ">="], [0.0, 0.5, -0.5, 0.0, 0.25, 0.0, 1.0, -1.0, 0.1, 0.1])]
+    specs = [((9, 7, 130), dtype, levels), ((5, 6, 64), dtype, levels[:5]), ((4, 3, 70), dtype, levels[3:])]
+    fields, maps, outs, arr = make_batch(cuda, specs, seed=30)
+    for i, (shape, _, lv) in enumerate(specs):
+        fields[i].copy_(_special_field(shape, dtype, 40 + i, [(r, float(np.float32(t))) for r, t in lv]).to(cuda))
+    ref = oracle_hists(orc, fields, maps)
+    for segs in (1, 8):
+        for o in outs:
+            o.fill_(3)
+        b = lib.mkfBatchCreate(arr, len(specs), C.byref(bshape(5, segs)), M.MKF_BATCH_ZERO_OUTPUT, None)
+        assert b, M.last_error()
+        try:
+            M.check(lib.mkfBatchRun(b, None))
+            torch.cuda.synchronize()
+            for i, (o, bms) in enumerate(zip(outs, maps)):
+                np.testing.assert_array_equal(got(o, len(bms)), ref[i], err_msg=f"segments {segs} field {i}")
+        finally:
+            lib.mkfBatchDestroy(b)

@@ -49,6 +49,33 @@ def test_struct_layouts():
     assert M.Context.bytesF.offset == 16
 
 
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """The ctypes mirrors have the sizes and offsets gcc gives the header's structs."""
+    probes = {
+        "BinMapF64": ["t", "m"], "BMOrg": ["rowElem", "planePairs"], "BMFieldInfo": ["pD", "pFieldDevPtrTable"],
+        "Context": ["bytesF", "pDF", "bmo", "ki"], "MkfTiling": ["planePairs", "voxelsPerLane"],
+        "MkfBatchField": ["pF", "enc", "def", "nLevels", "pM", "pBPFD"],
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mkf.h"', "int main(void){"]
+    for s, fs in probes.items():
+        lines.append(f'printf("{s} %zu\\n", sizeof({s}));')
+        for f in fs:
+            lines.append(f'printf("{s}.{f} %zu\\n", offsetof({s}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                       text=True).stdout.splitlines())
+    for s, fs in probes.items():
+        cls = getattr(M, s)
+        assert int(got[s]) == C.sizeof(cls), s
+        for f in fs:
+            assert int(got[f"{s}.{f}"]) == getattr(cls, "def_" if f == "def" else f).offset, (s, f)
+
+
 @pytest.mark.parametrize("relop,m", [(">=", 6), ("<=", 3), ("!=", 5), ("<", 1), ("=", 2), (">", 4),
                                       ("", 0), ("!", 7), ("!<", 6), ("!>=", 1), ("<>", 5), (">=x<", 6),
                                       ("=!", 2), ("<<", 1)])
