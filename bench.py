@@ -173,6 +173,157 @@ def bench_batch(args, dev, stream):
     return res
 
 
+def bench_serve(args, dev):
+    """Request streams with Poisson arrivals at several rates through the
+    server (continuous batching, 64 GB budget), beside plain per-request
+    launches of the same requests in arrival order (copy in, one mkfFusedBPFD
+    per level, copy the histograms out, synchronise). Latency is completion
+    minus arrival; throughput is voxel-levels over first arrival to last
+    completion."""
+    import numpy as np
+    import torch
+
+    import mkf_amd as M
+    from mkf_amd import synth
+    from mkf_amd.serve import Server, poisson_arrivals, request_stream
+
+    lib = M.load()
+    NMAX = args.serve_requests
+    pool_spec, sim_spec, reqs = request_stream(NMAX, seed=args.serve_seed)
+    t0 = time.perf_counter()
+    pool = []
+    for i, (side, dt) in enumerate(pool_spec):
+        f = synth.grf_field_direct((side, side, side), 3.0, seed=7000 + i, device=dev, dtype=dt)
+        h = torch.empty(f.shape, dtype=dt, pin_memory=True)
+        h.copy_(f)
+        pool.append(h)
+        del f
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.serve_seed)
+    # Each simulation keeps a ring of 3 snapshots f(k+1) = f(k) + 0.01 noise;
+    # its requests take them in turn.
+    sims = []
+    for i, (side, dt) in enumerate(sim_spec):
+        ring = [synth.grf_field_direct((side, side, side), 3.0, seed=8000 + i, device=dev, dtype=dt)]
+        for _ in range(2):
+            ring.append(ring[-1] + 0.01 * torch.randn(ring[-1].shape, generator=g, device=dev, dtype=dt))
+        sims.append(ring)
+    fields, turn = [], [0] * len(sims)
+    for r in reqs:
+        if r["sim"] is None:
+            fields.append(pool[r["pool"]])
+        else:
+            s = r["sim"]
+            fields.append(sims[s][turn[s] % 3])
+            turn[s] += 1
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    log(f"[serve] {NMAX} requests over {len(pool)} pinned host fields and {len(sims)} simulations "
+        f"ready in {time.perf_counter() - t0:.1f}s")
+    work = [fields[i].numel() * len(r["levels"]) for i, r in enumerate(reqs)]
+    maps = [[M.bin_map(*lv) for lv in r["levels"]] for r in reqs]
+    stream = torch.cuda.current_stream()
+    sptr = C.c_void_p(stream.cuda_stream)
+    S = Server(budget=64 << 30, max_in_flight=64, max_batch=32)
+    for i in range(4):  # warm-up
+        S.submit(fields[i], reqs[i]["levels"], tag=10 ** 9 + i)
+    while S.outstanding:
+        S.poll(wait_us=10000)
+
+    def served(n, arr):
+        done, t_sub = {}, S.clock()
+        st0 = S.stats()
+        i = 0
+        while len(done) < n:
+            now = S.clock() - t_sub
+            while i < n and arr[i] <= now:
+                S.submit(fields[i], reqs[i]["levels"], tag=i)
+                i += 1
+            wait = min(0.002, arr[i] - now) if i < n else 0.002
+            for r in S.poll(wait_us=max(0, int(wait * 1e6))):
+                done[r["tag"]] = r
+        st1 = S.stats()
+        lat = np.array([done[k]["done"] - t_sub - arr[k] for k in range(n)])
+        span = max(done[k]["done"] for k in range(n)) - t_sub - arr[0]
+        parts = {"queue_ms": [done[k]["admit"] - t_sub - arr[k] for k in range(n)],
+                 "copy_ms": [done[k]["launch"] - done[k]["admit"] for k in range(n)],
+                 "run_ms": [done[k]["done"] - done[k]["launch"] for k in range(n)]}
+        return {"lat": lat, "span": span, "hist": [done[k]["hist"] for k in range(n)],
+                "batch": float(np.mean([done[k]["batch"] for k in range(n)])),
+                "bytes_in": st1["bytesIn"] - st0["bytesIn"],
+                "parts": {k: round(float(np.mean(v)) * 1e3, 3) for k, v in parts.items()}}
+
+    def plain(n, arr):
+        hist = torch.zeros((16, 256), dtype=torch.int64, device=dev)
+        out, lat = [], []
+        torch.cuda.synchronize()
+        t_sub = time.perf_counter()
+        bytes_in = 0
+        for i in range(n):
+            now = time.perf_counter() - t_sub
+            if now < arr[i]:
+                time.sleep(arr[i] - now)
+            f = fields[i]
+            if not f.is_cuda:
+                f = f.to(dev)
+                bytes_in += f.numel() * f.element_size()
+            Z, Y, X = f.shape
+            fi = M.FieldInfo(f.data_ptr(), (X, Y, Z), M.ENC_F32 if f.dtype == torch.float32 else M.ENC_F64)
+            L = len(maps[i])
+            hist[:L].zero_()
+            for j, bm in enumerate(maps[i]):
+                if lib.mkfFusedBPFD(None, hist[j].data_ptr(), None, fi.ref, C.byref(bm), None, sptr):
+                    raise RuntimeError(M.last_error())
+            out.append(hist[:L].cpu().numpy().view(np.uint64).copy())
+            lat.append(time.perf_counter() - t_sub - arr[i])
+        span = time.perf_counter() - t_sub - arr[0]
+        return {"lat": np.array(lat), "span": span, "hist": out, "bytes_in": bytes_in}
+
+    rows = []
+    for rate in args.serve_rates:
+        n = int(min(NMAX, max(40, rate * args.serve_seconds)))
+        arr = poisson_arrivals(n, rate, seed=args.serve_seed + int(rate))
+        arr = arr - arr[0]
+        sv = served(n, arr)
+        pl = plain(n, arr)
+        w = sum(work[:n])
+        same = all(np.array_equal(a, b) for a, b in zip(sv["hist"], pl["hist"]))
+        on_dev = np.array([reqs[k]["sim"] is not None for k in range(n)])
+
+        def pct(lat, sel):
+            v = lat[sel]
+            return {"p50_ms": round(float(np.percentile(v, 50)) * 1e3, 3),
+                    "p99_ms": round(float(np.percentile(v, 99)) * 1e3, 3)} if v.size else None
+        row = {"rate": rate, "requests": n,
+               "served": {"Gvox_levels_s": round(w / sv["span"] / 1e9, 2),
+                          "p50_ms": round(float(np.percentile(sv["lat"], 50)) * 1e3, 3),
+                          "p99_ms": round(float(np.percentile(sv["lat"], 99)) * 1e3, 3),
+                          "pcie_GBps": round(sv["bytes_in"] / sv["span"] / 1e9, 2), "mean_batch": round(sv["batch"], 2)},
+               "plain": {"Gvox_levels_s": round(w / pl["span"] / 1e9, 2),
+                         "p50_ms": round(float(np.percentile(pl["lat"], 50)) * 1e3, 3),
+                         "p99_ms": round(float(np.percentile(pl["lat"], 99)) * 1e3, 3),
+                         "pcie_GBps": round(pl["bytes_in"] / pl["span"] / 1e9, 2)},
+               "same_histograms": same,
+               "served_by_origin": {"device": pct(sv["lat"], on_dev), "host": pct(sv["lat"], ~on_dev)},
+               "plain_by_origin": {"device": pct(pl["lat"], on_dev), "host": pct(pl["lat"], ~on_dev)},
+               "served_mean_parts_ms": sv["parts"]}
+        row["speedup"] = round(row["served"]["Gvox_levels_s"] / row["plain"]["Gvox_levels_s"], 2)
+        log(f"[serve] {rate:6.0f}/s x{n}: served {row['served']['Gvox_levels_s']:8.1f} Gvox-levels/s "
+            f"p50 {row['served']['p50_ms']:9.2f} p99 {row['served']['p99_ms']:9.2f} ms (batch {sv['batch']:.1f}, "
+            f"PCIe {row['served']['pcie_GBps']:.1f} GB/s) | plain {row['plain']['Gvox_levels_s']:8.1f} "
+            f"p50 {row['plain']['p50_ms']:9.2f} p99 {row['plain']['p99_ms']:9.2f} ms | x{row['speedup']} same={same}")
+        log(f"[serve]          device-resident p50/p99 served {row['served_by_origin']['device']} plain "
+            f"{row['plain_by_origin']['device']}; served mean parts {sv['parts']}")
+        rows.append(row)
+    S.close()
+    del fields, sims, pool
+    torch.cuda.empty_cache()
+    return {"workload": f"Poisson arrivals; cube side log-uniform 64-768, f32:f64 3:1, 1-16 levels >= in [-1.5,1.5], "
+                        f"30% device-resident snapshots of {len(sim_spec)} simulations (f += 0.01 noise), host fields "
+                        f"pinned from a pool of {len(pool_spec)}; budget 64 GB; seed {args.serve_seed}",
+            "rates": rows}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,8 +338,13 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--tiling", type=str, default="",
                     help="planePairs,waves,rowPairs[,voxelsPerLane] (default: library choice)")
-    ap.add_argument("--mode", choices=["all", "fused", "batch"], default="all",
-                    help="fused: the 1024^3 kernel only; batch: the uneven batch only; all: both (default)")
+    ap.add_argument("--mode", choices=["all", "fused", "batch", "serve"], default="all",
+                    help="fused: the 1024^3 kernel only; batch: the uneven batch only; serve: the request "
+                         "streams only; all: everything (default)")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-seed", type=int, default=5)
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per rate (capped by requests)")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[50, 100, 200, 500, 1000, 2000])
     ap.add_argument("--batch-seed", type=int, default=4)
     ap.add_argument("--pmc-workload", action="store_true",
                     help="for counter runs: 3 fused launches, then 3 copies of a 1 GiB buffer (the calibration "
@@ -209,6 +365,10 @@ def main():
 
     lib = M.load()
     dev = torch.device("cuda:0")
+    if args.mode == "serve":
+        res = bench_serve(args, dev)
+        print(json.dumps({"metric": "served vs plain per-request launches, p50/p99 ms", "serve": res}), flush=True)
+        return
     if args.mode == "batch":
         res = bench_batch(args, dev, torch.cuda.current_stream())
         out = {"metric": "Gvox-levels/s batched uneven mix (64 fields x 8 levels)", "value": res["value"],
@@ -385,6 +545,7 @@ def main():
         del field
         torch.cuda.empty_cache()
         result["batched"] = bench_batch(args, dev, stream)
+        result["serve"] = bench_serve(args, dev)
     print(json.dumps(result), flush=True)
 
 

@@ -106,6 +106,73 @@ int mkfBatchAutotune(MkfBatch *b, const MkfBatchShape *candidates, int nCandidat
 int mkfBatchInfo(const MkfBatch *b, int *nItems, MkfBatchShape *shape, int *grid);
 void mkfBatchDestroy(MkfBatch *b);
 
+/* ---------------------------------------------------------------------------
+ * Serving form: a request loop with continuous batching
+ *
+ * A worker thread admits queued requests while fewer than maxInFlight are
+ * admitted: host requests in arrival order, while few copies are queued and
+ * the device bytes held for copied-in fields stay within hbmBudget;
+ * device-resident requests (no copy, no budget) ahead of host requests that
+ * wait. It copies host fields in on a copy stream and launches the batched
+ * kernel over every request whose field is on the device as soon as one of
+ * its two launch slots is free, so requests that arrive while others run
+ * join the next launch. Results
+ * (histograms written to the request's host buffer) are reported through
+ * mkfServePoll. No counterpart in the reference.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  size_t hbmBudget;     /* device bytes for copied-in fields; 0 = half the free memory at creation */
+  int32_t maxInFlight;  /* admitted, not yet reported requests; 0 = 64 */
+  int32_t maxBatch;     /* requests per launch; 0 = 32 */
+  MkfBatchShape shape;  /* work-item shape of every launch; zeros select the defaults */
+} MkfServeConfig;
+
+#define MKF_SERVE_ON_DEVICE 1u /* request flag: pF is device memory, read in place */
+#define MKF_SERVE_AFTER_STREAM 2u /* request flag: the field is read only after the work queued on
+                                     `stream` at submission (the producer's stream; NULL is the null stream) */
+
+typedef struct {
+  const void *pF;      /* host field (pinned memory overlaps its copy) or, with MKF_SERVE_ON_DEVICE, device field */
+  uint32_t flags;
+  void *stream;        /* with MKF_SERVE_AFTER_STREAM: the stream that produces the field */
+  NumEnc enc;          /* ENC_F32 or ENC_F64, planar (x fastest) */
+  FieldDef def[3];
+  int32_t nLevels;     /* 1 .. MKF_BATCH_MAX_LEVELS */
+  const BinMapF64 *pM; /* nLevels maps, copied at submission */
+  size_t *pBPFD;       /* host output: nLevels x 256 counts, written before the result is reported */
+  uint64_t tag;        /* returned in the result */
+} MkfServeRequest;
+
+typedef struct {
+  uint64_t tag;
+  int32_t status;      /* MKF_OK or an error code */
+  int32_t batchSize;   /* requests in the launch that served it */
+  double tSubmit, tAdmit, tLaunch, tDone; /* seconds on the server clock (mkfServeClock) */
+} MkfServeResult;
+
+typedef struct {
+  uint64_t submitted, completed, launches;
+  uint64_t bytesIn;    /* field bytes copied host to device */
+  uint64_t heldMax;    /* high-water mark of device bytes held for fields */
+  double gpuBusy;      /* summed launch-to-completion seconds */
+} MkfServeStats;
+
+typedef struct MkfServer MkfServer;
+
+/* Starts a server on the current device (cfg may be NULL). NULL on error. */
+MkfServer *mkfServeCreate(const MkfServeConfig *cfg);
+/* Queues a request; thread-safe and non-blocking. MKF_ERR_BUDGET if its
+ * field can never fit the budget. The field, and for a device field its
+ * memory, must stay valid and unchanged until the result is reported. */
+int mkfServeSubmit(MkfServer *s, const MkfServeRequest *r);
+/* Moves up to maxOut reported results to out, waiting up to waitMicros for
+ * the first. Returns the number moved, or < 0 on error. */
+int mkfServePoll(MkfServer *s, MkfServeResult *out, int maxOut, int waitMicros);
+double mkfServeClock(const MkfServer *s);
+int mkfServeGetStats(MkfServer *s, MkfServeStats *st);
+/* Finishes every queued request, stops the worker and frees the server. */
+void mkfServeDestroy(MkfServer *s);
+
 /* Read-only views of the measure tables used by mkfSelectMeasuresBPFD: the
  * symmetry group of each of the 256 patterns and the Euler weight of each of
  * the 22 groups (src/mkfUtil.c:33-58), for callers that verify them. */

@@ -77,6 +77,29 @@ class MkfBatchShape(C.Structure):
     _fields_ = [("planePairs", C.c_int32), ("segments", C.c_int32)]
 
 
+class MkfServeConfig(C.Structure):
+    _fields_ = [("hbmBudget", C.c_size_t), ("maxInFlight", C.c_int32), ("maxBatch", C.c_int32),
+                ("shape", MkfBatchShape)]
+
+
+class MkfServeRequest(C.Structure):
+    _fields_ = [("pF", C.c_void_p), ("flags", C.c_uint32), ("stream", C.c_void_p), ("enc", C.c_uint8),
+                ("def_", C.c_int * 3),
+                ("nLevels", C.c_int32), ("pM", C.POINTER(BinMapF64)), ("pBPFD", C.c_void_p), ("tag", C.c_uint64)]
+
+
+class MkfServeResult(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int32), ("batchSize", C.c_int32), ("tSubmit", C.c_double),
+                ("tAdmit", C.c_double), ("tLaunch", C.c_double), ("tDone", C.c_double)]
+
+
+class MkfServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("launches", C.c_uint64),
+                ("bytesIn", C.c_uint64), ("heldMax", C.c_uint64), ("gpuBusy", C.c_double)]
+
+
+MKF_SERVE_ON_DEVICE = 1
+MKF_SERVE_AFTER_STREAM = 2
 MKF_BATCH_MAX_LEVELS = 16
 MKF_BATCH_ZERO_OUTPUT = 1
 
@@ -121,6 +144,12 @@ _SIGNATURES = {
     "mkfBatchAutotune": (C.c_int, [_P, C.POINTER(MkfBatchShape), C.c_int, C.c_int, C.POINTER(MkfBatchShape), _P]),
     "mkfBatchInfo": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(MkfBatchShape), C.POINTER(C.c_int)]),
     "mkfBatchDestroy": (None, [_P]),
+    "mkfServeCreate": (_P, [C.POINTER(MkfServeConfig)]),
+    "mkfServeSubmit": (C.c_int, [_P, C.POINTER(MkfServeRequest)]),
+    "mkfServePoll": (C.c_int, [_P, C.POINTER(MkfServeResult), C.c_int, C.c_int]),
+    "mkfServeClock": (C.c_double, [_P]),
+    "mkfServeGetStats": (C.c_int, [_P, C.POINTER(MkfServeStats)]),
+    "mkfServeDestroy": (None, [_P]),
     "mkfPatternGroupTable": (C.POINTER(C.c_uint8), []),
     "mkfGroupWeightKTable": (C.POINTER(C.c_int8), []),
 }

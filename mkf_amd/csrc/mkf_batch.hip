@@ -259,6 +259,14 @@ __global__ void kBatchReset(unsigned int* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
 }
 
+// Copy of n 16-byte words by the compute units (either side may be mapped
+// pinned host memory), so small transfers on a compute stream never queue
+// behind bulk DMA transfers on other streams.
+__global__ void kCopyWords(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void kZeroList(unsigned long long* const* outs, const int* counts, int n) {
   for (int f = blockIdx.x; f < n; f += gridDim.x)
     for (int i = threadIdx.x; i < counts[f]; i += blockDim.x) outs[f][i] = 0ull;
@@ -297,25 +305,10 @@ __global__ __launch_bounds__(kBatchWaves* kWave) void kBatch(const BatchField* _
 using namespace mkf;
 
 // ---------------------------------------------------------------------------
-// Host plan
+// Host side: field descriptions, work items, the reusable launcher and the
+// graph-captured plan
 // ---------------------------------------------------------------------------
-struct MkfBatch {
-  std::vector<BatchField> host;
-  BatchField* dFields = nullptr;
-  BatchItem* dItems = nullptr;
-  size_t itemCap = 0;
-  unsigned int* dCounter = nullptr;
-  unsigned long long** dOuts = nullptr;
-  int* dCounts = nullptr;
-  int nItems = 0;
-  int grid = 0;
-  MkfBatchShape shape{32, 8};
-  int flags = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  hipStream_t cap = nullptr;
-};
-
+namespace mkf {
 namespace {
 
 int bfail(int code, const char* msg) { return setError(code, "batch: %s", msg); }
@@ -323,7 +316,8 @@ int bfail(int code, const char* msg) { return setError(code, "batch: %s", msg); 
 long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 
 bool shapeOk(const MkfBatchShape& s) {
-  return s.planePairs >= 1 && s.planePairs <= kMaxPlanePairs && (s.segments == 1 || s.segments == 2 || s.segments == 4 || s.segments == 8);
+  return s.planePairs >= 1 && s.planePairs <= kMaxPlanePairs &&
+         (s.segments == 1 || s.segments == 2 || s.segments == 4 || s.segments == 8);
 }
 
 MkfBatchShape normalise(const MkfBatchShape* s) {
@@ -333,6 +327,31 @@ MkfBatchShape normalise(const MkfBatchShape* s) {
     r.segments = s->segments == 0 ? 8 : s->segments;
   }
   return r;
+}
+
+// Validates field `index` and fills its device descriptor (without shape).
+int describeField(const MkfBatchField& src, int index, BatchField& b) {
+  memset(&b, 0, sizeof(b));
+  FieldDef d[3];
+  const int nd = copyNonZeroDef(d, src.def, 3);
+  if (!src.pF || !src.pBPFD || (src.enc != ENC_F32 && src.enc != ENC_F64) || src.nLevels < 1 ||
+      src.nLevels > kMaxLevels || !src.pM || nd <= 0 || src.def[0] < 0 || src.def[1] < 0 || src.def[2] < 0)
+    return setError(MKF_ERR_ARG, "batch: invalid description of field %d", index);
+  b.p = src.pF;
+  b.out = (unsigned long long*)src.pBPFD;
+  b.enc = src.enc;
+  b.X = d[0];
+  b.Y = d[1];
+  b.Z = d[2];
+  b.sy = d[0];
+  b.sz = (long long)d[0] * d[1];
+  b.nLevels = src.nLevels;
+  for (int l = 0; l < src.nLevels; ++l) {
+    // f32 fields compare against the threshold rounded to float.
+    b.t[l] = (src.enc == ENC_F32) ? (double)(float)src.pM[l].t[0] : src.pM[l].t[0];
+    b.m[l] = src.pM[l].m & 7u;
+  }
+  return MKF_OK;
 }
 
 // Tile shape of one field: the largest power-of-two x-tile up to the shape's
@@ -369,13 +388,158 @@ void buildItems(const std::vector<BatchField>& fs, std::vector<BatchItem>& items
   }
 }
 
+// Persistent grid: every CU holds as many workgroups as fit.
+int persistentGrid() {
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  int perCU = 1;
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kBatch, kBatchWaves * kWave, 0);
+  return std::max(1, cus * std::max(1, perCU));
+}
+
+// Queue reset (a kernel, so a captured graph has no memset node), optional
+// output clearing, then the persistent kernel.
+void launchAll(const BatchField* dFields, const BatchItem* dItems, int nItems, unsigned int* dCounter,
+               unsigned long long* const* dOuts, const int* dCounts, int nFields, bool zero, int grid,
+               hipStream_t s) {
+  hipLaunchKernelGGL(kBatchReset, dim3(1), dim3(64), 0, s, dCounter);
+  if (zero)
+    hipLaunchKernelGGL(kZeroList, dim3(std::min<int>(1024, nFields)), dim3(256), 0, s, dOuts, dCounts, nFields);
+  if (nItems > 0)
+    hipLaunchKernelGGL(kBatch, dim3(grid), dim3(kBatchWaves * kWave), 0, s, dFields, dItems, nItems, dCounter);
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+}  // namespace
+
+hipError_t copyWords(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes % 16 != 0) return hipErrorInvalidValue;
+  const size_t n = bytes / 16;
+  if (n == 0) return hipSuccess;
+  const int blocks = (int)std::min<size_t>(256, (n + 255) / 256);
+  hipLaunchKernelGGL(kCopyWords, dim3(blocks), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n);
+  return hipGetLastError();
+}
+
+// Launcher for a set of fields that changes from launch to launch (the
+// serving loop): descriptors and items go through pinned staging and one
+// copy per launch; buffers grow as needed and are kept.
+struct BatchLauncher {
+  char* hStage = nullptr;
+  char* hStageDev = nullptr;  // device address of hStage (mapped pinned memory)
+  char* dBuf = nullptr;
+  size_t cap = 0;
+  unsigned int* dCounter = nullptr;
+  hipEvent_t staged = nullptr;
+  int grid = 0;
+  std::vector<BatchField> fs;
+  std::vector<BatchItem> items;
+};
+
+BatchLauncher* batchLauncherCreate() {
+  BatchLauncher* L = new BatchLauncher();
+  if (hipMalloc((void**)&L->dCounter, sizeof(unsigned int)) != hipSuccess ||
+      hipEventCreateWithFlags(&L->staged, hipEventDisableTiming) != hipSuccess) {
+    bfail(MKF_ERR_MEM, "launcher allocation");
+    batchLauncherDestroy(L);
+    return nullptr;
+  }
+  L->grid = persistentGrid();
+  return L;
+}
+
+void batchLauncherDestroy(BatchLauncher* L) {
+  if (!L) return;
+  if (L->staged) {
+    (void)hipEventSynchronize(L->staged);
+    (void)hipEventDestroy(L->staged);
+  }
+  if (L->hStage) (void)hipHostFree(L->hStage);
+  if (L->dBuf) (void)hipFree(L->dBuf);
+  if (L->dCounter) (void)hipFree(L->dCounter);
+  delete L;
+}
+
+int batchLauncherRun(BatchLauncher* L, const MkfBatchField* fields, int n, const MkfBatchShape* shape, int flags,
+                     hipStream_t s) {
+  if (!L || !fields || n <= 0) return bfail(MKF_ERR_ARG, "no fields");
+  const MkfBatchShape sh = normalise(shape);
+  if (!shapeOk(sh)) return bfail(MKF_ERR_ARG, "shape: planePairs in 1..4096 and segments in {1, 2, 4, 8}");
+  L->fs.resize(n);
+  for (int f = 0; f < n; ++f) {
+    const int rc = describeField(fields[f], f, L->fs[f]);
+    if (rc != MKF_OK) return rc;
+    shapeField(L->fs[f], sh);
+  }
+  buildItems(L->fs, L->items);
+  if (L->items.size() > (size_t)0x7fffffff - 1) return bfail(MKF_ERR_UNSUPPORTED, "too many work items");
+  const size_t oF = 0, oI = align256(sizeof(BatchField) * n);
+  const size_t oO = oI + align256(sizeof(BatchItem) * L->items.size());
+  const size_t oC = oO + align256(sizeof(void*) * n);
+  const size_t total = oC + align256(sizeof(int) * n);
+  // The previous launch's staging copy must be done before the buffer is
+  // rewritten (or replaced).
+  if (hipEventSynchronize(L->staged) != hipSuccess) return bfail(MKF_ERR_HIP, "staging wait");
+  if (total > L->cap) {
+    const size_t cap = std::max(total, L->cap + L->cap / 2);
+    if (L->hStage) (void)hipHostFree(L->hStage);
+    if (L->dBuf) (void)hipFree(L->dBuf);
+    L->hStage = nullptr;
+    L->dBuf = nullptr;
+    L->cap = 0;
+    if (hipHostMalloc((void**)&L->hStage, cap, hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&L->hStageDev, L->hStage, 0) != hipSuccess ||
+        hipMalloc((void**)&L->dBuf, cap) != hipSuccess)
+      return bfail(MKF_ERR_MEM, "launcher buffers");
+    L->cap = cap;
+  }
+  memcpy(L->hStage + oF, L->fs.data(), sizeof(BatchField) * n);
+  if (!L->items.empty()) memcpy(L->hStage + oI, L->items.data(), sizeof(BatchItem) * L->items.size());
+  unsigned long long** outs = (unsigned long long**)(L->hStage + oO);
+  int* counts = (int*)(L->hStage + oC);
+  for (int f = 0; f < n; ++f) {
+    outs[f] = L->fs[f].out;
+    counts[f] = 256 * L->fs[f].nLevels;
+  }
+  if (copyWords(L->dBuf, L->hStageDev, total, s) != hipSuccess || hipEventRecord(L->staged, s) != hipSuccess)
+    return bfail(MKF_ERR_HIP, "descriptor upload");
+  launchAll((const BatchField*)(L->dBuf + oF), (const BatchItem*)(L->dBuf + oI), (int)L->items.size(), L->dCounter,
+            (unsigned long long* const*)(L->dBuf + oO), (const int*)(L->dBuf + oC), n,
+            (flags & MKF_BATCH_ZERO_OUTPUT) != 0, L->grid, s);
+  return hipGetLastError() == hipSuccess ? MKF_OK : bfail(MKF_ERR_HIP, "batch launch");
+}
+
+}  // namespace mkf
+
+// Graph-captured plan of a fixed set of fields.
+struct MkfBatch {
+  std::vector<BatchField> host;
+  BatchField* dFields = nullptr;
+  BatchItem* dItems = nullptr;
+  size_t itemCap = 0;
+  unsigned int* dCounter = nullptr;
+  unsigned long long** dOuts = nullptr;
+  int* dCounts = nullptr;
+  int nItems = 0;
+  int grid = 0;
+  MkfBatchShape shape{32, 8};
+  int flags = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+};
+
+namespace {
+
 int upload(MkfBatch* B, const MkfBatchShape& s, hipStream_t st) {
   for (auto& b : B->host) shapeField(b, s);
   std::vector<BatchItem> items;
   buildItems(B->host, items);
   if (items.size() > (size_t)0x7fffffff - 1) return bfail(MKF_ERR_UNSUPPORTED, "too many work items");
   if (items.size() > B->itemCap) {
-    if (B->dItems) hipFree(B->dItems);
+    if (B->dItems) (void)hipFree(B->dItems);
     B->dItems = nullptr;
     B->itemCap = 0;
     if (hipMalloc((void**)&B->dItems, sizeof(BatchItem) * items.size()) != hipSuccess)
@@ -392,23 +556,15 @@ int upload(MkfBatch* B, const MkfBatchShape& s, hipStream_t st) {
   return hipStreamSynchronize(st) == hipSuccess ? MKF_OK : bfail(MKF_ERR_HIP, "upload sync");
 }
 
-// Records the batch into stream s: queue reset (a kernel, so a captured
-// graph has no memset node), optional output clearing, then the persistent
-// kernel.
 hipError_t record(MkfBatch* B, hipStream_t s) {
-  hipLaunchKernelGGL(kBatchReset, dim3(1), dim3(64), 0, s, B->dCounter);
-  if (B->flags & MKF_BATCH_ZERO_OUTPUT)
-    hipLaunchKernelGGL(kZeroList, dim3(std::min<int>(1024, (int)B->host.size())), dim3(256), 0, s, B->dOuts,
-                       B->dCounts, (int)B->host.size());
-  if (B->nItems > 0)
-    hipLaunchKernelGGL(kBatch, dim3(B->grid), dim3(kBatchWaves * kWave), 0, s, B->dFields, B->dItems, B->nItems,
-                       B->dCounter);
+  launchAll(B->dFields, B->dItems, B->nItems, B->dCounter, B->dOuts, B->dCounts, (int)B->host.size(),
+            (B->flags & MKF_BATCH_ZERO_OUTPUT) != 0, B->grid, s);
   return hipGetLastError();
 }
 
 int capture(MkfBatch* B) {
-  if (B->exec) hipGraphExecDestroy(B->exec);
-  if (B->graph) hipGraphDestroy(B->graph);
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
   B->exec = nullptr;
   B->graph = nullptr;
   if (hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -436,48 +592,19 @@ MkfBatch* mkfBatchCreate(const MkfBatchField* fields, int nFields, const MkfBatc
     bfail(MKF_ERR_ARG, "shape: planePairs in 1..4096 and segments in {1, 2, 4, 8}");
     return nullptr;
   }
-  std::vector<BatchField> host;
+  std::vector<BatchField> host(nFields);
   std::vector<unsigned long long*> outs;
   std::vector<int> counts;
   for (int f = 0; f < nFields; ++f) {
-    const MkfBatchField& src = fields[f];
-    BatchField b;
-    memset(&b, 0, sizeof(b));
-    FieldDef d[3];
-    const int nd = copyNonZeroDef(d, src.def, 3);
-    if (!src.pF || !src.pBPFD || (src.enc != ENC_F32 && src.enc != ENC_F64) || src.nLevels < 1 ||
-        src.nLevels > kMaxLevels || !src.pM || nd <= 0 || src.def[0] < 0 || src.def[1] < 0 || src.def[2] < 0) {
-      setError(MKF_ERR_ARG, "batch: invalid description of field %d", f);
-      return nullptr;
-    }
-    b.p = src.pF;
-    b.out = (unsigned long long*)src.pBPFD;
-    b.enc = src.enc;
-    b.X = d[0];
-    b.Y = d[1];
-    b.Z = d[2];
-    b.sy = d[0];
-    b.sz = (long long)d[0] * d[1];
-    b.nLevels = src.nLevels;
-    for (int l = 0; l < src.nLevels; ++l) {
-      // f32 fields compare against the threshold rounded to float.
-      b.t[l] = (src.enc == ENC_F32) ? (double)(float)src.pM[l].t[0] : src.pM[l].t[0];
-      b.m[l] = src.pM[l].m & 7u;
-    }
-    host.push_back(b);
-    outs.push_back(b.out);
-    counts.push_back(256 * src.nLevels);
+    if (describeField(fields[f], f, host[f]) != MKF_OK) return nullptr;
+    outs.push_back(host[f].out);
+    counts.push_back(256 * host[f].nLevels);
   }
   MkfBatch* B = new MkfBatch();
   B->flags = flags;
   B->host.swap(host);
   hipStream_t st = (hipStream_t)stream;
-  int dev = 0, cus = 256;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  int perCU = 1;
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kBatch, kBatchWaves * kWave, 0);
-  B->grid = std::max(1, cus * std::max(1, perCU));
+  B->grid = persistentGrid();
   if (hipMalloc((void**)&B->dFields, sizeof(BatchField) * nFields) != hipSuccess ||
       hipMalloc((void**)&B->dCounter, sizeof(unsigned int)) != hipSuccess ||
       hipMalloc((void**)&B->dOuts, sizeof(void*) * nFields) != hipSuccess ||
@@ -528,8 +655,8 @@ int mkfBatchAutotune(MkfBatch* B, const MkfBatchShape* candidates, int nCandidat
     off += (size_t)b.nLevels * 256;
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
   MkfBatchShape best{0, 0};
   float bestMs = 1e30f;
   const int saveFlags = B->flags;
@@ -539,26 +666,26 @@ int mkfBatchAutotune(MkfBatch* B, const MkfBatchShape* candidates, int nCandidat
     const MkfBatchShape s = normalise(&candidates[c]);
     if (!shapeOk(s)) continue;
     if ((rc = upload(B, s, st)) != MKF_OK) break;
-    record(B, st);  // warm-up
-    hipEventRecord(e0, st);
-    for (int r = 0; r < std::max(1, reps); ++r) record(B, st);
-    hipEventRecord(e1, st);
+    (void)record(B, st);  // warm-up
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < std::max(1, reps); ++r) (void)record(B, st);
+    (void)hipEventRecord(e1, st);
     if (hipEventSynchronize(e1) != hipSuccess) {
       rc = bfail(MKF_ERR_HIP, "tuning run");
       break;
     }
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
     if (ms < bestMs) {
       bestMs = ms;
       best = s;
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   for (size_t f = 0; f < B->host.size(); ++f) B->host[f].out = saved[f];
   B->flags = saveFlags;
-  hipFree(scratch);
+  (void)hipFree(scratch);
   if (rc != MKF_OK) return rc;
   if (best.planePairs <= 0) return bfail(MKF_ERR_ARG, "no usable candidate");
   if ((rc = upload(B, best, st)) != MKF_OK || (rc = capture(B)) != MKF_OK) return rc;
@@ -576,14 +703,14 @@ int mkfBatchInfo(const MkfBatch* B, int* nItems, MkfBatchShape* shape, int* grid
 
 void mkfBatchDestroy(MkfBatch* B) {
   if (!B) return;
-  if (B->exec) hipGraphExecDestroy(B->exec);
-  if (B->graph) hipGraphDestroy(B->graph);
-  if (B->cap) hipStreamDestroy(B->cap);
-  if (B->dFields) hipFree(B->dFields);
-  if (B->dItems) hipFree(B->dItems);
-  if (B->dCounter) hipFree(B->dCounter);
-  if (B->dOuts) hipFree(B->dOuts);
-  if (B->dCounts) hipFree(B->dCounts);
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  if (B->cap) (void)hipStreamDestroy(B->cap);
+  if (B->dFields) (void)hipFree(B->dFields);
+  if (B->dItems) (void)hipFree(B->dItems);
+  if (B->dCounter) (void)hipFree(B->dCounter);
+  if (B->dOuts) (void)hipFree(B->dOuts);
+  if (B->dCounts) (void)hipFree(B->dCounts);
   delete B;
 }
 

@@ -47,4 +47,18 @@ hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long lo
 hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
                       unsigned long long* bpfd, const Tiling& tl, hipStream_t s);
 
+// Copies bytes (a multiple of 16) with a kernel on s; either side may be
+// the device address of mapped pinned host memory.
+hipError_t copyWords(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// Reusable launcher of the batched histogram kernel for a set of fields that
+// changes from launch to launch. Descriptions are validated as by
+// mkfBatchCreate; each run stages them through pinned memory (waiting for the
+// previous run's staging copy) and launches asynchronously on s.
+struct BatchLauncher;
+BatchLauncher* batchLauncherCreate();
+int batchLauncherRun(BatchLauncher* L, const MkfBatchField* fields, int n, const MkfBatchShape* shape, int flags,
+                     hipStream_t s);
+void batchLauncherDestroy(BatchLauncher* L);
+
 }  // namespace mkf

@@ -55,6 +55,11 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         "BinMapF64": ["t", "m"], "BMOrg": ["rowElem", "planePairs"], "BMFieldInfo": ["pD", "pFieldDevPtrTable"],
         "Context": ["bytesF", "pDF", "bmo", "ki"], "MkfTiling": ["planePairs", "voxelsPerLane"],
         "MkfBatchField": ["pF", "enc", "def", "nLevels", "pM", "pBPFD"],
+        "MkfBatchShape": ["planePairs", "segments"],
+        "MkfServeConfig": ["hbmBudget", "maxInFlight", "maxBatch", "shape"],
+        "MkfServeRequest": ["pF", "flags", "stream", "enc", "def", "nLevels", "pM", "pBPFD", "tag"],
+        "MkfServeResult": ["tag", "status", "batchSize", "tSubmit", "tDone"],
+        "MkfServeStats": ["submitted", "bytesIn", "heldMax", "gpuBusy"],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mkf.h"', "int main(void){"]
     for s, fs in probes.items():

@@ -44,3 +44,22 @@ def test_batch_mix_definition():
     assert all(1.0 <= m[2] <= 6.0 for m in mix)
     assert len({m[3] for m in mix}) == 64
     assert synth.batch_mix(seed=4) == mix and synth.batch_mix(seed=5) != mix
+
+
+def test_request_stream_definition():
+    from mkf_amd.serve import poisson_arrivals, request_stream
+    pool, sims, reqs = request_stream(2000, seed=5)
+    assert len(pool) == 24 and len(sims) == 4 and len(reqs) == 2000
+    sides = np.array([s for s, _ in pool + sims])
+    assert sides.min() >= 64 and sides.max() <= 768
+    dev = np.mean([r["sim"] is not None for r in reqs])
+    assert abs(dev - 0.3) < 0.04
+    nl = np.array([len(r["levels"]) for r in reqs])
+    assert nl.min() == 1 and nl.max() == 16
+    for r in reqs:
+        ts = [t for _, t in r["levels"]]
+        assert ts == sorted(ts) and -1.5 <= ts[0] and ts[-1] <= 1.5
+    assert request_stream(50, seed=5)[2] == request_stream(50, seed=5)[2]
+    arr = poisson_arrivals(20000, 500.0, seed=1)
+    assert np.all(np.diff(arr) > 0)
+    assert abs(arr[-1] / 20000 - 1 / 500.0) < 0.05 / 500.0
