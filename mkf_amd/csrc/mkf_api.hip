@@ -156,8 +156,9 @@ int mkfPBMAsync(BMPackWord* pW, BMOrg* pO, const BMFieldInfo* pI, const BinMapF6
 int mkfBPFDAsync(size_t* pBPFD, const BMOrg* pO, const BMPackWord* pW, const MkfTiling* pT, void* stream) {
   if (!pBPFD || !pO || !pW) return fail(MKF_ERR_ARG, "NULL argument");
   const int X = (int)pO->rowElem, Y = (int)pO->rowPairs + 1, Z = (int)pO->planePairs + 1;
+  // 4 voxels per lane (nibbles of the PBM words), or 1 when asked for.
   Tiling tl = toTiling(pT, X, Y, Z, 4);
-  tl.V = 1;  // one PBM bit per lane
+  tl.V = (tl.V == 1) ? 1 : 4;
   int rc = checkTiling(tl, 4);
   if (rc) return rc;
   const hipError_t e = launchBPFD(pW, X, Y, Z, pO->rowWS, pO->planeWS, (unsigned long long*)pBPFD, tl,

@@ -159,6 +159,7 @@ struct VecLoad<double, 2> {
 // (walker), or from memory when that wave belongs to another workgroup.
 template <typename T, int M, int V>
 struct FieldRows {
+  static constexpr bool kSelfHalo = false;
   const T* fw;       // field + xb, xb = min(x0, X - 1) (uniform)
   T thr;
   long long sy, sz;  // element strides of y and z (x stride is 1)
@@ -214,6 +215,7 @@ struct FieldRows {
 // the wave's two words are read by the scalar unit. Offsets are in words
 // (sy = rowWS, sz = planeWS).
 struct PbmRows {
+  static constexpr bool kSelfHalo = false;
   const uint32_t* ws;  // PBM + first word of the segment, clamped into the row
   long long sy, sz;
   uint32_t o1;         // offset of the segment's second word, 0 if outside the row
@@ -236,6 +238,36 @@ struct PbmRows {
     return packPairs<1>(b, laneBit(R0 >> 1) ? 1u : 0u) & keep;
   }
   __device__ __forceinline__ uint32_t haloBit(long long off) const { return uniformLoad(ws + (off + hw)) & 1u; }
+};
+
+// Rows of an existing packed binary map, 4 voxels per lane: lane l holds
+// voxels x0 + 4l .. 4l+3, nibble l & 7 of word x0/32 + (l >> 3). It reads
+// that word and the next one, so the bit right of its last voxel -- lane 63's
+// too -- is in its own registers and no halo is exchanged. The five bits
+// b0..b4 become the pair word (byte k = b_k | b_(k+1) << 1) by one multiply:
+// v * 0x204081 places copies of v at bits 0, 7, 14 and 21 without overlap, so
+// byte k holds bits k and k+1 of v at its bits 0 and 1.
+struct PbmRows4 {
+  static constexpr bool kSelfHalo = true;
+  const uint32_t* row;  // PBM (word offsets below are from the row start)
+  long long sy, sz;     // word strides of y and z
+  uint32_t w0, w1;      // the lane's word and the next, clamped into the row
+  uint32_t sh;          // 4 * (lane & 7)
+  uint32_t keep;        // pair-word bytes of cells that exist (per lane)
+
+  struct Raw {
+    uint32_t lo, hi;
+  };
+
+  __device__ __forceinline__ Raw load(long long off) const {
+    return Raw{globalLoad(row, off + w0), globalLoad(row, off + w1)};
+  }
+  __device__ __forceinline__ uint32_t pairs(const Raw& r, uint64_t& R0) const {
+    R0 = 0;
+    const uint32_t v = __builtin_amdgcn_alignbit(r.hi, r.lo, sh) & 31u;
+    return (v * 0x204081u) & 0x03030303u & keep;
+  }
+  __device__ __forceinline__ uint32_t haloBit(long long) const { return 0u; }
 };
 
 // Per-wave tally. Cells that do not exist have code 0 (their pair bits are
@@ -288,18 +320,20 @@ __device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1
     for (int r = 0; r <= TY; ++r) {
       uint64_t R0;
       P[r] = src.pairs(cur[r], R0);
-      if (lane == 0) xch[par][r][w] = (uint32_t)R0;
+      if (!Src::kSelfHalo && lane == 0) xch[par][r][w] = (uint32_t)R0;
     }
-    __syncthreads();
-    if (halo == kHaloMem) {
-      const long long zo = (long long)z * src.sz;
+    if constexpr (!Src::kSelfHalo) {
+      __syncthreads();
+      if (halo == kHaloMem) {
+        const long long zo = (long long)z * src.sz;
 #pragma unroll
-      for (int r = 0; r <= TY; ++r) P[r] |= (src.haloBit(yo[r] + zo) & lane63) << kRightBit<V>;
-    } else {
+        for (int r = 0; r <= TY; ++r) P[r] |= (src.haloBit(yo[r] + zo) & lane63) << kRightBit<V>;
+      } else {
 #pragma unroll
-      for (int r = 0; r <= TY; ++r) {
-        const uint32_t hb = __builtin_amdgcn_readfirstlane(xch[par][r][nb]) & ldsKeep;
-        P[r] |= (hb & lane63) << kRightBit<V>;
+        for (int r = 0; r <= TY; ++r) {
+          const uint32_t hb = __builtin_amdgcn_readfirstlane(xch[par][r][nb]) & ldsKeep;
+          P[r] |= (hb & lane63) << kRightBit<V>;
+        }
       }
     }
     uint32_t Hc[TY];

@@ -2,8 +2,9 @@
 //
 //   kPBM         field -> packed binary map      (replaces mapField /
 //                mapStrideField, src/binMapCUDA.cuh:282-323)
-//   kBPFD        PBM -> 256-bin pattern histogram (replaces
-//                addMultiPlaneSeqBPFD / addPlaneBPFD, src/bpfdCUDA.cuh:158-206)
+//   kBPFD4/kBPFD PBM -> 256-bin pattern histogram, 4 or 1 voxels per lane
+//                (replaces addMultiPlaneSeqBPFD / addPlaneBPFD,
+//                src/bpfdCUDA.cuh:158-206)
 //   kFusedBPFD   field -> histogram, reading the field once (no reference
 //                counterpart; the reference round-trips the PBM through memory)
 #include "mkf_device.hpp"
@@ -12,42 +13,54 @@
 namespace mkf {
 
 // ---------------------------------------------------------------------------
-// Packed binary map: one wave per x-segment and RU consecutive rows.
+// Packed binary map. A wave owns one 64-voxel x-segment and a run of
+// consecutive rows (y fastest, then z); it keeps kPbmAhead row loads in
+// flight, thresholds each row with one ballot and stores the ballot as the
+// segment's two PBM words (lanes 0 and 1). The waves of a workgroup take
+// consecutive segments of the same rows, so a workgroup reads whole rows.
+// Rows advance by incrementing (y, z): no division per row.
 // ---------------------------------------------------------------------------
-constexpr int kPbmWaves = 4;
-constexpr int kPbmRows = 8;
+constexpr int kPbmWaves = 8;
+constexpr int kPbmAhead = 8;
 
 template <typename T, int M>
 __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const T* __restrict__ f, T thr, int X, int Y,
                                                            long long sy, long long sz, int nSegX, long long nRows,
-                                                           uint32_t* __restrict__ pW, uint32_t rowWS,
-                                                           uint32_t planeWS) {
+                                                           long long rowsPerWave, uint32_t* __restrict__ pW,
+                                                           uint32_t rowWS, uint32_t planeWS) {
   const int lane = threadIdx.x & (kWave - 1);
-  const long long gw = (long long)blockIdx.x * kPbmWaves + (threadIdx.x >> 6);
+  const long long gw = (long long)blockIdx.x * kPbmWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int seg = (int)(gw % nSegX);
-  const long long row0 = (gw / nSegX) * kPbmRows;
-  if (row0 >= nRows) return;
+  const long long r0 = (gw / nSegX) * rowsPerWave;
+  if (r0 >= nRows) return;
+  const long long n = min(rowsPerWave, nRows - r0);
+  int y = (int)(r0 % Y);
+  long long z = r0 / Y;
   const int x0 = seg * kWave;
-  const int xc = min(x0 + lane, X - 1);
+  const T* fx = f + min(x0 + lane, X - 1);
   const uint64_t tail = lowBits(X - x0);
-  const int nw = min(2, (int)rowWS - 2 * seg);
-  T v[kPbmRows];
+  const bool writer = lane < min(2, (int)rowWS - 2 * seg);
+  uint32_t* px = pW + 2 * seg + (lane & 1);
+  for (long long i = 0; i < n; i += kPbmAhead) {
+    T v[kPbmAhead];
+    size_t wo[kPbmAhead];
 #pragma unroll
-  for (int u = 0; u < kPbmRows; ++u) {
-    const long long row = row0 + u;
-    if (row < nRows) {
-      const long long y = row % Y, z = row / Y;
-      v[u] = f[y * sy + z * sz + xc];
+    for (int u = 0; u < kPbmAhead; ++u) {
+      if (i + u < n) {
+        v[u] = fx[(long long)y * sy + z * sz];
+        wo[u] = (size_t)y * rowWS + (size_t)z * planeWS;
+        if (++y == Y) {
+          y = 0;
+          ++z;
+        }
+      }
     }
-  }
 #pragma unroll
-  for (int u = 0; u < kPbmRows; ++u) {
-    const long long row = row0 + u;
-    if (row < nRows) {
-      const long long y = row % Y, z = row / Y;
-      const uint64_t R = relMask<M>(v[u], thr) & tail;
-      const size_t wi = (size_t)y * rowWS + (size_t)z * planeWS + 2 * seg;
-      if (lane < nw) pW[wi + lane] = lane ? (uint32_t)(R >> 32) : (uint32_t)R;
+    for (int u = 0; u < kPbmAhead; ++u) {
+      if (i + u < n) {
+        const uint64_t R = relMask<M>(v[u], thr) & tail;
+        if (writer) px[wo[u]] = (lane & 1) ? (uint32_t)(R >> 32) : (uint32_t)R;
+      }
     }
   }
 }
@@ -208,6 +221,35 @@ __global__ __launch_bounds__(1024) void kBPFD(const uint32_t* __restrict__ pW, i
   flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
 }
 
+// Histogram of an existing PBM with 4 voxels per lane (PbmRows4): each lane
+// reads its PBM word and the next, so there is no halo exchange.
+template <int TY>
+__global__ __launch_bounds__(1024) void kBPFD4(const uint32_t* __restrict__ pW, int X, int Y, int Z,
+                                               uint32_t rowWS, uint32_t planeWS, TileGrid g,
+                                               unsigned long long* __restrict__ bpfd) {
+  __shared__ uint32_t hist[16][256];
+  int xt, yr, zr;
+  if (!tileOf(g, xt, yr, zr)) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const WaveGeom q = waveGeom(g, xt, yr, zr, TY, 4, X, Y, Z);
+  for (int i = lane; i < 256; i += kWave) hist[q.w][i] = 0;
+
+  PbmRows4 src;
+  src.row = pW;
+  src.sy = rowWS;
+  src.sz = planeWS;
+  const uint32_t wl = (uint32_t)(q.x0 >> 5) + (uint32_t)(lane >> 3);
+  src.w0 = min(wl, rowWS - 1u);
+  src.w1 = min(wl + 1u, rowWS - 1u);
+  src.sh = 4u * (uint32_t)(lane & 7);
+  uint32_t cellsPerRow;
+  src.keep = cellKeep<4>(q.x0, X, cellsPerRow);
+  Tally tally;
+  walkStack<TY, 4>(src, (uint32_t(*)[TY + 1][16]) nullptr, q.w, kHaloNone, q.y0, q.nY, q.z0, q.nZ, hist[q.w],
+                   tally);
+  flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -260,15 +302,19 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   return tl;
 }
 
+// About 16K waves in all (twice what 256 CUs hold at 32 waves each), each
+// streaming a run of rows of one segment.
 template <typename T, int M>
 static hipError_t launchPBMT(const FieldView& fv, T thr, uint32_t* pW, uint32_t rowWS, uint32_t planeWS,
                              hipStream_t s) {
   const int nSegX = (int)ceilDiv(fv.X, kWave);
   const long long nRows = (long long)fv.Y * fv.Z;
-  const long long nWaves = nSegX * ceilDiv(nRows, kPbmRows);
+  const long long wantChunks = std::max<long long>(1, (16384 + nSegX - 1) / nSegX);
+  const long long rowsPerWave = ceilDiv(nRows, wantChunks);
+  const long long nWaves = nSegX * ceilDiv(nRows, rowsPerWave);
   const long long nBlocks = ceilDiv(nWaves, kPbmWaves);
-  hipLaunchKernelGGL((kPBM<T, M>), dim3((unsigned)nBlocks), dim3(kPbmWaves * kWave), 0, s,
-                     (const T*)fv.p, thr, fv.X, fv.Y, fv.sy, fv.sz, nSegX, nRows, pW, rowWS, planeWS);
+  hipLaunchKernelGGL((kPBM<T, M>), dim3((unsigned)nBlocks), dim3(kPbmWaves * kWave), 0, s, (const T*)fv.p, thr,
+                     fv.X, fv.Y, fv.sy, fv.sz, nSegX, nRows, rowsPerWave, pW, rowWS, planeWS);
   return hipGetLastError();
 }
 
@@ -354,11 +400,19 @@ hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long lo
 hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
                       unsigned long long* bpfd, const Tiling& tl0, hipStream_t s) {
   if (X < 2 || Y < 2 || Z < 2) return hipSuccess;
-  Tiling tl = tl0;
-  tl.V = 1;  // one PBM bit per lane
+  // tl.V is 4 (nibbles of the PBM words) or 1.
+  const Tiling& tl = tl0;
   if (!tilingSupported(tl, 4)) return hipErrorInvalidValue;
   const TileGrid g = makeGrid(X, Y, Z, tl);
   const dim3 grid((unsigned)g.chunk * 8u), block(g.waves * kWave);
+  if (tl.V == 4) {
+    switch (tl.Ty) {
+      case 2: hipLaunchKernelGGL((kBPFD4<2>), grid, block, 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd); break;
+      case 8: hipLaunchKernelGGL((kBPFD4<8>), grid, block, 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd); break;
+      default: hipLaunchKernelGGL((kBPFD4<4>), grid, block, 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd); break;
+    }
+    return hipGetLastError();
+  }
   switch (tl.Ty) {
     case 2: hipLaunchKernelGGL((kBPFD<2>), grid, block, 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd); break;
     case 8: hipLaunchKernelGGL((kBPFD<8>), grid, block, 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd); break;
