@@ -97,9 +97,11 @@ __device__ void batchFlush(BatchShared& sh, const BatchField& fd) {
 }
 
 // Pair bits of one row at relation M: bit 0 from the lane's voxel, bit 1
-// from its right neighbour. Relations that are the complement of a one- or
-// two-ballot mask (">=", "<=", "=", all) select inverted constants rather than
-// inverting the mask. NaN compares false both ways and so takes the EQ bit.
+// from its right neighbour. The lane's bit enters as the carry of an add (or
+// the borrow of a subtract, for relations that are the complement of a
+// ballot: ">=", "<=", "=", all), so a row costs its compares, one v_cndmask
+// and one v_addc / v_subb. NaN compares false both ways and so takes the EQ
+// bit.
 template <int M, typename T>
 __device__ __forceinline__ uint32_t pairBits(T f, T fr, T t) {
   constexpr bool inv = (M == 2 || M == 3 || M == 6 || M == 7);
@@ -114,16 +116,20 @@ __device__ __forceinline__ uint32_t pairBits(T f, T fr, T t) {
     a = ballot(f < t) | ballot(f > t);
     b = ballot(fr < t) | ballot(fr > t);
   }
-  const uint32_t pa = laneBit(a) ? (inv ? 0u : 1u) : (inv ? 1u : 0u);
-  const uint32_t pb = laneBit(b) ? (inv ? 0u : 2u) : (inv ? 2u : 0u);
-  return pa | pb;
+  if constexpr (inv) {
+    return subLaneBit(laneBit(b) ? 1u : 3u, a);
+  } else {
+    return addLaneBit(laneBit(b) ? 2u : 0u, a);
+  }
 }
 
 // One level of one z-step: the pair nibbles of the wave's 5 rows at level
 // relation M, then (from the second plane on) the tally of its row pairs'
 // cells into histL. rgt[r] is the voxel right of each lane's voxel (lane 63:
 // the halo voxel). vr[r] holds the lanes whose cell in row pair r exists
-// (none for row pairs beyond the field), so nothing else is masked.
+// (none for row pairs beyond the field), so nothing else is masked. A cell
+// is mixed when its code is neither 0 nor 255: two compares, the rest in
+// scalar registers.
 template <int M, typename T>
 __device__ __forceinline__ uint32_t levelStep(const T (&cur)[kBatchTY + 1], const T (&rgt)[kBatchTY + 1], T thr,
                                               const uint64_t (&vr)[kBatchTY], bool count, uint32_t hp,
@@ -140,9 +146,10 @@ __device__ __forceinline__ uint32_t levelStep(const T (&cur)[kBatchTY + 1], cons
 #pragma unroll
     for (int r = 0; r < kBatchTY; ++r) {
       const uint32_t code = (C >> (8 * r)) & 255u;
-      ones += (uint32_t)__builtin_popcountll(ballot(code == 255u) & vr[r]);
-      if (laneBit(vr[r]) && code - 1u < 254u)
-        __hip_atomic_fetch_add(histL + code, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const uint64_t all1 = ballot(code == 255u) & vr[r];
+      const uint64_t mixed = vr[r] & ~(all1 | ballot(code == 0u));
+      ones += (uint32_t)__builtin_popcountll(all1);
+      if (laneBit(mixed)) __hip_atomic_fetch_add(histL + code, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     if ((threadIdx.x & (kWave - 1)) == 0 && ones) atomicAdd(histL + 255, ones);
   }

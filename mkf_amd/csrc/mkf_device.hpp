@@ -63,6 +63,46 @@ __device__ __forceinline__ uint64_t relMaskDyn(T f, T t, uint32_t m) {
 // Lane l reads bit l of a wave-uniform 64-bit mask (one v_cndmask).
 __device__ __forceinline__ bool laneBit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 
+// x + (bit l of mask) for lane l: one v_addc_co_u32 with the mask as its
+// carry-in (register operands only; the carry-out is discarded).
+__device__ __forceinline__ uint32_t addLaneBit(uint32_t x, uint64_t mask) {
+  uint32_t r;
+  uint64_t co;
+  asm("v_addc_co_u32_e64 %0, %1, %2, 0, %3" : "=v"(r), "=s"(co) : "v"(x), "s"(mask));
+  return r;
+}
+// x - (bit l of mask) for lane l: one v_subb_co_u32 with the mask as borrow.
+__device__ __forceinline__ uint32_t subLaneBit(uint32_t x, uint64_t mask) {
+  uint32_t r;
+  uint64_t co;
+  asm("v_subb_co_u32_e64 %0, %1, %2, 0, %3" : "=v"(r), "=s"(co) : "v"(x), "s"(mask));
+  return r;
+}
+
+// 2x + (bit l of mask) for lane l: one v_addc_co_u32 (x + x + carry).
+__device__ __forceinline__ uint32_t dblAddLaneBit(uint32_t x, uint64_t mask) {
+  uint32_t r;
+  uint64_t co;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(co) : "v"(x), "s"(mask));
+  return r;
+}
+
+// v | (the value of v in lane l ^ 1), (l ^ 2), lane l + 4 or l + 8 of the
+// same 16-lane row (0 where that lane is outside the row): DPP quad_perm
+// [1,0,3,2], quad_perm [2,3,0,1], row_shl:4, row_shl:8.
+__device__ __forceinline__ uint32_t orQuadXor1(uint32_t v) {
+  return v | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t orQuadXor2(uint32_t v) {
+  return v | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t orRowShl4(uint32_t v) {
+  return v | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x104, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t orRowShl8(uint32_t v) {
+  return v | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x108, 0xF, 0xF, true);
+}
+
 // Lane l receives lane l+1's value through DPP (wave_shl:1, one v_mov per
 // dword); lane 63, which has no source, keeps its own value.
 __device__ __forceinline__ uint32_t laneNext(uint32_t v) {
