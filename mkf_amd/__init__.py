@@ -211,15 +211,19 @@ def dims(def3):
 
 
 class FieldInfo:
-    """A BMFieldInfo for one device field plus the arrays it points to."""
+    """A BMFieldInfo plus the arrays it points to. dev_ptr is one device
+    pointer or a table of them (None for empty entries); mask defaults to
+    every entry of the table."""
 
-    def __init__(self, dev_ptr: int, def3, enc: int, prof_id: int = 0, strides=None):
+    def __init__(self, dev_ptr, def3, enc: int, prof_id: int = 0, strides=None, mask=None):
         self.def3 = dims(def3)
-        self.table = (ConstFieldPtr * 1)()
-        self.table[0].p = dev_ptr
+        ptrs = list(dev_ptr) if isinstance(dev_ptr, (list, tuple)) else [dev_ptr]
+        self.table = (ConstFieldPtr * len(ptrs))()
+        for i, p in enumerate(ptrs):
+            self.table[i].p = p
         self.strides = None if strides is None else (C.c_long * 3)(*strides)
         self.info = BMFieldInfo()
-        self.info.fieldTableMask = 1
+        self.info.fieldTableMask = (1 << len(ptrs)) - 1 if mask is None else mask
         self.info.elemID = enc
         self.info.oprID = 0
         self.info.profID = prof_id

@@ -88,12 +88,14 @@ int resolveField(const BMFieldInfo* pI, FieldView* fv, FieldDef def[3]) {
   if (pI->pFieldDevPtrTable) nField = countValidPtrByMask(pI->pFieldDevPtrTable, pI->fieldTableMask);
   const int nDim = copyNonZeroDef(def, pI->pD, 3);
   if (nDim <= 0 || nField <= 0) return fail(MKF_ERR_ARG, "no valid field (nDim=%d nField=%d)", nDim, nField);
-  if (nField > 1) return fail(MKF_ERR_UNSUPPORTED, "multi-field sum (%d fields) is not supported", nField);
   if (pI->elemID != ENC_F32 && pI->elemID != ENC_F64)
     return fail(MKF_ERR_ARG, "element encoding 0x%02X is not F32/F64", pI->elemID);
-  ConstFieldPtr fp;
-  copyValidPtrByMask(&fp, 1, pI->pFieldDevPtrTable, pI->fieldTableMask);
-  fv->p = fp.p;
+  // The first (up to) 4 valid entries of the table, summed in table order
+  // (src/binMapCUDA.cuh:178-212; BMCU_FIELD_MAX, src/binMapUtil.h:105).
+  ConstFieldPtr fp[kMaxFields];
+  *fv = FieldView{};
+  fv->nF = copyValidPtrByMask(fp, kMaxFields, pI->pFieldDevPtrTable, pI->fieldTableMask);
+  for (int i = 0; i < fv->nF; ++i) fv->p[i] = fp[i].p;
   fv->enc = pI->elemID;
   fv->X = def[0];
   fv->Y = def[1];
@@ -103,10 +105,14 @@ int resolveField(const BMFieldInfo* pI, FieldView* fv, FieldDef def[3]) {
     // was dropped by the compaction above.
     if (!pI->pD || pI->pD[0] != def[0] || pI->pD[1] != def[1] || pI->pD[2] != def[2])
       return fail(MKF_ERR_UNSUPPORTED, "strided field with dims <= 1");
-    if (pI->pS[0] != 1) return fail(MKF_ERR_UNSUPPORTED, "x stride %ld != 1", (long)pI->pS[0]);
+    if (pI->pS[0] < 1 || pI->pS[1] < 1 || pI->pS[2] < 1)
+      return fail(MKF_ERR_ARG, "strides must be positive (%ld, %ld, %ld)", (long)pI->pS[0], (long)pI->pS[1],
+                  (long)pI->pS[2]);
+    fv->sx = pI->pS[0];
     fv->sy = pI->pS[1];
     fv->sz = pI->pS[2];
   } else {
+    fv->sx = 1;
     fv->sy = def[0];
     fv->sz = (long long)def[0] * def[1];
   }
@@ -340,7 +346,7 @@ Bool32 mkfAccGetBPFDSimple(size_t rBPFD[MKF_BINS], BMPackWord* pW, const double*
       hipMalloc((void**)&dH, MKF_BINS * 8) == hipSuccess &&
       hipMemcpy(dF, pF, nF * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
       hipMemset(dH, 0, MKF_BINS * 8) == hipSuccess) {
-    FieldView fv{dF, ENC_F64, def[0], def[1], def[2], def[0], (long long)def[0] * def[1]};
+    const FieldView fv = planarField(dF, ENC_F64, def[0], def[1], def[2]);
     const Tiling tl = chooseTiling(fv.X, fv.Y, fv.Z, 8);
     if (launchPBM(fv, *pM, dW, org.rowWS, org.planeWS, 0) == hipSuccess &&
         launchBPFD(dW, fv.X, fv.Y, fv.Z, org.rowWS, org.planeWS, dH, tl, 0) == hipSuccess &&
@@ -374,7 +380,7 @@ Bool32 mkfAccCUDAGetBPFD(size_t rBPFD[MKF_BINS], const void* pF, const FieldDef 
   if (hipMalloc(&dF, nF * eb) == hipSuccess && hipMalloc((void**)&dH, MKF_BINS * 8) == hipSuccess &&
       hipMemcpy(dF, pF, nF * eb, hipMemcpyHostToDevice) == hipSuccess &&
       hipMemset(dH, 0, MKF_BINS * 8) == hipSuccess) {
-    FieldView fv{dF, enc, d[0], d[1], d[2], d[0], (long long)d[0] * d[1]};
+    const FieldView fv = planarField(dF, enc, d[0], d[1], d[2]);
     const Tiling tl = chooseTiling(fv.X, fv.Y, fv.Z, (int)eb);
     if (launchFused(fv, *pM, dH, tl, 0) == hipSuccess &&
         hipMemcpy(rBPFD, dH, MKF_BINS * 8, hipMemcpyDeviceToHost) == hipSuccess) {

@@ -191,32 +191,45 @@ struct VecLoad<double, 2> {
   typedef double type __attribute__((ext_vector_type(2), aligned(8)));
 };
 
+// Kernel argument: up to 4 fields of identical dims and element strides,
+// thresholded as their sum in table order.
+template <typename T>
+struct FieldArgs {
+  const T* p[4];
+  int nF;
+  long long sx, sy, sz;
+};
+
 // Rows of a scalar field thresholded on the fly (fused path). Lane l holds
 // voxels x0 + V*l + k. Addresses are a wave-uniform row pointer plus a lane
 // offset; a wave whose voxels all exist reads them with one vector load per
 // row (4 floats or 2 doubles = 16 bytes), a partial wave clamps each voxel.
-// The bit right of the wave's last voxel comes from the next wave through LDS
+// The general form (G: several fields, or an x stride) gathers each voxel
+// and sums the fields in table order (src/binMapCUDA.cuh:196-201); the
+// plain form compiles to the single-field, unit-stride loads only. The bit
+// right of the wave's last voxel comes from the next wave through LDS
 // (walker), or from memory when that wave belongs to another workgroup.
-template <typename T, int M, int V>
+template <typename T, int M, int V, bool G = false>
 struct FieldRows {
   static constexpr bool kSelfHalo = false;
-  const T* fw;       // field + xb, xb = min(x0, X - 1) (uniform)
+  const T* fw[4];    // fields + xb * sx, xb = min(x0, X - 1) (uniform)
+  int nF;            // fields summed (1..4)
+  long long sx;      // element stride of x
   T thr;
-  long long sy, sz;  // element strides of y and z (x stride is 1)
-  uint32_t lx;       // lane offset of voxel 0 from xb, clamped into the row
+  long long sy, sz;  // element strides of y and z
+  uint32_t lx;       // lane offset (in voxels) of voxel 0 from xb, clamped into the row
   int last;          // X - 1 - xb: clamp for partial waves
-  bool full;         // every voxel of every lane exists
-  int hxo;           // offset from xb of the next wave's first voxel, clamped
+  bool vec;          // unit x stride and every voxel of every lane exists
+  int hxo;           // offset (in voxels) from xb of the next wave's first voxel, clamped
   uint32_t keep;     // pair-word bytes of cells that exist (per lane)
 
   struct Raw {
     T v[V];
   };
 
-  __device__ __forceinline__ Raw load(long long off) const {
-    const T* row = fw + off;
+  __device__ __forceinline__ Raw loadOne(const T* row) const {
     Raw r;
-    if (V == 1 || full) {
+    if (!G && vec) {
       const typename VecLoad<T, V>::type x = *(const typename VecLoad<T, V>::type*)(row + lx);
       if constexpr (V == 1) {
         r.v[0] = x;
@@ -226,7 +239,18 @@ struct FieldRows {
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < V; ++k) r.v[k] = row[min((int)lx + k, last)];
+      for (int k = 0; k < V; ++k) r.v[k] = row[G ? (long long)min((int)lx + k, last) * sx : min((int)lx + k, last)];
+    }
+    return r;
+  }
+  __device__ __forceinline__ Raw load(long long off) const {
+    Raw r = loadOne(fw[0] + off);
+    if constexpr (G) {
+      for (int f = 1; f < nF; ++f) {
+        const Raw q = loadOne(fw[f] + off);
+#pragma unroll
+        for (int k = 0; k < V; ++k) r.v[k] += q.v[k];
+      }
     }
     return r;
   }
@@ -247,7 +271,12 @@ struct FieldRows {
   }
   // Bit of the voxel right of the wave, read from memory (wave-uniform).
   __device__ __forceinline__ uint32_t haloBit(long long off) const {
-    return (uint32_t)(relMask<M>(uniformLoad(fw + (off + hxo)), thr) & 1ull);
+    const long long o = off + (G ? (long long)hxo * sx : (long long)hxo);
+    T h = uniformLoad(fw[0] + o);
+    if constexpr (G) {
+      for (int f = 1; f < nF; ++f) h += uniformLoad(fw[f] + o);
+    }
+    return (uint32_t)(relMask<M>(h, thr) & 1ull);
   }
 };
 

@@ -8,14 +8,32 @@
 
 namespace mkf {
 
-// A planar-row field on the device: x is contiguous, y and z have element
-// strides sy and sz.
+// Up to 4 device fields of identical dims and element strides, thresholded
+// as their sum in table order (src/binMapCUDA.cuh:178-212: s = f0, then
+// s += f1, f2, f3 in the element type).
+constexpr int kMaxFields = 4;
 struct FieldView {
-  const void* p;
+  const void* p[kMaxFields];
+  int nF;
   NumEnc enc;  // ENC_F32 or ENC_F64
   int X, Y, Z;
-  long long sy, sz;
+  long long sx, sy, sz;
 };
+
+// One planar field (x contiguous, y and z strides X and X*Y).
+inline FieldView planarField(const void* p, NumEnc enc, int X, int Y, int Z) {
+  FieldView fv{};
+  fv.p[0] = p;
+  fv.nF = 1;
+  fv.enc = enc;
+  fv.X = X;
+  fv.Y = Y;
+  fv.Z = Z;
+  fv.sx = 1;
+  fv.sy = X;
+  fv.sz = (long long)X * Y;
+  return fv;
+}
 
 // Work decomposition of the histogram kernels: a workgroup of up to ZW waves
 // side by side along x (one 64-voxel segment each) holds Ty row pairs and

@@ -34,11 +34,11 @@ constexpr int kPbmWaves = 8;
 constexpr int kPbmAhead = MKF_PBM_AHEAD;
 constexpr int kPbmVoxels = 16;  // per lane
 
-template <typename T, int M>
-__global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const T* __restrict__ f, T thr, int X, int Y,
-                                                           long long sy, long long sz, int nSegX, long long nRows,
-                                                           long long rowsPerWave, uint32_t* __restrict__ pW,
-                                                           uint32_t rowWS, uint32_t planeWS) {
+template <typename T, int M, bool G>
+__global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, T thr, int X, int Y, int nSegX,
+                                                           long long nRows, long long rowsPerWave,
+                                                           uint32_t* __restrict__ pW, uint32_t rowWS,
+                                                           uint32_t planeWS) {
   constexpr int V = kPbmVoxels;
   constexpr int E = 16 / sizeof(T);  // elements per 16-byte load
   constexpr int W = kWave * V;
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const T* __restrict__ f
   long long z = r0 / Y;
   const int x0 = seg * W;
   const int xl = x0 + V * lane;
-  const bool full = x0 + W <= X;
+  const bool vec = !G && x0 + W <= X;
   const uint32_t keepBits = (xl >= X) ? 0u : ((X - xl >= V) ? 0xFFFFu : ((1u << (X - xl)) - 1u));
   const uint32_t word = (uint32_t)(x0 >> 5) + (uint32_t)(lane >> 1);
   const bool writer = (lane & 1) == 0 && word < rowWS;
@@ -64,17 +64,24 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const T* __restrict__ f
 #pragma unroll
     for (int u = 0; u < kPbmAhead; ++u) {
       if (i + u < n) {
-        const T* row = f + (long long)y * sy + z * sz;
-        if (full) {
+        // Several fields are summed in table order (general form only).
+        const long long ro = (long long)y * fa.sy + z * fa.sz;
+        for (int fi = 0; fi < (G ? fa.nF : 1); ++fi) {
+          const T* row = fa.p[fi] + ro;
+          T t[V];
+          if (vec) {
 #pragma unroll
-          for (int q = 0; q < V / E; ++q) {
-            const Vec t = *(const Vec*)(row + xl + E * q);
+            for (int q = 0; q < V / E; ++q) {
+              const Vec w4 = *(const Vec*)(row + xl + E * q);
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[u][E * q + e] = t[e];
+              for (int e = 0; e < E; ++e) t[E * q + e] = w4[e];
+            }
+          } else {
+#pragma unroll
+            for (int k = 0; k < V; ++k) t[k] = row[G ? (long long)min(xl + k, X - 1) * fa.sx : min(xl + k, X - 1)];
           }
-        } else {
 #pragma unroll
-          for (int k = 0; k < V; ++k) v[u][k] = row[min(xl + k, X - 1)];
+          for (int k = 0; k < V; ++k) v[u][k] = fi ? v[u][k] + t[k] : t[k];
         }
         wo[u] = (size_t)y * rowWS + (size_t)z * planeWS + word;
         if (++y == Y) {
@@ -196,9 +203,8 @@ __device__ __forceinline__ uint32_t cellKeep(int x0, int X, uint32_t& cellsPerRo
   return keep;
 }
 
-template <typename T, int M, int V, int TY>
-__global__ __launch_bounds__(1024) void kFusedBPFD(const T* __restrict__ f, T thr, int X, int Y, int Z,
-                                                    long long sy, long long sz, TileGrid g,
+template <typename T, int M, int V, int TY, bool G>
+__global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr, int X, int Y, int Z, TileGrid g,
                                                     unsigned long long* __restrict__ bpfd) {
   __shared__ uint32_t hist[16][256];
   __shared__ uint32_t xch[2][TY + 1][16];
@@ -209,14 +215,22 @@ __global__ __launch_bounds__(1024) void kFusedBPFD(const T* __restrict__ f, T th
   for (int i = lane; i < 256; i += kWave) hist[q.w][i] = 0;
 
   const int xb = min(q.x0, X - 1);
-  FieldRows<T, M, V> src;
-  src.fw = f + xb;
+  FieldRows<T, M, V, G> src;
+  if constexpr (G) {
+    for (int i = 0; i < 4; ++i) src.fw[i] = fa.p[i] + (long long)xb * fa.sx;
+    src.nF = fa.nF;
+    src.sx = fa.sx;
+  } else {
+    src.fw[0] = fa.p[0] + xb;  // one field, unit x stride
+    src.nF = 1;
+    src.sx = 1;
+  }
   src.thr = thr;
-  src.sy = sy;
-  src.sz = sz;
+  src.sy = fa.sy;
+  src.sz = fa.sz;
   src.lx = (uint32_t)(min(q.x0 + V * lane, X - 1) - xb);
   src.last = X - 1 - xb;
-  src.full = q.x0 + q.W <= X;
+  src.vec = !G && (V == 1 || q.x0 + q.W <= X);
   src.hxo = min(q.x0 + q.W, X - 1) - xb;
   uint32_t cellsPerRow;
   src.keep = cellKeep<V>(q.x0, X, cellsPerRow);
@@ -286,6 +300,18 @@ __global__ __launch_bounds__(1024) void kBPFD4(const uint32_t* __restrict__ pW, 
 // ---------------------------------------------------------------------------
 static inline long long ceilDiv(long long a, long long b) { return (a + b - 1) / b; }
 
+// Kernel view of a field description; unused table slots repeat field 0.
+template <typename T>
+static FieldArgs<T> fieldArgs(const FieldView& fv) {
+  FieldArgs<T> a;
+  for (int i = 0; i < 4; ++i) a.p[i] = (const T*)fv.p[i < fv.nF ? i : 0];
+  a.nF = fv.nF;
+  a.sx = fv.sx;
+  a.sy = fv.sy;
+  a.sz = fv.sz;
+  return a;
+}
+
 static TileGrid makeGrid(int X, int Y, int Z, const Tiling& tl) {
   TileGrid g;
   g.nSegX = (int)ceilDiv(X - 1, (long long)kWave * tl.V);
@@ -344,8 +370,12 @@ static hipError_t launchPBMT(const FieldView& fv, T thr, uint32_t* pW, uint32_t 
   const long long rowsPerWave = ceilDiv(nRows, wantChunks);
   const long long nWaves = nSegX * ceilDiv(nRows, rowsPerWave);
   const long long nBlocks = ceilDiv(nWaves, kPbmWaves);
-  hipLaunchKernelGGL((kPBM<T, M>), dim3((unsigned)nBlocks), dim3(kPbmWaves * kWave), 0, s, (const T*)fv.p, thr,
-                     fv.X, fv.Y, fv.sy, fv.sz, nSegX, nRows, rowsPerWave, pW, rowWS, planeWS);
+  if (fv.nF > 1 || fv.sx != 1)
+    hipLaunchKernelGGL((kPBM<T, M, true>), dim3((unsigned)nBlocks), dim3(kPbmWaves * kWave), 0, s, fieldArgs<T>(fv),
+                       thr, fv.X, fv.Y, nSegX, nRows, rowsPerWave, pW, rowWS, planeWS);
+  else
+    hipLaunchKernelGGL((kPBM<T, M, false>), dim3((unsigned)nBlocks), dim3(kPbmWaves * kWave), 0, s,
+                       fieldArgs<T>(fv), thr, fv.X, fv.Y, nSegX, nRows, rowsPerWave, pW, rowWS, planeWS);
   return hipGetLastError();
 }
 
@@ -377,8 +407,18 @@ hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint
 template <typename T, int M, int V, int TY>
 static hipError_t launchFusedT(const FieldView& fv, T thr, unsigned long long* bpfd, const TileGrid& g,
                                hipStream_t s) {
-  hipLaunchKernelGGL((kFusedBPFD<T, M, V, TY>), dim3((unsigned)g.chunk * 8u), dim3(g.waves * kWave), 0, s,
-                     (const T*)fv.p, thr, fv.X, fv.Y, fv.Z, fv.sy, fv.sz, g, bpfd);
+  // Several fields or an x stride take the general form, one voxel per lane.
+  const dim3 grid((unsigned)g.chunk * 8u), block(g.waves * kWave);
+  if (fv.nF > 1 || fv.sx != 1) {
+    if constexpr (V == 1)
+      hipLaunchKernelGGL((kFusedBPFD<T, M, 1, TY, true>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X, fv.Y,
+                         fv.Z, g, bpfd);
+    else
+      return hipErrorInvalidValue;
+  } else {
+    hipLaunchKernelGGL((kFusedBPFD<T, M, V, TY, false>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X, fv.Y,
+                       fv.Z, g, bpfd);
+  }
   return hipGetLastError();
 }
 
@@ -408,8 +448,10 @@ static hipError_t dispatchFusedTY(const FieldView& fv, T thr, uint32_t m, unsign
 }
 
 template <typename T, int VW>
-static hipError_t dispatchFused(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
+static hipError_t dispatchFused(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl0,
                                 hipStream_t s) {
+  Tiling tl = tl0;
+  if (fv.nF > 1 || fv.sx != 1) tl.V = 1;  // the general form gathers one voxel per lane
   const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, tl);
   if (g.nTiles <= 0) return hipSuccess;
   if (tl.V == 1) return dispatchFusedTY<T, 1>(fv, thr, m, bpfd, g, tl.Ty, s);

@@ -10,7 +10,7 @@ import torch
 
 import mkf_amd as M
 from mkf_amd import synth
-from tests.gpu_util import dev_bpfd_from_pbm, dev_fused, dev_pbm, host_np
+from tests.gpu_util import dev_bpfd_from_pbm, dev_fused, dev_pbm, enc_of, host_np
 
 pytestmark = pytest.mark.gpu
 
@@ -152,6 +152,83 @@ def test_strided_subvolume(lib, orc, cuda):
     np.testing.assert_array_equal(h.cpu().numpy().view(np.uint64), h_ref)
 
 
+def _all_paths_on(lib, fi, X, Y, Z, relop, thr, cuda):
+    """PBM (binMapCUDA), PBM histogram (mkfCUDAGetBPFD) and fused histogram of
+    one BMFieldInfo -> (words, hist, fused hist)."""
+    bm = M.bin_map(relop, thr)
+    org = M.BMOrg()
+    n = lib.setBMO(C.byref(org), M.dims((X, Y, Z)), b"\x00")
+    words = torch.full((n,), -1, dtype=torch.int32, device=cuda)
+    assert lib.binMapCUDA(None, words.data_ptr(), C.byref(org), fi.ref, C.byref(bm)), M.last_error()
+    h = torch.zeros(256, dtype=torch.int64, device=cuda)
+    assert lib.mkfCUDAGetBPFD(None, h.data_ptr(), C.byref(org), words.data_ptr(), 3)
+    hf = torch.zeros(256, dtype=torch.int64, device=cuda)
+    M.check(lib.mkfFusedBPFD(None, hf.data_ptr(), None, fi.ref, C.byref(bm), None, None), "fused")
+    torch.cuda.synchronize()
+    return (words.cpu().numpy().view(np.uint32).reshape(Z, Y, org.rowWS), h.cpu().numpy().view(np.uint64),
+            hf.cpu().numpy().view(np.uint64))
+
+
+@pytest.mark.parametrize("nf", [2, 3, 4])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_multi_field_sum(lib, orc, cuda, nf, dtype):
+    """Several enabled fields are thresholded as their sum in table order, in
+    the element type (src/binMapCUDA.cuh:196-201)."""
+    print(f"multi-field {nf} {dtype}", flush=True)
+    Z, Y, X = 23, 29, 135
+    fs = [synth.grf_field((Z, Y, X), sigma=1.5, seed=60 + i, device=cuda, dtype=dtype) for i in range(nf)]
+    s = host_np(fs[0]).copy()
+    for f in fs[1:]:
+        s = s + host_np(f)  # same element type, same order
+    w_ref, h_ref = oracle_all(orc, s, ">=", 0.25)
+    fi = M.FieldInfo([f.data_ptr() for f in fs], (X, Y, Z), enc_of(fs[0]))
+    w, h, hf = _all_paths_on(lib, fi, X, Y, Z, ">=", 0.25, cuda)
+    np.testing.assert_array_equal(w, w_ref)
+    np.testing.assert_array_equal(h, h_ref)
+    np.testing.assert_array_equal(hf, h_ref)
+
+
+def test_multi_field_table_mask_and_limit(lib, orc, cuda):
+    """Entries outside the mask and NULL entries are skipped; at most 4 of the
+    remaining ones are summed (BMCU_FIELD_MAX)."""
+    Z, Y, X = 11, 13, 70
+    fs = [synth.grf_field((Z, Y, X), sigma=1.0, seed=70 + i, device=cuda, dtype=torch.float32) for i in range(6)]
+    ptrs = [fs[0].data_ptr(), fs[1].data_ptr(), None, fs[2].data_ptr(), fs[3].data_ptr(), fs[4].data_ptr(),
+            fs[5].data_ptr()]
+    # entry 1 is valid but not enabled, entry 2 enabled but NULL
+    fi = M.FieldInfo(ptrs, (X, Y, Z), M.ENC_F32, mask=0b1111101)
+    used = [fs[0], fs[2], fs[3], fs[4]]  # entries 0, 3, 4, 5: the first four enabled non-NULL ones
+    s = host_np(used[0]).copy()
+    for f in used[1:]:
+        s = s + host_np(f)
+    w_ref, h_ref = oracle_all(orc, s, "<", 0.0)
+    w, h, hf = _all_paths_on(lib, fi, X, Y, Z, "<", 0.0, cuda)
+    np.testing.assert_array_equal(w, w_ref)
+    np.testing.assert_array_equal(hf, h_ref)
+
+
+@pytest.mark.parametrize("sx", [2, 3])
+def test_x_strided_field(lib, orc, cuda, sx):
+    """pS = (sx, sy, sz): an x-strided view (every sx-th element of a row)."""
+    Z, Y, X = 17, 19, 150
+    base = synth.grf_field((Z, Y, X * sx), sigma=2.0, seed=80 + sx, device=cuda, dtype=torch.float64)
+    view = host_np(base)[:, :, ::sx]
+    w_ref, h_ref = oracle_all(orc, np.ascontiguousarray(view), ">=", 0.1)
+    fi = M.FieldInfo(base.data_ptr(), (X, Y, Z), M.ENC_F64, strides=(sx, X * sx, X * sx * Y))
+    w, h, hf = _all_paths_on(lib, fi, X, Y, Z, ">=", 0.1, cuda)
+    np.testing.assert_array_equal(w, w_ref)
+    np.testing.assert_array_equal(h, h_ref)
+    np.testing.assert_array_equal(hf, h_ref)
+    # two strided fields summed
+    base2 = synth.grf_field((Z, Y, X * sx), sigma=2.0, seed=90 + sx, device=cuda, dtype=torch.float64)
+    s = view + host_np(base2)[:, :, ::sx]
+    _, h_ref2 = oracle_all(orc, np.ascontiguousarray(s), ">=", 0.1)
+    fi2 = M.FieldInfo([base.data_ptr(), base2.data_ptr()], (X, Y, Z), M.ENC_F64,
+                      strides=(sx, X * sx, X * sx * Y))
+    _, _, hf2 = _all_paths_on(lib, fi2, X, Y, Z, ">=", 0.1, cuda)
+    np.testing.assert_array_equal(hf2, h_ref2)
+
+
 def test_invalid_inputs_return_null(lib, cuda):
     bm = M.bin_map(">=", 0.5)
     org = M.BMOrg()
@@ -161,7 +238,7 @@ def test_invalid_inputs_return_null(lib, cuda):
     assert not lib.binMapCUDA(None, None, C.byref(org), fi.ref, C.byref(bm))
     fi = M.FieldInfo(f.data_ptr(), (40, 4, 4), 0x08)  # ENC_U8: not a float field
     assert not lib.binMapCUDA(None, None, C.byref(org), fi.ref, C.byref(bm))
-    fi = M.FieldInfo(f.data_ptr(), (40, 4, 4), M.ENC_F32, strides=(2, 80, 320))
+    fi = M.FieldInfo(f.data_ptr(), (40, 4, 4), M.ENC_F32, strides=(0, 80, 320))
     assert not lib.binMapCUDA(None, None, C.byref(org), fi.ref, C.byref(bm))
     assert "stride" in M.last_error()
     assert lib.mkfCUDAGetBPFD(None, None, C.byref(org), None, 3) is None
@@ -266,3 +343,30 @@ def test_config_odd_dims_sdf(lib, orc, cuda):
     """509 x 511 x 517 float32 SDF of 32 spheres and 8 boxes, "<=" 0."""
     f = synth.sdf_field((517, 511, 509), seed=11, device=cuda, dtype=torch.float32)
     check_all_paths(lib, orc, f, "<=", 0.0)
+
+
+def test_auto_context_two_fields(lib, orc, cuda):
+    """A context holding two fields back to back (autoFieldPtr): the PBM is
+    of their sum."""
+    X, Y, Z = 70, 30, 20
+    a = synth.uniform_field((Z, Y, X), seed=3)
+    b = synth.uniform_field((Z, Y, X), seed=4)
+    f = np.ascontiguousarray(np.stack([a, b]))  # field 0 then field 1
+    w_ref, h_ref = oracle_all(orc, a + b, ">=", 1.0)
+    ctx = M.Context()
+    ctx.nField, ctx.nElem, ctx.enc, ctx.bytesElem = 2, X * Y * Z, M.ENC_F32, 4
+    ctx.bytesF = f.nbytes
+    ctx.nU = w_ref.size
+    ctx.bytesU = w_ref.size * 4
+    ctx.nZ, ctx.bytesZ = 256, 2048
+    hu = np.zeros(w_ref.size, dtype=np.uint32)
+    hz = np.zeros(256, dtype=np.uint64)
+    ctx.pHF, ctx.pHU, ctx.pHZ = f.ctypes.data, hu.ctypes.data, hz.ctypes.data
+    dz = torch.zeros(256, dtype=torch.int64, device=cuda)
+    ctx.pDZ = dz.data_ptr()
+    assert lib.mkfCUDAGetBPFDautoCtx(C.byref(ctx), M.dims((X, Y, Z)), C.byref(M.bin_map(">=", 1.0)), 0) == 1
+    np.testing.assert_array_equal(hu.reshape(w_ref.shape), w_ref)
+    np.testing.assert_array_equal(hz, h_ref)
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipFree(C.c_void_p(ctx.pDF))
+    hip.hipFree(C.c_void_p(ctx.pDU))
