@@ -107,6 +107,21 @@ int mkfBatchInfo(const MkfBatch *b, int *nItems, MkfBatchShape *shape, int *grid
 void mkfBatchDestroy(MkfBatch *b);
 
 /* ---------------------------------------------------------------------------
+ * Host fields streamed through the device in z-slabs
+ * ------------------------------------------------------------------------- */
+/* Adds the histogram of a HOST field (planar, f32 or f64) to pBPFDH (host,
+ * 256 counts). The field goes through two device slab buffers of
+ * slabPlanes + 1 planes (slabPlanes <= 0: about 256 MiB per slab):
+ * the copy of slab k+1 overlaps the fused kernel on slab k, and consecutive
+ * slabs share one plane, so device memory stays bounded for any field size.
+ * Blocking. Pinned host memory lets the copies run at the full link rate.
+ * No counterpart in the reference, whose host-field path
+ * (mkfAccCUDAGetBPFD, src/mkfACC.c:248-318) copies the whole field; that
+ * entry point uses this one here. */
+int mkfHostStreamBPFD(size_t *pBPFDH, const void *pF, NumEnc enc, const FieldDef def[3], const BinMapF64 *pM,
+                      int slabPlanes);
+
+/* ---------------------------------------------------------------------------
  * Serving form: a request loop with continuous batching
  *
  * A worker thread admits queued requests while fewer than maxInFlight are

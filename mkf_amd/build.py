@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 C_FLAGS = ["-O2", "-std=gnu11", "-fPIC", "-Wall", f"-I{INCLUDE}"]
 
-HIP_SOURCES = ["mkf_kernels.hip", "mkf_api.hip", "mkf_batch.hip", "mkf_serve.hip"]
+HIP_SOURCES = ["mkf_kernels.hip", "mkf_api.hip", "mkf_batch.hip", "mkf_serve.hip", "mkf_stream.hip"]
 C_SOURCES = ["binmap_util.c", "measures.c"]
 
 

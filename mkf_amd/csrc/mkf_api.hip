@@ -366,34 +366,14 @@ Bool32 mkfAccGetBPFDSimple(size_t rBPFD[MKF_BINS], BMPackWord* pW, const double*
 }
 
 // src/mkfACC.c:248-318: host field of either encoding in, host histogram
-// out. The PBM is not needed by the caller, so the fused kernel is used.
+// out. The PBM is not needed by the caller, so the field is streamed through
+// the fused kernel in z-slabs (mkfHostStreamBPFD): bounded device memory,
+// copies overlapping the kernel.
 Bool32 mkfAccCUDAGetBPFD(size_t rBPFD[MKF_BINS], const void* pF, const FieldDef def[3], const NumEnc enc,
                          const BinMapF64* pM) {
   if (!rBPFD || !pF || !def || !pM || (enc != ENC_F32 && enc != ENC_F64)) return 0;
-  const size_t eb = enc & ENC_MASK_NUM;
-  FieldDef d[3];
-  if (copyNonZeroDef(d, def, 3) <= 0) return 0;
-  const size_t nF = (size_t)d[0] * d[1] * d[2];
-  void* dF = nullptr;
-  unsigned long long* dH = nullptr;
-  Bool32 ok = 0;
-  if (hipMalloc(&dF, nF * eb) == hipSuccess && hipMalloc((void**)&dH, MKF_BINS * 8) == hipSuccess &&
-      hipMemcpy(dF, pF, nF * eb, hipMemcpyHostToDevice) == hipSuccess &&
-      hipMemset(dH, 0, MKF_BINS * 8) == hipSuccess) {
-    const FieldView fv = planarField(dF, enc, d[0], d[1], d[2]);
-    const Tiling tl = chooseTiling(fv.X, fv.Y, fv.Z, (int)eb);
-    if (launchFused(fv, *pM, dH, tl, 0) == hipSuccess &&
-        hipMemcpy(rBPFD, dH, MKF_BINS * 8, hipMemcpyDeviceToHost) == hipSuccess) {
-      ok = 1;
-    } else {
-      hipFail(hipGetLastError(), "mkfAccCUDAGetBPFD");
-    }
-  } else {
-    fail(MKF_ERR_MEM, "mkfAccCUDAGetBPFD: device buffers for %zu voxels", nF);
-  }
-  if (dF) hipFree(dF);
-  if (dH) hipFree(dH);
-  return ok;
+  memset(rBPFD, 0, sizeof(size_t) * MKF_BINS);
+  return mkfHostStreamBPFD(rBPFD, pF, enc, def, pM, 0) == MKF_OK;
 }
 
 }  // extern "C"

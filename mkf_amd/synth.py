@@ -88,7 +88,7 @@ def _grf_numpy(shape, sigma, seed):
     F *= np.exp(c * kz * kz)[:, None, None]
     F *= np.exp(c * ky * ky)[None, :, None]
     F *= np.exp(c * kx * kx)[None, None, :]
-    f = np.fft.irfftn(F, s=(Z, Y, X))
+    f = np.fft.irfftn(F, s=(Z, Y, X), axes=(0, 1, 2))
     f -= f.mean()
     f /= np.sqrt((f * f).mean())
     return np.ascontiguousarray(f)

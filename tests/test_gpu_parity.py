@@ -370,3 +370,45 @@ def test_auto_context_two_fields(lib, orc, cuda):
     hip = C.CDLL("libamdhip64.so.7")
     hip.hipFree(C.c_void_p(ctx.pDF))
     hip.hipFree(C.c_void_p(ctx.pDU))
+
+
+@pytest.mark.parametrize("slab", [1, 3, 7, 0, 1000])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_host_stream_slabs(lib, orc, cuda, slab, dtype):
+    """A host field streamed through the fused kernel in z-slabs that share
+    one plane: every plane pair is counted once, for any slab depth."""
+    print(f"host stream slab={slab} {dtype}", flush=True)
+    Z, Y, X = 37, 29, 101
+    f = synth.grf_field((Z, Y, X), sigma=2.0, seed=12, device="cpu", dtype=torch.float64).numpy()
+    f = np.ascontiguousarray(f.astype(dtype))
+    enc = M.ENC_F32 if dtype == np.float32 else M.ENC_F64
+    _, h_ref = oracle_all(orc, f, ">=", 0.0)
+    h = np.full(256, 5, dtype=np.uint64)  # the histogram is added to
+    bm = M.bin_map(">=", 0.0)
+    M.check(lib.mkfHostStreamBPFD(h.ctypes.data_as(C.POINTER(C.c_size_t)), f.ctypes.data, enc, M.dims((X, Y, Z)),
+                                  C.byref(bm), slab), "mkfHostStreamBPFD")
+    np.testing.assert_array_equal(h, h_ref + 5)
+
+
+def test_host_stream_pinned_and_edges(lib, orc, cuda):
+    bm = M.bin_map("<", 0.3)
+    f = synth.grf_field((64, 48, 300), sigma=3.0, seed=13, device=cuda, dtype=torch.float32)
+    p = torch.empty(f.shape, dtype=f.dtype, pin_memory=True)
+    p.copy_(f)
+    _, h_ref = oracle_all(orc, p.numpy(), "<", 0.3)
+    h = np.zeros(256, dtype=np.uint64)
+    M.check(lib.mkfHostStreamBPFD(h.ctypes.data_as(C.POINTER(C.c_size_t)), p.data_ptr(), M.ENC_F32,
+                                  M.dims((300, 48, 64)), C.byref(bm), 5))
+    np.testing.assert_array_equal(h, h_ref)
+    # two planes: one pair, one slab
+    two = np.ascontiguousarray(p.numpy()[:2])
+    _, h2 = oracle_all(orc, two, "<", 0.3)
+    h = np.zeros(256, dtype=np.uint64)
+    M.check(lib.mkfHostStreamBPFD(h.ctypes.data_as(C.POINTER(C.c_size_t)), two.ctypes.data, M.ENC_F32,
+                                  M.dims((300, 48, 2)), C.byref(bm), 0))
+    np.testing.assert_array_equal(h, h2)
+    # a dim of one: no cells, nothing added
+    h = np.zeros(256, dtype=np.uint64)
+    M.check(lib.mkfHostStreamBPFD(h.ctypes.data_as(C.POINTER(C.c_size_t)), two.ctypes.data, M.ENC_F32,
+                                  M.dims((300, 48, 1)), C.byref(bm), 0))
+    assert not h.any()
