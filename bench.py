@@ -402,11 +402,14 @@ def main():
     eb = 4 if args.dtype == "f32" else 8
     fi = M.FieldInfo(field.data_ptr(), (S, S, S), enc)
     bm = M.bin_map(">=", args.threshold)
-    tiling = None
     if args.tiling:
         tiling = M.MkfTiling(*[int(v) for v in args.tiling.split(",")])
     else:
-        tiling = lib.mkfDefaultTiling(M.dims((S, S, S)), eb)
+        # The library's autotuner picks the tiling on this device and field.
+        tiling = M.MkfTiling()
+        M.check(lib.mkfFusedAutotune(fi.ref, C.byref(bm), C.byref(tiling), C.c_void_p(0)), "mkfFusedAutotune")
+        log(f"[bench] autotuned tiling: planePairs {tiling.planePairs}, waves {tiling.waves}, "
+            f"rowPairs {tiling.rowPairs}, voxelsPerLane {tiling.voxelsPerLane}")
     stream = torch.cuda.current_stream()
     sptr = C.c_void_p(stream.cuda_stream)
     hist = torch.zeros(256, dtype=torch.int64, device=dev)
@@ -490,7 +493,8 @@ def main():
         "data": f"synthetic: seeded GRF (N(0,1) noise, periodic Gaussian blur sigma={args.sigma:g}, standardised), seed {args.seed}",
         "config": {"workload": workload, "dims": [S, S, S], "elem_bytes": eb, "relation": ">=",
                    "threshold": args.threshold,
-                   "tiling": {"planePairs": tiling.planePairs, "waves": tiling.waves, "rowPairs": tiling.rowPairs}},
+                   "tiling": {"planePairs": tiling.planePairs, "waves": tiling.waves, "rowPairs": tiling.rowPairs,
+                              "voxelsPerLane": tiling.voxelsPerLane, "autotuned": not args.tiling}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel_ms_mean": round(kern_avg, 4), "kernel_ms_min": round(min(kern_ms), 4),
@@ -505,7 +509,6 @@ def main():
     nwords = lib.setBMO(C.byref(org), M.dims((S, S, S)), b"\x00")
     pbm = torch.empty(nwords, dtype=torch.int32, device=dev)
     h2 = torch.zeros(256, dtype=torch.int64, device=dev)
-    t1 = lib.mkfDefaultTiling(M.dims((S, S, S)), 4)
     pbm_ms, bpfd_ms = [], []
     for i in range(max(3, K // 4) + 2):
         e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
@@ -513,7 +516,7 @@ def main():
         e[0].record(stream)
         M.check(lib.mkfPBMAsync(pbm.data_ptr(), C.byref(org), fi.ref, C.byref(bm), sptr), "mkfPBMAsync")
         e[1].record(stream)
-        M.check(lib.mkfBPFDAsync(h2.data_ptr(), C.byref(org), pbm.data_ptr(), C.byref(t1), sptr), "mkfBPFDAsync")
+        M.check(lib.mkfBPFDAsync(h2.data_ptr(), C.byref(org), pbm.data_ptr(), None, sptr), "mkfBPFDAsync")
         e[2].record(stream)
         torch.cuda.synchronize()
         if i >= 2:

@@ -24,9 +24,10 @@ extern "C" {
 /* Work decomposition of the histogram kernels; NULL selects the built-in
  * choice for the field's shape. A workgroup of `waves` (1..16) waves sits
  * side by side along x, each lane holding voxelsPerLane consecutive voxels
- * (1, or 4 for float / 2 for double fields; 0 = the library's choice; the
- * PBM histogram always uses 1), holds rowPairs (2, 4 or 8) row pairs and
- * marches over planePairs (>= 1) plane pairs. */
+ * (1, or 4 for float / 2 for double fields, 0 = the library's choice; the
+ * histogram of a PBM uses 4 unless 1 is asked for; several fields or an x
+ * stride use 1), holds rowPairs (2, 4 or 8) row pairs and marches over
+ * planePairs (>= 1) plane pairs. */
 typedef struct {
   int32_t planePairs;
   int32_t waves;
@@ -51,8 +52,15 @@ int mkfPBMAsync(BMPackWord *pW, BMOrg *pO, const BMFieldInfo *pI, const BinMapF6
 /* Stream-ordered histogram of a packed binary map; adds into pBPFD. */
 int mkfBPFDAsync(size_t *pBPFD, const BMOrg *pO, const BMPackWord *pW, const MkfTiling *pT, void *stream);
 
-/* Tiling the library would choose for a field of dims def and element size. */
+/* Tiling the library chooses for the fused kernel on a field of dims def
+ * and element size (the histogram of a PBM has its own default). */
 MkfTiling mkfDefaultTiling(const FieldDef def[3], int elemBytes);
+
+/* Times the fused kernel on this field (the caller's data, a scratch
+ * histogram) under a few tilings, keeps the fastest for fields of this shape
+ * and encoding (process-wide), and stores it in *pChosen (may be NULL).
+ * Later mkfFusedBPFD calls without a tiling use it. Blocking. */
+int mkfFusedAutotune(const BMFieldInfo *pI, const BinMapF64 *pM, MkfTiling *pChosen, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Batched form: many fields x many threshold levels in one launch

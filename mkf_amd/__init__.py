@@ -144,6 +144,7 @@ _SIGNATURES = {
     "mkfBatchAutotune": (C.c_int, [_P, C.POINTER(MkfBatchShape), C.c_int, C.c_int, C.POINTER(MkfBatchShape), _P]),
     "mkfBatchInfo": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(MkfBatchShape), C.POINTER(C.c_int)]),
     "mkfBatchDestroy": (None, [_P]),
+    "mkfFusedAutotune": (C.c_int, [C.POINTER(BMFieldInfo), C.POINTER(BinMapF64), C.POINTER(MkfTiling), _P]),
     "mkfHostStreamBPFD": (C.c_int, [_SZP, _P, C.c_uint8, C.POINTER(C.c_int), C.POINTER(BinMapF64), C.c_int]),
     "mkfServeCreate": (_P, [C.POINTER(MkfServeConfig)]),
     "mkfServeSubmit": (C.c_int, [_P, C.POINTER(MkfServeRequest)]),

@@ -10,7 +10,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
 #include <mutex>
+#include <tuple>
+#include <vector>
 
 #include "mkf_internal.h"
 
@@ -119,6 +122,36 @@ int resolveField(const BMFieldInfo* pI, FieldView* fv, FieldDef def[3]) {
   return MKF_OK;
 }
 
+// Fused-kernel tilings chosen by mkfFusedAutotune, per field shape.
+struct ShapeKey {
+  int X, Y, Z, enc, nF;
+  long long sx;
+  bool operator<(const ShapeKey& o) const {
+    return std::tie(X, Y, Z, enc, nF, sx) < std::tie(o.X, o.Y, o.Z, o.enc, o.nF, o.sx);
+  }
+};
+std::mutex gTuneLock;
+std::map<ShapeKey, Tiling> gTuned;
+
+ShapeKey keyOf(const FieldView& fv) { return ShapeKey{fv.X, fv.Y, fv.Z, (int)fv.enc, fv.nF, fv.sx}; }
+
+// The fused kernel's tiling: the caller's, else a tuned one, else the
+// built-in choice.
+Tiling fusedTiling(const MkfTiling* pT, const FieldView& fv) {
+  const int eb = fv.enc & ENC_MASK_NUM;
+  if (pT) {
+    Tiling tl{pT->planePairs, pT->waves, pT->rowPairs, pT->voxelsPerLane};
+    if (tl.V == 0) tl.V = chooseTiling(fv.X, fv.Y, fv.Z, eb).V;
+    return tl;
+  }
+  {
+    std::lock_guard<std::mutex> g(gTuneLock);
+    auto it = gTuned.find(keyOf(fv));
+    if (it != gTuned.end()) return it->second;
+  }
+  return chooseTiling(fv.X, fv.Y, fv.Z, eb);
+}
+
 Tiling toTiling(const MkfTiling* pT, int X, int Y, int Z, int elemBytes) {
   if (pT) {
     Tiling tl{pT->planePairs, pT->waves, pT->rowPairs, pT->voxelsPerLane};
@@ -163,8 +196,8 @@ int mkfBPFDAsync(size_t* pBPFD, const BMOrg* pO, const BMPackWord* pW, const Mkf
   if (!pBPFD || !pO || !pW) return fail(MKF_ERR_ARG, "NULL argument");
   const int X = (int)pO->rowElem, Y = (int)pO->rowPairs + 1, Z = (int)pO->planePairs + 1;
   // 4 voxels per lane (nibbles of the PBM words), or 1 when asked for.
-  Tiling tl = toTiling(pT, X, Y, Z, 4);
-  tl.V = (tl.V == 1) ? 1 : 4;
+  Tiling tl = chooseBpfdTiling(X, Y, Z);
+  if (pT) tl = Tiling{pT->planePairs, pT->waves, pT->rowPairs, pT->voxelsPerLane == 1 ? 1 : 4};
   int rc = checkTiling(tl, 4);
   if (rc) return rc;
   const hipError_t e = launchBPFD(pW, X, Y, Z, pO->rowWS, pO->planeWS, (unsigned long long*)pBPFD, tl,
@@ -180,7 +213,7 @@ int mkfFusedBPFD(KernInfo* pK, size_t* pBPFD, BMOrg* pO, const BMFieldInfo* pI, 
   int rc = resolveField(pI, &fv, def);
   if (rc) return rc;
   if (pO) setBMO(pO, def, 0);
-  const Tiling tl = toTiling(pT, fv.X, fv.Y, fv.Z, fv.enc & ENC_MASK_NUM);
+  const Tiling tl = fusedTiling(pT, fv);
   rc = checkTiling(tl, fv.enc & ENC_MASK_NUM);
   if (rc) return rc;
   const hipStream_t s = (hipStream_t)stream;
@@ -195,6 +228,65 @@ int mkfFusedBPFD(KernInfo* pK, size_t* pBPFD, BMOrg* pO, const BMFieldInfo* pI, 
   }
   const hipError_t e = launchFused(fv, *pM, (unsigned long long*)pBPFD, tl, s);
   return e == hipSuccess ? MKF_OK : hipFail(e, "fused launch");
+}
+
+int mkfFusedAutotune(const BMFieldInfo* pI, const BinMapF64* pM, MkfTiling* pChosen, void* stream) {
+  if (!pM) return fail(MKF_ERR_ARG, "NULL map");
+  FieldView fv;
+  FieldDef def[3];
+  int rc = resolveField(pI, &fv, def);
+  if (rc) return rc;
+  const int eb = fv.enc & ENC_MASK_NUM;
+  // The built-in choice and the shapes that came out ahead of it on some
+  // MI355X at 1024^3; several fields or an x stride run one voxel per lane.
+  std::vector<Tiling> cand{chooseTiling(fv.X, fv.Y, fv.Z, eb)};
+  if (eb == 4) {
+    cand.push_back(Tiling{256, 16, 4, 4});
+    cand.push_back(Tiling{64, 16, 4, 1});
+  } else {
+    cand.push_back(Tiling{128, 16, 8, 1});
+    cand.push_back(Tiling{512, 16, 4, 2});
+  }
+  const hipStream_t s = (hipStream_t)stream;
+  unsigned long long* scratch = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipMalloc((void**)&scratch, MKF_BINS * 8) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    if (scratch) (void)hipFree(scratch);
+    if (e0) (void)hipEventDestroy(e0);
+    return fail(MKF_ERR_MEM, "autotune scratch");
+  }
+  Tiling best = cand[0];
+  float bestMs = 1e30f;
+  rc = MKF_OK;
+  for (const Tiling& tl : cand) {
+    if (!tilingSupported(tl, eb)) continue;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launchFused(fv, *pM, scratch, tl, s);
+    (void)hipEventRecord(e0, s);
+    for (int i = 0; i < 5 && e == hipSuccess; ++i) e = launchFused(fv, *pM, scratch, tl, s);
+    (void)hipEventRecord(e1, s);
+    if (e != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      rc = hipFail(e != hipSuccess ? e : hipGetLastError(), "autotune launch");
+      break;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < bestMs) {
+      bestMs = ms;
+      best = tl;
+    }
+  }
+  (void)hipFree(scratch);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != MKF_OK) return rc;
+  {
+    std::lock_guard<std::mutex> g(gTuneLock);
+    gTuned[keyOf(fv)] = best;
+  }
+  if (pChosen) *pChosen = MkfTiling{best.TP, best.ZW, best.Ty, best.V};
+  return MKF_OK;
 }
 
 // src/binMapCUDA.cu:145-264.
@@ -347,7 +439,7 @@ Bool32 mkfAccGetBPFDSimple(size_t rBPFD[MKF_BINS], BMPackWord* pW, const double*
       hipMemcpy(dF, pF, nF * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
       hipMemset(dH, 0, MKF_BINS * 8) == hipSuccess) {
     const FieldView fv = planarField(dF, ENC_F64, def[0], def[1], def[2]);
-    const Tiling tl = chooseTiling(fv.X, fv.Y, fv.Z, 8);
+    const Tiling tl = chooseBpfdTiling(fv.X, fv.Y, fv.Z);
     if (launchPBM(fv, *pM, dW, org.rowWS, org.planeWS, 0) == hipSuccess &&
         launchBPFD(dW, fv.X, fv.Y, fv.Z, org.rowWS, org.planeWS, dH, tl, 0) == hipSuccess &&
         hipMemcpy(pW, dW, words * 4, hipMemcpyDeviceToHost) == hipSuccess &&

@@ -49,7 +49,8 @@ struct Tiling {
 // code.
 int setError(int code, const char* fmt, ...);
 
-Tiling chooseTiling(int X, int Y, int Z, int elemBytes);
+Tiling chooseTiling(int X, int Y, int Z, int elemBytes);  // fused kernel
+Tiling chooseBpfdTiling(int X, int Y, int Z);              // histogram of a PBM
 bool tilingSupported(const Tiling& tl, int elemBytes);
 
 // Thresholds fv into the PBM pW (rowWS / planeWS word strides).

@@ -330,30 +330,52 @@ bool tilingSupported(const Tiling& tl, int elemBytes) {
   return vOk && (tl.Ty == 2 || tl.Ty == 4 || tl.Ty == 8) && tl.TP >= 1 && tl.ZW >= 1 && tl.ZW <= 16;
 }
 
-// Workgroups as wide as the row (up to 16 waves) and the longest z-range
-// that still gives every CU about 64 lanes' worth of voxels x 16 waves.
-// Measured on MI355X at 1024^3 (HIP events, warm clocks):
-//   f32: 4 voxels/lane, 4 row pairs, 256 plane pairs  0.723 ms (5.94 TB/s);
-//        1 voxel/lane, 4 row pairs, 128 plane pairs   0.730 ms;
-//        4 voxels/lane, 8 row pairs                   0.76-0.77 ms.
-//   f64: 1 voxel/lane, 8 row pairs, 128 plane pairs   1.342 ms (6.40 TB/s);
-//        2 voxels/lane, 8 row pairs                   1.41 ms.
+// Fused kernel: workgroups as wide as the row (up to 16 waves) and the
+// longest slab that still gives the grid a dtype's target number of waves.
+// Measured on MI355X at 1024^3 (HIP events, warm clocks, 5 interleaved runs
+// of 30 launches on one machine):
+//   f32: 1 voxel/lane, 4 row pairs, 128 plane pairs   0.714 ms (6.02 TB/s)
+//        4 voxels/lane, 4 row pairs, 256 plane pairs  0.729 ms
+//        1 voxel/lane, 4 row pairs, 64 plane pairs    0.724 ms
+//   f64: 1 voxel/lane, 8 row pairs, 512 plane pairs   1.306 ms (6.58 TB/s)
+//        1 voxel/lane, 8 row pairs, 128 plane pairs   1.343 ms
+//        2 voxels/lane, 4 row pairs, 512 plane pairs  1.323 ms
+// (another machine had the f32 order of the first two reversed by 1 %).
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   Tiling tl;
+  long long targetWaves;
   if (elemBytes >= 8) {
     tl.V = 1;
     tl.Ty = 8;
-    tl.TP = 128;
+    tl.TP = 512;
+    targetWaves = 4096;
   } else {
-    tl.V = 4;
+    tl.V = 1;
     tl.Ty = 4;
-    tl.TP = 256;
+    tl.TP = 128;
+    targetWaves = 16384;
   }
   tl.ZW = 16;
-  const long long targetLaneWork = 256LL * 16 * 4;  // waves x voxels per lane
   while (tl.TP > 8) {
     const TileGrid g = makeGrid(X, Y, Z, tl);
-    if ((long long)g.nTiles * g.waves * tl.V >= targetLaneWork) break;
+    if ((long long)g.nTiles * g.waves >= targetWaves) break;
+    tl.TP >>= 1;
+  }
+  return tl;
+}
+
+// Histogram of an existing PBM: 4 voxels per lane (kBPFD4), 4 row pairs,
+// slabs of up to 256 plane pairs halved until the grid has about 16K
+// wave-lanes' worth of work (1024^3: 0.34 ms).
+Tiling chooseBpfdTiling(int X, int Y, int Z) {
+  Tiling tl;
+  tl.V = 4;
+  tl.Ty = 4;
+  tl.TP = 256;
+  tl.ZW = 16;
+  while (tl.TP > 8) {
+    const TileGrid g = makeGrid(X, Y, Z, tl);
+    if ((long long)g.nTiles * g.waves * tl.V >= 256LL * 16 * 4) break;
     tl.TP >>= 1;
   }
   return tl;

@@ -412,3 +412,20 @@ def test_host_stream_pinned_and_edges(lib, orc, cuda):
     M.check(lib.mkfHostStreamBPFD(h.ctypes.data_as(C.POINTER(C.c_size_t)), two.ctypes.data, M.ENC_F32,
                                   M.dims((300, 48, 1)), C.byref(bm), 0))
     assert not h.any()
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_fused_autotune(lib, orc, cuda, dtype):
+    """The autotuner leaves the caller's histogram alone, returns a supported
+    tiling, and later calls without a tiling (which use it) stay exact."""
+    f = synth.grf_field((40, 50, 333), sigma=2.0, seed=14, device=cuda, dtype=dtype)
+    Z, Y, X = f.shape
+    fi = M.FieldInfo(f.data_ptr(), (X, Y, Z), enc_of(f))
+    bm = M.bin_map(">", -0.2)
+    t = M.MkfTiling()
+    M.check(lib.mkfFusedAutotune(fi.ref, C.byref(bm), C.byref(t), None), "autotune")
+    assert t.planePairs >= 1 and 1 <= t.waves <= 16 and t.rowPairs in (2, 4, 8)
+    assert t.voxelsPerLane in ((1, 4) if dtype == torch.float32 else (1, 2))
+    _, h_ref = oracle_all(orc, host_np(f), ">", -0.2)
+    h, _ = dev_fused(lib, f, ">", -0.2)
+    np.testing.assert_array_equal(h, h_ref)
