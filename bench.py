@@ -471,12 +471,16 @@ def main():
         f"(min {min(kern_ms):.3f}), {achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.1%} of HBM peak")
 
     workload = f"fused PBM+BPFD, {S}x{S}x{S} {'float32' if eb == 4 else 'float64'} GRF (sigma {args.sigma:g}), >= {args.threshold:g}"
+    # HBM bytes per launch from the rocprof FETCH_SIZE / WRITE_SIZE runs in
+    # profiles/, per field shape, dtype and tiling.
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "fused_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            tab = json.load(fh)
-        traffic = tab.get(f"{S}_{args.dtype}")
+            ent = json.load(fh).get(f"{S}_{args.dtype}")
+        if isinstance(ent, dict):
+            ent = ent.get(f"{tiling.planePairs},{tiling.waves},{tiling.rowPairs},{tiling.voxelsPerLane}")
+        traffic = ent
 
     result = {
         "metric": METRIC,

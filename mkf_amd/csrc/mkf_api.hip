@@ -250,33 +250,48 @@ int mkfFusedAutotune(const BMFieldInfo* pI, const BinMapF64* pM, MkfTiling* pCho
   const hipStream_t s = (hipStream_t)stream;
   unsigned long long* scratch = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  rc = MKF_OK;
   if (hipMalloc((void**)&scratch, MKF_BINS * 8) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
       hipEventCreate(&e1) != hipSuccess) {
     if (scratch) (void)hipFree(scratch);
     if (e0) (void)hipEventDestroy(e0);
     return fail(MKF_ERR_MEM, "autotune scratch");
   }
-  Tiling best = cand[0];
-  float bestMs = 1e30f;
-  rc = MKF_OK;
-  for (const Tiling& tl : cand) {
-    if (!tilingSupported(tl, eb)) continue;
-    hipError_t e = hipSuccess;
-    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launchFused(fv, *pM, scratch, tl, s);
+  // Warm-up of about 0.3 s so the clocks have left idle, then three rounds
+  // over the candidates; a candidate scores its best round.
+  hipError_t e = hipSuccess;
+  {
+    float spent = 0.f;
     (void)hipEventRecord(e0, s);
-    for (int i = 0; i < 5 && e == hipSuccess; ++i) e = launchFused(fv, *pM, scratch, tl, s);
-    (void)hipEventRecord(e1, s);
-    if (e != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-      rc = hipFail(e != hipSuccess ? e : hipGetLastError(), "autotune launch");
-      break;
-    }
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    if (ms < bestMs) {
-      bestMs = ms;
-      best = tl;
+    while (e == hipSuccess && spent < 300.f) {
+      for (int i = 0; i < 8 && e == hipSuccess; ++i) e = launchFused(fv, *pM, scratch, cand[0], s);
+      (void)hipEventRecord(e1, s);
+      if (hipEventSynchronize(e1) != hipSuccess) break;
+      (void)hipEventElapsedTime(&spent, e0, e1);
     }
   }
+  std::vector<float> score(cand.size(), 1e30f);
+  for (int round = 0; round < 3 && e == hipSuccess; ++round) {
+    for (size_t c = 0; c < cand.size() && e == hipSuccess; ++c) {
+      if (!tilingSupported(cand[c], eb)) continue;
+      e = launchFused(fv, *pM, scratch, cand[c], s);
+      (void)hipEventRecord(e0, s);
+      for (int i = 0; i < 5 && e == hipSuccess; ++i) e = launchFused(fv, *pM, scratch, cand[c], s);
+      (void)hipEventRecord(e1, s);
+      if (e != hipSuccess || hipEventSynchronize(e1) != hipSuccess) break;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      score[c] = std::min(score[c], ms);
+    }
+  }
+  if (e != hipSuccess) rc = hipFail(e, "autotune launch");
+  Tiling best = cand[0];
+  float bestMs = 1e30f;
+  for (size_t c = 0; c < cand.size(); ++c)
+    if (score[c] < bestMs) {
+      bestMs = score[c];
+      best = cand[c];
+    }
   (void)hipFree(scratch);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
