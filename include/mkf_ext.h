@@ -49,6 +49,22 @@ int mkfFusedBPFD(KernInfo *pK, size_t *pBPFD, BMOrg *pO, const BMFieldInfo *pI, 
 /* Stream-ordered packed binary map into caller memory pW (device). */
 int mkfPBMAsync(BMPackWord *pW, BMOrg *pO, const BMFieldInfo *pI, const BinMapF64 *pM, void *stream);
 
+/* Phase moments fused into the packed-binary-map pass: as mkfPBMAsync (pW
+ * may be NULL to write no map), and ADDS to pMom (device, MKF_MOM_N doubles)
+ * the sum and sum of squares of the field values per phase (map bit 0, 1)
+ * and the phase counts:
+ *   {sum0, sumsq0, sum1, sumsq1, n0, n1}
+ * The first four follow the categorical layout of the reference's
+ * CUDAImgMom::addCat (src/binMapCUDA.cuh:89-121: category c at 2c, its
+ * square at 2c + 1), which the reference declares but never instantiates.
+ * Several fields contribute their sum (the thresholded value). Values are
+ * converted to double and summed in double; per-workgroup partials are added
+ * in a fixed order, so repeated calls give identical results. A NaN voxel
+ * makes its phase's sums NaN. */
+#define MKF_MOM_N 6
+int mkfPBMMomentsAsync(BMPackWord *pW, double *pMom, BMOrg *pO, const BMFieldInfo *pI, const BinMapF64 *pM,
+                       void *stream);
+
 /* Stream-ordered histogram of a packed binary map; adds into pBPFD. */
 int mkfBPFDAsync(size_t *pBPFD, const BMOrg *pO, const BMPackWord *pW, const MkfTiling *pT, void *stream);
 
@@ -195,6 +211,26 @@ double mkfServeClock(const MkfServer *s);
 int mkfServeGetStats(MkfServer *s, MkfServeStats *st);
 /* Finishes every queued request, stops the worker and frees the server. */
 void mkfServeDestroy(MkfServer *s);
+
+/* Crofton-formula estimates of all four Minkowski measures from a BPFD, after
+ * Ohser & Muecklich (2000): m = {Kv, Mv, Sv, Vv}, in the order of MKF_REFML.
+ * No counterpart in the reference: its book measures (mkfRefMeasureBPFD,
+ * src/mkfUtil.c:83-90) need refMeasures.c, which is absent from its tree, and
+ * the per-group S/M weights in src/mkfHacks.c:9-21 are experiments that do not
+ * converge on digitised balls. This function is NOT the reference's Mv/Sv.
+ *  - Kv: 4 pi chi of the 6-adjacency foreground (the textbook Euler weights;
+ *    mkfSelectMeasuresBPFD keeps the reference's table, which differs on two
+ *    pattern groups);
+ *  - Sv: 2 x mean boundary crossings per unit length over the 13 lattice
+ *    directions (Crofton), weighted by spherical Voronoi cells;
+ *  - Mv: 2 pi x mean Euler number per unit area of the sections by the 13
+ *    lattice-plane orientations (integral of mean curvature (k1+k2)/2);
+ *  - Vv: fraction of cells whose lowest corner is set (as volFrac).
+ * s > 0: densities per unit volume of a lattice of spacing s (the volume is
+ * the cell count times s^3). s == 0: totals in lattice units {K, M, S, cells
+ * with the lowest corner set}. Returns 4, or MKF_ERR_ARG for NULL, s < 0, or
+ * an empty histogram with s > 0. Thread-safe. */
+int mkfCroftonMeasuresBPFD(double m[4], const size_t aBPFD[MKF_BINS], double s);
 
 /* Read-only views of the measure tables used by mkfSelectMeasuresBPFD: the
  * symmetry group of each of the 256 patterns and the Euler weight of each of

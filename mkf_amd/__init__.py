@@ -23,6 +23,7 @@ BMC_LT, BMC_EQ, BMC_GT, BMC_NV, BMC_AL = 1, 2, 4, 0, 7
 MKFCU_PROFILE_FAST = 3
 MKFCU_PROFILE_FLEX = 2
 MKF_OK = 0
+MKF_ERR_ARG, MKF_ERR_MEM, MKF_ERR_HIP, MKF_ERR_UNSUPPORTED, MKF_ERR_BUDGET = -1, -2, -3, -4, -5
 
 
 class BinMapF64(C.Structure):
@@ -101,6 +102,7 @@ class MkfServeStats(C.Structure):
 MKF_SERVE_ON_DEVICE = 1
 MKF_SERVE_AFTER_STREAM = 2
 MKF_BATCH_MAX_LEVELS = 16
+MKF_MOM_N = 6
 MKF_BATCH_ZERO_OUTPUT = 1
 
 _P = C.c_void_p
@@ -132,10 +134,12 @@ _SIGNATURES = {
     "mkfRefMeasureBPFD": (None, [C.POINTER(C.c_float), _SZP, C.c_float]),
     "mkfSelectMeasuresBPFD": (C.c_int, [C.POINTER(C.c_float), C.c_char_p, _SZP, C.c_float, C.c_int]),
     "volFrac": (C.c_float, [_SZP]),
+    "mkfCroftonMeasuresBPFD": (C.c_int, [C.POINTER(C.c_double), _SZP, C.c_double]),
     "mkfLastError": (C.c_char_p, []),
     "mkfFusedBPFD": (C.c_int, [C.POINTER(KernInfo), _P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo),
                                C.POINTER(BinMapF64), C.POINTER(MkfTiling), _P]),
     "mkfPBMAsync": (C.c_int, [_P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo), C.POINTER(BinMapF64), _P]),
+    "mkfPBMMomentsAsync": (C.c_int, [_P, _P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo), C.POINTER(BinMapF64), _P]),
     "mkfBPFDAsync": (C.c_int, [_P, C.POINTER(BMOrg), _P, C.POINTER(MkfTiling), _P]),
     "mkfDefaultTiling": (MkfTiling, [C.POINTER(C.c_int), C.c_int]),
     "mkfBatchCreate": (_P, [C.POINTER(MkfBatchField), C.c_int, C.POINTER(MkfBatchShape), C.c_int, _P]),
@@ -245,3 +249,16 @@ def enc_of(dtype_name: str) -> int:
     if dtype_name in ("float64", "torch.float64", "f64"):
         return ENC_F64
     raise ValueError(f"unsupported dtype {dtype_name}")
+
+
+def crofton_measures(hist, s: float = 0.0):
+    """{Kv, Mv, Sv, Vv} (densities for s > 0, lattice-unit totals for s == 0)
+    of a 256-bin histogram through mkfCroftonMeasuresBPFD."""
+    import numpy as np
+
+    h = np.ascontiguousarray(np.asarray(hist, dtype=np.uint64).reshape(256))
+    m = (C.c_double * 4)()
+    rc = load().mkfCroftonMeasuresBPFD(m, h.ctypes.data_as(_SZP), C.c_double(s))
+    if rc != 4:
+        raise RuntimeError(f"mkfCroftonMeasuresBPFD: {rc}")
+    return np.array(list(m))

@@ -16,6 +16,7 @@
  * sum(w * n) * pi/6 = 4 pi chi for objects away from the border.
  */
 #include <math.h>
+#include <pthread.h>
 #include <string.h>
 
 #include "mkf.h"
@@ -94,4 +95,137 @@ int mkfSelectMeasuresBPFD(float m[4], char symCh[], const size_t aBPFD[MKF_BINS]
   }
   memcpy(symCh, "Kv       Vv", 12);
   return 2;
+}
+
+/* ---------------------------------------------------------------------------
+ * Crofton-formula estimates of all four measures (no reference counterpart:
+ * the reference's book measures are absent from its tree). Method after
+ * Ohser & Muecklich, "Statistical Analysis of Microstructures in Materials
+ * Science" (2000), ch. 4-5: intrinsic volumes of a digitised set from its
+ * 2x2x2 configuration histogram, with the 13 lattice directions of the cubic
+ * lattice weighted by the spherical Voronoi cells of the 26 unit directions.
+ *
+ * For each pattern p the library derives, once:
+ *  - K: 4 pi times the pattern's share of the Euler characteristic of the
+ *    6-adjacency foreground (vertex 1/8, axis edge 1/4, face 1/2, cube 1);
+ *  - S: 2 sum_v c_v P_L(v), P_L the boundary crossings per unit length along
+ *    direction v. A crossing is a pair of cell vertices along v that differ;
+ *    a pair along an axis lies in 4 cells, along a face diagonal in 2, along
+ *    a space diagonal in 1, so it counts 1/4, 1/2 or 1 of a crossing here;
+ *  - M: 2 pi sum_v c_v d_v chi(v), chi(v) the Euler number of the sections by
+ *    the lattice planes with normal v (spacing d_v = 1, 1/sqrt2, 1/sqrt3).
+ *    Axis planes cut a cell in two square faces (each shared by two cells),
+ *    face-diagonal planes in one 1 x sqrt2 rectangle, space-diagonal planes in
+ *    two triangles. A square or rectangle adds (Q1 - Q3)/4 (the mean of its
+ *    4- and 8-adjacency Euler numbers, Q1/Q3 = one/three corners set); a
+ *    triangle adds 1/6 for one vertex set and -1/6 for two (6-adjacency).
+ * c_v is the pair's share of the sphere: twice the area fraction of one
+ * direction's spherical Voronoi cell among the 26 directions.
+ * ------------------------------------------------------------------------- */
+#define CROF_AXIS 0.04577789120476086  /* Voronoi area fraction, (1,0,0) */
+#define CROF_FACE 0.03698062787607520  /* (1,1,0) */
+#define CROF_SPACE 0.03519563978231666 /* (1,1,1) */
+
+static double gCrofK[CELL8_PATTERNS], gCrofM[CELL8_PATTERNS], gCrofS[CELL8_PATTERNS];
+static pthread_once_t gCrofOnce = PTHREAD_ONCE_INIT;
+
+static int vbit(int p, int x, int y, int z) { return (p >> (x + 2 * y + 4 * z)) & 1; }
+
+/* Mean of the 4- and 8-adjacency Euler numbers of one 2x2 plane cell. */
+static double quadChi(int n) { return n == 1 ? 0.25 : (n == 3 ? -0.25 : 0.0); }
+static double triChi(int n) { return n == 1 ? 1.0 / 6 : (n == 2 ? -1.0 / 6 : 0.0); }
+
+static void crofInit(void) {
+  const double cA = 2 * CROF_AXIS, cF = 2 * CROF_FACE, cS = 2 * CROF_SPACE;
+  const double r2 = sqrt(2.0), r3 = sqrt(3.0);
+  for (int p = 0; p < CELL8_PATTERNS; p++) {
+    int nv = 0, ne = 0, nf = 0;
+    int tA = 0, tF = 0, tS = 0; /* differing vertex pairs per direction class */
+    for (int i = 0; i < 8; i++) {
+      nv += (p >> i) & 1;
+      for (int j = i + 1; j < 8; j++) {
+        const int d = __builtin_popcount(i ^ j); /* 1 axis, 2 face diagonal, 3 space diagonal */
+        const int diff = ((p >> i) ^ (p >> j)) & 1;
+        if (d == 1) {
+          tA += diff;
+          ne += ((p >> i) & (p >> j)) & 1;
+        } else if (d == 2) {
+          tF += diff;
+        } else {
+          tS += diff;
+        }
+      }
+    }
+    /* faces: fix one axis at 0 or 1 */
+    double mAxis = 0;
+    for (int ax = 0; ax < 3; ax++)
+      for (int c = 0; c < 2; c++) {
+        int n = 0;
+        for (int v = 0; v < 8; v++)
+          if (((v >> ax) & 1) == c) n += (p >> v) & 1;
+        nf += (n == 4);
+        mAxis += quadChi(n) / 2;
+      }
+    /* face-diagonal planes: for axes a < b, a + b == 1 and a == b */
+    double mFace = 0;
+    for (int a = 0; a < 3; a++)
+      for (int b = a + 1; b < 3; b++) {
+        int nSum = 0, nEq = 0;
+        for (int v = 0; v < 8; v++) {
+          const int va = (v >> a) & 1, vb = (v >> b) & 1;
+          if (va + vb == 1) nSum += (p >> v) & 1;
+          if (va == vb) nEq += (p >> v) & 1;
+        }
+        mFace += quadChi(nSum) + quadChi(nEq);
+      }
+    /* space-diagonal planes: normals (1,1,1), (-1,1,1), (1,-1,1), (1,1,-1);
+       the levels holding three vertices are the two triangles */
+    double mSpace = 0;
+    for (int flip = -1; flip < 3; flip++) {
+      int cnt[4] = {0, 0, 0, 0}, n[4] = {0, 0, 0, 0};
+      for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2; y++)
+          for (int z = 0; z < 2; z++) {
+            int u[3] = {x, y, z};
+            if (flip >= 0) u[flip] = 1 - u[flip]; /* reflection maps the normal to (1,1,1) */
+            const int lvl = u[0] + u[1] + u[2];
+            cnt[lvl]++;
+            n[lvl] += vbit(p, x, y, z);
+          }
+      for (int l = 0; l < 4; l++)
+        if (cnt[l] == 3) mSpace += triChi(n[l]);
+    }
+    const int cube = (p == 0xFF);
+    gCrofK[p] = 4 * M_PI * (nv / 8.0 - ne / 4.0 + nf / 2.0 - cube);
+    gCrofS[p] = 2 * (cA * tA / 4.0 + cF * tF / (2.0 * r2) + cS * tS / r3);
+    gCrofM[p] = 2 * M_PI * (cA * mAxis + cF * mFace / r2 + cS * mSpace / r3);
+  }
+}
+
+int mkfCroftonMeasuresBPFD(double m[4], const size_t aBPFD[MKF_BINS], double s) {
+  if (!m || !aBPFD || !(s >= 0)) return MKF_ERR_ARG;
+  pthread_once(&gCrofOnce, crofInit);
+  double k = 0, mm = 0, sa = 0, t = 0, v = 0;
+  for (int p = 0; p < MKF_BINS; p++) {
+    const double h = (double)aBPFD[p];
+    k += h * gCrofK[p];
+    mm += h * gCrofM[p];
+    sa += h * gCrofS[p];
+    t += h;
+    if (p & 1) v += h;
+  }
+  if (s > 0) {
+    if (t == 0) return MKF_ERR_ARG;
+    const double vol = t * s * s * s;
+    m[0] = k / vol;
+    m[1] = mm * s / vol;
+    m[2] = sa * s * s / vol;
+    m[3] = v / t;
+  } else {
+    m[0] = k;
+    m[1] = mm;
+    m[2] = sa;
+    m[3] = v;
+  }
+  return 4;
 }

@@ -192,6 +192,29 @@ int mkfPBMAsync(BMPackWord* pW, BMOrg* pO, const BMFieldInfo* pI, const BinMapF6
   return e == hipSuccess ? MKF_OK : hipFail(e, "PBM launch");
 }
 
+int mkfPBMMomentsAsync(BMPackWord* pW, double* pMom, BMOrg* pO, const BMFieldInfo* pI, const BinMapF64* pM,
+                       void* stream) {
+  if (!pMom || !pM) return fail(MKF_ERR_ARG, "NULL moments or map");
+  FieldView fv;
+  FieldDef def[3];
+  int rc = resolveField(pI, &fv, def);
+  if (rc) return rc;
+  BMOrg org;
+  setBMO(&org, def, 0);
+  if (pO) *pO = org;
+  const hipStream_t s = (hipStream_t)stream;
+  // Workgroup partials come from the stream-ordered pool, so calls on
+  // different streams never share scratch.
+  double* part = nullptr;
+  const size_t bytes = (size_t)pbmMomentParts(fv) * 6 * sizeof(double);
+  hipError_t e = hipMallocAsync((void**)&part, bytes, s);
+  if (e != hipSuccess) return hipFail(e, "moments scratch");
+  e = launchPBMMoments(fv, *pM, pW, org.rowWS, org.planeWS, part, pMom, s);
+  const hipError_t ef = hipFreeAsync(part, s);
+  if (e != hipSuccess) return hipFail(e, "PBM moments launch");
+  return ef == hipSuccess ? MKF_OK : hipFail(ef, "moments scratch free");
+}
+
 int mkfBPFDAsync(size_t* pBPFD, const BMOrg* pO, const BMPackWord* pW, const MkfTiling* pT, void* stream) {
   if (!pBPFD || !pO || !pW) return fail(MKF_ERR_ARG, "NULL argument");
   const int X = (int)pO->rowElem, Y = (int)pO->rowPairs + 1, Z = (int)pO->planePairs + 1;

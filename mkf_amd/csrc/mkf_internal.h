@@ -57,6 +57,14 @@ bool tilingSupported(const Tiling& tl, int elemBytes);
 hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint32_t rowWS, uint32_t planeWS,
                      hipStream_t s);
 
+// As launchPBM (pW may be nullptr: no map), and ADDS the phase moments
+// {sum0, sumsq0, sum1, sumsq1, n0, n1} of the thresholded values to mom
+// (device, 6 doubles). part is device scratch of pbmMomentParts(fv) x 6
+// doubles.
+long long pbmMomentParts(const FieldView& fv);
+hipError_t launchPBMMoments(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint32_t rowWS,
+                            uint32_t planeWS, double* part, double* mom, hipStream_t s);
+
 // Adds the BPFD of fv thresholded by m to bpfd (device, 256 x u64), reading
 // the field once; no PBM is written.
 hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, const Tiling& tl,

@@ -34,11 +34,20 @@ constexpr int kPbmWaves = 8;
 constexpr int kPbmAhead = MKF_PBM_AHEAD;
 constexpr int kPbmVoxels = 16;  // per lane
 
-template <typename T, int M, bool G>
+// Phase moments (MOM): each lane also sums, in double, the field values of
+// its voxels and their squares per phase (map bit 0 or 1) and counts them.
+// Selects, not differences, split the phases, so a NaN stays in its own
+// phase. Each workgroup writes its totals {s0, q0, s1, q1, n0, n1} to
+// part[blockIdx.x] after a fixed-order reduction; kMomentsReduce adds the
+// partials in a fixed order, so the result does not depend on scheduling.
+// With pW == nullptr no map is written.
+constexpr int kMomN = 6;
+
+template <typename T, int M, bool G, bool MOM>
 __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, T thr, int X, int Y, int nSegX,
                                                            long long nRows, long long rowsPerWave,
                                                            uint32_t* __restrict__ pW, uint32_t rowWS,
-                                                           uint32_t planeWS) {
+                                                           uint32_t planeWS, double* __restrict__ part) {
   constexpr int V = kPbmVoxels;
   constexpr int E = 16 / sizeof(T);  // elements per 16-byte load
   constexpr int W = kWave * V;
@@ -47,8 +56,8 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   const long long gw = (long long)blockIdx.x * kPbmWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int seg = (int)(gw % nSegX);
   const long long r0 = (gw / nSegX) * rowsPerWave;
-  if (r0 >= nRows) return;
-  const long long n = min(rowsPerWave, nRows - r0);
+  if (!MOM && r0 >= nRows) return;  // with moments every wave reaches the block reduction
+  const long long n = max(0ll, min(rowsPerWave, nRows - r0));
   int y = (int)(r0 % Y);
   long long z = r0 / Y;
   const int x0 = seg * W;
@@ -56,8 +65,10 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   const bool vec = !G && x0 + W <= X;
   const uint32_t keepBits = (xl >= X) ? 0u : ((X - xl >= V) ? 0xFFFFu : ((1u << (X - xl)) - 1u));
   const uint32_t word = (uint32_t)(x0 >> 5) + (uint32_t)(lane >> 1);
-  const bool writer = (lane & 1) == 0 && word < rowWS;
+  const bool writer = pW != nullptr && (lane & 1) == 0 && word < rowWS;
   const uint32_t sh = (uint32_t)(V * (lane & 1));
+  double s0 = 0, q0 = 0, s1 = 0, q1 = 0;
+  uint32_t n1 = 0, nAll = 0;
   for (long long i = 0; i < n; i += kPbmAhead) {
     T v[kPbmAhead][V];
     size_t wo[kPbmAhead];
@@ -98,8 +109,62 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
         for (int k = V - 2; k >= 0; --k) g = dblAddLaneBit(g, relMask<M>(v[u][k], thr));
         const uint32_t w32 = orQuadXor1((g & keepBits) << sh);
         if (writer) pW[wo[u]] = w32;
+        if constexpr (MOM) {
+          const uint32_t in = g & keepBits;
+#pragma unroll
+          for (int k = 0; k < V; ++k) {
+            const double f = (double)v[u][k];
+            const double fi = ((in >> k) & 1u) ? f : 0.0;
+            const double fo = ((keepBits & ~in) >> k) & 1u ? f : 0.0;
+            s1 += fi;
+            q1 = __builtin_fma(fi, fi, q1);
+            s0 += fo;
+            q0 = __builtin_fma(fo, fo, q0);
+          }
+          n1 += (uint32_t)__builtin_popcount(in);
+          nAll += (uint32_t)__builtin_popcount(keepBits);
+        }
       }
     }
+  }
+  if constexpr (MOM) {
+    __shared__ double red[kPbmWaves][kMomN];
+    double t[kMomN] = {s0, q0, s1, q1, (double)(nAll - n1), (double)n1};
+#pragma unroll
+    for (int j = 0; j < kMomN; ++j) {
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) t[j] += __shfl_xor(t[j], o);
+    }
+    const int w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < kMomN; ++j) red[w][j] = t[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < kMomN) {
+      double a = red[0][threadIdx.x];
+      for (int k = 1; k < kPbmWaves; ++k) a += red[k][threadIdx.x];
+      part[(size_t)blockIdx.x * kMomN + threadIdx.x] = a;
+    }
+  }
+}
+
+// Adds the per-workgroup moment partials to out[kMomN] in a fixed order:
+// thread t sums partials t, t + 256, ... and a pairwise tree sums the threads.
+__global__ __launch_bounds__(256) void kMomentsReduce(const double* __restrict__ part, long long nParts,
+                                                      double* __restrict__ out) {
+  __shared__ double red[256];
+  for (int j = 0; j < kMomN; ++j) {
+    double a = 0;
+    for (long long i = threadIdx.x; i < nParts; i += 256) a += part[i * kMomN + j];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] += red[0];
+    __syncthreads();
   }
 }
 
@@ -383,36 +448,47 @@ Tiling chooseBpfdTiling(int X, int Y, int Z) {
 
 // About 32K waves in all (four times what 256 CUs hold at 32 waves each),
 // each streaming a run of rows of one 1024-voxel segment.
-template <typename T, int M>
+struct PbmGrid {
+  int nSegX;
+  long long nRows, rowsPerWave, nBlocks;
+};
+
+static PbmGrid pbmGrid(const FieldView& fv) {
+  PbmGrid g;
+  g.nSegX = (int)ceilDiv(fv.X, kWave * kPbmVoxels);
+  g.nRows = (long long)fv.Y * fv.Z;
+  const long long wantChunks = std::max<long long>(1, (MKF_PBM_WAVE_TARGET + g.nSegX - 1) / g.nSegX);
+  g.rowsPerWave = ceilDiv(g.nRows, wantChunks);
+  const long long nWaves = g.nSegX * ceilDiv(g.nRows, g.rowsPerWave);
+  g.nBlocks = ceilDiv(nWaves, kPbmWaves);
+  return g;
+}
+
+template <typename T, int M, bool MOM>
 static hipError_t launchPBMT(const FieldView& fv, T thr, uint32_t* pW, uint32_t rowWS, uint32_t planeWS,
-                             hipStream_t s) {
-  const int nSegX = (int)ceilDiv(fv.X, kWave * kPbmVoxels);
-  const long long nRows = (long long)fv.Y * fv.Z;
-  const long long wantChunks = std::max<long long>(1, (MKF_PBM_WAVE_TARGET + nSegX - 1) / nSegX);
-  const long long rowsPerWave = ceilDiv(nRows, wantChunks);
-  const long long nWaves = nSegX * ceilDiv(nRows, rowsPerWave);
-  const long long nBlocks = ceilDiv(nWaves, kPbmWaves);
+                             double* part, hipStream_t s) {
+  const PbmGrid g = pbmGrid(fv);
   if (fv.nF > 1 || fv.sx != 1)
-    hipLaunchKernelGGL((kPBM<T, M, true>), dim3((unsigned)nBlocks), dim3(kPbmWaves * kWave), 0, s, fieldArgs<T>(fv),
-                       thr, fv.X, fv.Y, nSegX, nRows, rowsPerWave, pW, rowWS, planeWS);
+    hipLaunchKernelGGL((kPBM<T, M, true, MOM>), dim3((unsigned)g.nBlocks), dim3(kPbmWaves * kWave), 0, s,
+                       fieldArgs<T>(fv), thr, fv.X, fv.Y, g.nSegX, g.nRows, g.rowsPerWave, pW, rowWS, planeWS, part);
   else
-    hipLaunchKernelGGL((kPBM<T, M, false>), dim3((unsigned)nBlocks), dim3(kPbmWaves * kWave), 0, s,
-                       fieldArgs<T>(fv), thr, fv.X, fv.Y, nSegX, nRows, rowsPerWave, pW, rowWS, planeWS);
+    hipLaunchKernelGGL((kPBM<T, M, false, MOM>), dim3((unsigned)g.nBlocks), dim3(kPbmWaves * kWave), 0, s,
+                       fieldArgs<T>(fv), thr, fv.X, fv.Y, g.nSegX, g.nRows, g.rowsPerWave, pW, rowWS, planeWS, part);
   return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, bool MOM>
 static hipError_t dispatchPBM(const FieldView& fv, T thr, uint32_t m, uint32_t* pW, uint32_t rowWS,
-                              uint32_t planeWS, hipStream_t s) {
+                              uint32_t planeWS, double* part, hipStream_t s) {
   switch (m & 7u) {
-    case 0: return launchPBMT<T, 0>(fv, thr, pW, rowWS, planeWS, s);
-    case 1: return launchPBMT<T, 1>(fv, thr, pW, rowWS, planeWS, s);
-    case 2: return launchPBMT<T, 2>(fv, thr, pW, rowWS, planeWS, s);
-    case 3: return launchPBMT<T, 3>(fv, thr, pW, rowWS, planeWS, s);
-    case 4: return launchPBMT<T, 4>(fv, thr, pW, rowWS, planeWS, s);
-    case 5: return launchPBMT<T, 5>(fv, thr, pW, rowWS, planeWS, s);
-    case 6: return launchPBMT<T, 6>(fv, thr, pW, rowWS, planeWS, s);
-    default: return launchPBMT<T, 7>(fv, thr, pW, rowWS, planeWS, s);
+    case 0: return launchPBMT<T, 0, MOM>(fv, thr, pW, rowWS, planeWS, part, s);
+    case 1: return launchPBMT<T, 1, MOM>(fv, thr, pW, rowWS, planeWS, part, s);
+    case 2: return launchPBMT<T, 2, MOM>(fv, thr, pW, rowWS, planeWS, part, s);
+    case 3: return launchPBMT<T, 3, MOM>(fv, thr, pW, rowWS, planeWS, part, s);
+    case 4: return launchPBMT<T, 4, MOM>(fv, thr, pW, rowWS, planeWS, part, s);
+    case 5: return launchPBMT<T, 5, MOM>(fv, thr, pW, rowWS, planeWS, part, s);
+    case 6: return launchPBMT<T, 6, MOM>(fv, thr, pW, rowWS, planeWS, part, s);
+    default: return launchPBMT<T, 7, MOM>(fv, thr, pW, rowWS, planeWS, part, s);
   }
 }
 
@@ -421,9 +497,28 @@ hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint
   if (fv.X < 1 || fv.Y < 1 || fv.Z < 1) return hipErrorInvalidValue;
   // f32 fields compare against the threshold rounded to float
   // (src/binMapCUDA.cuh:138-139).
-  if (fv.enc == ENC_F32) return dispatchPBM<float>(fv, (float)m.t[0], m.m, pW, rowWS, planeWS, s);
-  if (fv.enc == ENC_F64) return dispatchPBM<double>(fv, m.t[0], m.m, pW, rowWS, planeWS, s);
+  if (fv.enc == ENC_F32) return dispatchPBM<float, false>(fv, (float)m.t[0], m.m, pW, rowWS, planeWS, nullptr, s);
+  if (fv.enc == ENC_F64) return dispatchPBM<double, false>(fv, m.t[0], m.m, pW, rowWS, planeWS, nullptr, s);
   return hipErrorInvalidValue;
+}
+
+long long pbmMomentParts(const FieldView& fv) { return pbmGrid(fv).nBlocks; }
+
+hipError_t launchPBMMoments(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint32_t rowWS,
+                            uint32_t planeWS, double* part, double* mom, hipStream_t s) {
+  if (fv.X < 1 || fv.Y < 1 || fv.Z < 1) return hipErrorInvalidValue;
+  // Per-lane counts are 32-bit: at most 16 voxels per row.
+  if (pbmGrid(fv).rowsPerWave > (1ll << 27)) return hipErrorInvalidValue;
+  hipError_t e;
+  if (fv.enc == ENC_F32)
+    e = dispatchPBM<float, true>(fv, (float)m.t[0], m.m, pW, rowWS, planeWS, part, s);
+  else if (fv.enc == ENC_F64)
+    e = dispatchPBM<double, true>(fv, m.t[0], m.m, pW, rowWS, planeWS, part, s);
+  else
+    return hipErrorInvalidValue;
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kMomentsReduce, dim3(1), dim3(256), 0, s, part, pbmGrid(fv).nBlocks, mom);
+  return hipGetLastError();
 }
 
 template <typename T, int M, int V, int TY>

@@ -34,7 +34,12 @@ its un-vendored ../Common/src/util.h is absent):
     its table (src/mkfUtil.c:33-38) pins. Vv: set lower corners over cells,
     checked by direct counting.
   * Parity unpinned: the reference's book measures Mv and Sv (refMeasures.c
-    is absent from its tree; as it compiles both are -1).
+    is absent from its tree; as it compiles both are -1). The library's
+    non-reference Crofton estimates are checked by oracle/crofton.py.
+  * phase moments: the categorical sums of src/binMapCUDA.cuh:89-121
+    (declared, never instantiated in the reference) restated with exactly
+    rounded sums (math.fsum), pinned by closed forms on integer ramps and
+    constant fields and by brute-force loops on tiny volumes.
 """
 
 import ctypes as C
@@ -131,6 +136,25 @@ def unpack_bits(words, X):
 
 def pbm_numpy(field, t, m):
     return pack_bits(threshold_bits(field, t, m))
+
+
+def phase_moments(field, t, m):
+    """{sum0, sumsq0, sum1, sumsq1, n0, n1} of a (Z, Y, X) field split by its
+    threshold bits: per phase the exactly rounded sum of the values (as
+    double) and of their squares (each square rounded to double), and the
+    voxel count. Layout of CUDAImgMom::addCat (src/binMapCUDA.cuh:117-121:
+    category c at 2c, its square at 2c + 1), with the counts appended."""
+    import math
+
+    f = np.asarray(field)
+    b = threshold_bits(f, t, m).ravel()
+    v = f.astype(np.float64).ravel()
+    out = []
+    for ph in (False, True):
+        sel = v[b == ph]
+        out += [math.fsum(sel), math.fsum(sel * sel)]
+    n1 = int(np.count_nonzero(b))
+    return np.array(out + [float(b.size - n1), float(n1)])
 
 
 # ---------------------------------------------------------------------------

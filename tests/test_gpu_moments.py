@@ -1,0 +1,152 @@
+"""GPU: mkfPBMMomentsAsync, phase moments fused into the PBM pass, against the
+oracle's exactly rounded sums (the categorical sums the reference declares in
+CUDAImgMom, src/binMapCUDA.cuh:89-121, and never instantiates).
+
+Counts are exact. The sums are floating point: the kernel adds values in
+double per lane, then per wave, workgroup and grid in fixed orders. Each sum
+must be within 1e-12 x (sum of the magnitudes of its terms) of the exactly
+rounded sum; the bound of naive summation for the longest chain here (a few
+hundred terms per lane plus ~25 tree levels) is below 1e-13 of that.
+The map written alongside equals mkfPBMAsync's bit for bit."""
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import mkf_amd as M
+from mkf_amd import synth
+from tests.gpu_util import enc_of, host_np
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _moments(lib, fi, X, Y, Z, relop, thr, cuda, with_map=True, pre=None):
+    bm = M.bin_map(relop, thr)
+    org = M.BMOrg()
+    n = lib.setBMO(C.byref(org), M.dims((X, Y, Z)), b"\x00")
+    words = torch.full((n,), -1, dtype=torch.int32, device=cuda) if with_map else None
+    mom = torch.zeros(M.MKF_MOM_N, dtype=torch.float64, device=cuda)
+    if pre is not None:
+        mom += torch.tensor(pre, dtype=torch.float64, device=cuda)
+    rc = lib.mkfPBMMomentsAsync(words.data_ptr() if with_map else None, mom.data_ptr(), C.byref(org), fi.ref,
+                                C.byref(bm), None)
+    M.check(rc, "mkfPBMMomentsAsync")
+    torch.cuda.synchronize()
+    w = words.cpu().numpy().view(np.uint32).reshape(Z, Y, org.rowWS) if with_map else None
+    return mom.cpu().numpy(), w
+
+
+def _pbm(lib, fi, X, Y, Z, relop, thr, cuda):
+    bm = M.bin_map(relop, thr)
+    org = M.BMOrg()
+    n = lib.setBMO(C.byref(org), M.dims((X, Y, Z)), b"\x00")
+    words = torch.full((n,), -1, dtype=torch.int32, device=cuda)
+    M.check(lib.mkfPBMAsync(words.data_ptr(), C.byref(org), fi.ref, C.byref(bm), None), "mkfPBMAsync")
+    torch.cuda.synchronize()
+    return words.cpu().numpy().view(np.uint32).reshape(Z, Y, org.rowWS)
+
+
+def _check(got, f_np, relop, thr):
+    bm = M.bin_map(relop, thr)
+    want = M_oracle().phase_moments(f_np, bm.t[0], bm.m)
+    assert got[4] == want[4] and got[5] == want[5]
+    b = M_oracle().threshold_bits(f_np, bm.t[0], bm.m).ravel()
+    v = f_np.astype(np.float64).ravel()
+    for ph in (0, 1):
+        sel = v[b == bool(ph)]
+        for j, mag in ((2 * ph, np.abs(sel).sum()), (2 * ph + 1, (sel * sel).sum())):
+            if not math.isfinite(want[j]):
+                assert (math.isnan(got[j]) and math.isnan(want[j])) or got[j] == want[j], (j, got[j], want[j])
+            else:
+                assert abs(got[j] - want[j]) <= TOL * mag, (j, got[j], want[j])
+
+
+def M_oracle():
+    from oracle import oracle as O
+
+    return O
+
+
+SHAPES = [(2, 2, 2), (3, 5, 31), (4, 4, 1025), (9, 17, 200), (40, 50, 333), (3, 2, 2049)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_moments_match_oracle(lib, cuda, shape, dtype):
+    print(f"moments {shape} {dtype}", flush=True)
+    Z, Y, X = shape
+    f = synth.grf_field(shape, sigma=1.5, seed=sum(shape), device=cuda, dtype=dtype) * 3 + 0.5
+    fi = M.FieldInfo(f.data_ptr(), (X, Y, Z), enc_of(f))
+    f_np = host_np(f)
+    for relop, thr in ((">=", 0.25), ("<", -1.0), ("!=", 0.0), ("", 0.0)):
+        got, w = _moments(lib, fi, X, Y, Z, relop, thr, cuda)
+        _check(got, f_np, relop, thr)
+        np.testing.assert_array_equal(w, _pbm(lib, fi, X, Y, Z, relop, thr, cuda))
+
+
+def test_moments_accumulate_are_deterministic_and_map_optional(lib, cuda):
+    Z, Y, X = 64, 96, 700
+    f = synth.grf_field((Z, Y, X), sigma=2.0, seed=5, device=cuda, dtype=torch.float32)
+    fi = M.FieldInfo(f.data_ptr(), (X, Y, Z), M.ENC_F32)
+    a, _ = _moments(lib, fi, X, Y, Z, ">", 0.1, cuda)
+    b, _ = _moments(lib, fi, X, Y, Z, ">", 0.1, cuda, with_map=False)
+    assert a.tobytes() == b.tobytes()  # same partials, same fixed order
+    pre = [1.0, 2.0, 3.0, 4.0, 5.0, 6.0]
+    c, _ = _moments(lib, fi, X, Y, Z, ">", 0.1, cuda, pre=pre)
+    np.testing.assert_allclose(c, a + np.array(pre), rtol=1e-15)
+    _check(a, host_np(f), ">", 0.1)
+    assert a[4] + a[5] == X * Y * Z
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_moments_special_values(lib, cuda, dtype):
+    rng = np.random.default_rng(9)
+    choices = np.array([0.0, -0.0, 1e-40, -1e-40, 0.25, -0.25, 1.0, 3.0, np.inf])
+    vals = rng.choice(choices, size=(5, 7, 97))
+    f = torch.from_numpy(vals).to(dtype).to(cuda)
+    Z, Y, X = f.shape
+    fi = M.FieldInfo(f.data_ptr(), (X, Y, Z), enc_of(f))
+    # inf lands in phase 1 for '>=' and in phase 0 for '<': its phase's sums are inf
+    for relop in (">=", "<"):
+        got, _ = _moments(lib, fi, X, Y, Z, relop, 0.5, cuda)
+        _check(got, host_np(f), relop, 0.5)
+        inf_ph = 1 if relop == ">=" else 0
+        assert math.isinf(got[2 * inf_ph]) and math.isfinite(got[2 * (1 - inf_ph)])
+    # NaN compares as EQ: with '<' it joins phase 0 (with the inf), whose sums become NaN
+    vals[2, 3, 50] = np.nan
+    f = torch.from_numpy(vals).to(dtype).to(cuda)
+    fi = M.FieldInfo(f.data_ptr(), (X, Y, Z), enc_of(f))
+    got, _ = _moments(lib, fi, X, Y, Z, "<", 0.5, cuda)
+    _check(got, host_np(f), "<", 0.5)
+    assert math.isnan(got[0]) and math.isnan(got[1]) and math.isfinite(got[2]) and math.isfinite(got[3])
+    assert got[4] + got[5] == X * Y * Z
+
+
+def test_moments_of_summed_and_strided_fields(lib, cuda):
+    Z, Y, X = 13, 21, 150
+    fs = [synth.grf_field((Z, Y, X), sigma=1.5, seed=40 + i, device=cuda, dtype=torch.float64) for i in range(3)]
+    fi = M.FieldInfo([g.data_ptr() for g in fs], (X, Y, Z), M.ENC_F64)
+    s = host_np(fs[0]) + host_np(fs[1]) + host_np(fs[2])
+    got, _ = _moments(lib, fi, X, Y, Z, ">=", 0.0, cuda)
+    _check(got, s, ">=", 0.0)
+    sx = 3
+    base = synth.grf_field((Z, Y, X * sx), sigma=2.0, seed=44, device=cuda, dtype=torch.float32)
+    fi = M.FieldInfo(base.data_ptr(), (X, Y, Z), M.ENC_F32, strides=(sx, X * sx, X * sx * Y))
+    got, _ = _moments(lib, fi, X, Y, Z, "<=", -0.3, cuda)
+    _check(got, np.ascontiguousarray(host_np(base)[:, :, ::sx]), "<=", -0.3)
+
+
+def test_moments_uniform_256(lib, cuda):
+    """256^3 uniform [0,1) f32 split at 0.5: phase means near 0.25 / 0.75,
+    and the oracle's sums to the stated tolerance."""
+    f = torch.rand((256, 256, 256), generator=torch.Generator(device=cuda).manual_seed(11), device=cuda)
+    fi = M.FieldInfo(f.data_ptr(), (256, 256, 256), M.ENC_F32)
+    got, _ = _moments(lib, fi, 256, 256, 256, ">=", 0.5, cuda)
+    _check(got, host_np(f), ">=", 0.5)
+    assert got[0] / got[4] == pytest.approx(0.25, abs=1e-3)
+    assert got[2] / got[5] == pytest.approx(0.75, abs=1e-3)
