@@ -267,8 +267,8 @@ int mkfFusedAutotune(const BMFieldInfo* pI, const BinMapF64* pM, MkfTiling* pCho
     cand.push_back(Tiling{256, 16, 4, 4});
     cand.push_back(Tiling{64, 16, 4, 1});
   } else {
-    cand.push_back(Tiling{128, 16, 8, 1});
-    cand.push_back(Tiling{512, 16, 4, 2});
+    cand.push_back(Tiling{512, 16, 8, 1});
+    cand.push_back(Tiling{256, 16, 4, 2});
   }
   const hipStream_t s = (hipStream_t)stream;
   unsigned long long* scratch = nullptr;

@@ -17,7 +17,9 @@ namespace mkf {
 // of floats, eight of doubles), so a wave covers 1024 voxels of a row and
 // two lanes make one PBM word: each wave stores 32 whole words (a 128-byte
 // line) per row. (With 256 voxels per wave, i.e. 32-byte stores, the same
-// kernel was 6-11 % slower at 1024^3.) A wave owns one 1024-voxel segment
+// kernel was 6-11 % slower at 1024^3. Non-temporal loads made it 1.4-2x
+// slower: each 16-byte load of a wave covers a quarter of every line it
+// touches, and the other quarters come from the cache only if it keeps them.) A wave owns one 1024-voxel segment
 // and a run of consecutive rows (y fastest, then z), kPbmAhead rows at a
 // time. Per row each lane turns its 16 compare ballots into a 16-bit group
 // (one v_cndmask, then v_addc doubling steps) and one DPP step ORs the two
@@ -397,21 +399,24 @@ bool tilingSupported(const Tiling& tl, int elemBytes) {
 
 // Fused kernel: workgroups as wide as the row (up to 16 waves) and the
 // longest slab that still gives the grid a dtype's target number of waves.
-// Measured on MI355X at 1024^3 (HIP events, warm clocks, 5 interleaved runs
-// of 30 launches on one machine):
-//   f32: 1 voxel/lane, 4 row pairs, 128 plane pairs   0.714 ms (6.02 TB/s)
-//        4 voxels/lane, 4 row pairs, 256 plane pairs  0.729 ms
-//        1 voxel/lane, 4 row pairs, 64 plane pairs    0.724 ms
-//   f64: 1 voxel/lane, 8 row pairs, 512 plane pairs   1.306 ms (6.58 TB/s)
-//        1 voxel/lane, 8 row pairs, 128 plane pairs   1.343 ms
-//        2 voxels/lane, 4 row pairs, 512 plane pairs  1.323 ms
-// (another machine had the f32 order of the first two reversed by 1 %).
+// Measured on MI355X at 1024^3 with the interior rows of each stack read by
+// non-temporal loads (HIP events, warm clocks, best of 3 interleaved rounds
+// of 20 launches):
+//   f32: 1 voxel/lane, 4 row pairs, 128 plane pairs   0.700 ms (6.13 TB/s)
+//        4 voxels/lane, 4 row pairs, 256 plane pairs  0.696 ms
+//        1 voxel/lane, 4 row pairs, 64 plane pairs    0.712 ms
+//        1 voxel/lane, 8 row pairs, 128 plane pairs   0.816 ms
+//   f64: 2 voxels/lane, 4 row pairs, 512 plane pairs  1.181 ms (7.27 TB/s)
+//        1 voxel/lane, 8 row pairs, 512 plane pairs   1.280 ms
+//        1 voxel/lane, 8 row pairs, 256 plane pairs   1.293 ms
+// The f32 leaders are within 1 % and swap between machines; the library's
+// autotuner times them on the caller's device.
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   Tiling tl;
   long long targetWaves;
   if (elemBytes >= 8) {
-    tl.V = 1;
-    tl.Ty = 8;
+    tl.V = 2;
+    tl.Ty = 4;
     tl.TP = 512;
     targetWaves = 4096;
   } else {
