@@ -104,6 +104,17 @@ __device__ __forceinline__ uint32_t orRowShl4(uint32_t v) {
 __device__ __forceinline__ uint32_t orRowShl8(uint32_t v) {
   return v | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x108, 0xF, 0xF, true);
 }
+// v | (the value of v in lane 7 - i of the same 8-lane half-row), and in
+// lane 15 - i of the same 16-lane row: DPP row_half_mirror, row_mirror. After
+// orQuadXor1 and orQuadXor2 every lane of a quad holds the quad's OR; one
+// half-mirror step then gives every lane its 8-lane group's OR, and one more
+// mirror step its 16-lane row's OR.
+__device__ __forceinline__ uint32_t orHalfMirror(uint32_t v) {
+  return v | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t orRowMirror(uint32_t v) {
+  return v | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, true);
+}
 
 // Lane l receives lane l+1's value through DPP (wave_shl:1, one v_mov per
 // dword); lane 63, which has no source, keeps its own value.

@@ -13,24 +13,27 @@
 namespace mkf {
 
 // ---------------------------------------------------------------------------
-// Packed binary map. A lane holds 16 consecutive voxels (four 16-byte loads
-// of floats, eight of doubles), so a wave covers 1024 voxels of a row and
-// two lanes make one PBM word: each wave stores 32 whole words (a 128-byte
-// line) per row. (With 256 voxels per wave, i.e. 32-byte stores, the same
-// kernel was 6-11 % slower at 1024^3. Non-temporal loads made it 1.4-2x
-// slower: each 16-byte load of a wave covers a quarter of every line it
-// touches, and the other quarters come from the cache only if it keeps them.) A wave owns one 1024-voxel segment
-// and a run of consecutive rows (y fastest, then z), kPbmAhead rows at a
-// time. Per row each lane turns its 16 compare ballots into a 16-bit group
-// (one v_cndmask, then v_addc doubling steps) and one DPP step ORs the two
-// halves of each word. The waves of a workgroup take consecutive segments of
-// the same rows; rows advance by incrementing (y, z), with no division.
-// ---------------------------------------------------------------------------
+// Packed binary map. A lane holds 16 voxels of a row as E-voxel pieces
+// (E = 4 floats or 2 doubles, one 16-byte load each) spaced 64 E voxels apart,
+// so each load of the wave reads 1 KiB of consecutive bytes -- whole lines,
+// read with non-temporal loads since every voxel is read once -- and a wave
+// covers 1024 voxels of a row. PBM word 2 E q + g of the wave's segment comes
+// from load q of the 32 / E lanes of group g: each lane packs its E compare
+// bits (v_addc doubling steps with the ballot as carry-in), shifts them to
+// its place, and three or four DPP OR steps (quad xor 1, quad xor 2,
+// half-row mirror, row mirror) give the whole word to every lane of the
+// group. 32 writer lanes then store the row's 32 words as one 128-byte line.
+// (Each lane holding 16 consecutive voxels, i.e. four lane-strided 16-byte
+// loads, was 7-10 % slower at 1024^3 and could not use non-temporal loads;
+// 256 voxels per wave with 32-byte stores was slower still.) A wave owns one
+// 1024-voxel segment and a run of consecutive rows (y fastest, then z),
+// kPbmAhead rows at a time; rows advance by incrementing (y, z), with no
+// division.
 #ifndef MKF_PBM_AHEAD
 #define MKF_PBM_AHEAD 2
 #endif
 #ifndef MKF_PBM_WAVE_TARGET
-#define MKF_PBM_WAVE_TARGET 32768
+#define MKF_PBM_WAVE_TARGET 16384
 #endif
 constexpr int kPbmWaves = 8;
 constexpr int kPbmAhead = MKF_PBM_AHEAD;
@@ -63,12 +66,25 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   int y = (int)(r0 % Y);
   long long z = r0 / Y;
   const int x0 = seg * W;
-  const int xl = x0 + V * lane;
+  // Lane-contiguous layout: load q of a row gives lane l the E voxels at
+  // x0 + 64 E q + E l, so each 16-byte load of the wave covers 1 KiB of
+  // consecutive bytes (whole lines) and can bypass the caches. PBM word
+  // 2 E q + g of the segment is built from the LPW = 32 / E lanes of group g,
+  // each contributing E bits.
+  constexpr int NQ = V / E;    // 16-byte loads per lane and row
+  constexpr int LPW = 32 / E;  // lanes per PBM word
+  typedef const __attribute__((address_space(1))) Vec* GVec;
   const bool vec = !G && x0 + W <= X;
-  const uint32_t keepBits = (xl >= X) ? 0u : ((X - xl >= V) ? 0xFFFFu : ((1u << (X - xl)) - 1u));
-  const uint32_t word = (uint32_t)(x0 >> 5) + (uint32_t)(lane >> 1);
-  const bool writer = pW != nullptr && (lane & 1) == 0 && word < rowWS;
-  const uint32_t sh = (uint32_t)(V * (lane & 1));
+  uint32_t kq[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int xs = x0 + kWave * E * q + E * lane;
+    kq[q] = (xs >= X) ? 0u : ((X - xs >= E) ? ((1u << E) - 1u) : ((1u << (X - xs)) - 1u));
+  }
+  const int jl = lane % LPW;  // lane within its word group; writers take word q = jl
+  const uint32_t word = (uint32_t)(x0 >> 5) + (uint32_t)(2 * E * jl + lane / LPW);
+  const bool writer = pW != nullptr && jl < NQ && word < rowWS;
+  const uint32_t sh = (uint32_t)(E * jl);
   double s0 = 0, q0 = 0, s1 = 0, q1 = 0;
   uint32_t n1 = 0, nAll = 0;
   for (long long i = 0; i < n; i += kPbmAhead) {
@@ -84,14 +100,20 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
           T t[V];
           if (vec) {
 #pragma unroll
-            for (int q = 0; q < V / E; ++q) {
-              const Vec w4 = *(const Vec*)(row + xl + E * q);
+            for (int q = 0; q < NQ; ++q) {
+              const Vec w4 = __builtin_nontemporal_load((GVec)(row + x0 + kWave * E * q + E * lane));
 #pragma unroll
               for (int e = 0; e < E; ++e) t[E * q + e] = w4[e];
             }
           } else {
 #pragma unroll
-            for (int k = 0; k < V; ++k) t[k] = row[G ? (long long)min(xl + k, X - 1) * fa.sx : min(xl + k, X - 1)];
+            for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+              for (int e = 0; e < E; ++e) {
+                const int x = min(x0 + kWave * E * q + E * lane + e, X - 1);
+                t[E * q + e] = row[G ? (long long)x * fa.sx : x];
+              }
+            }
           }
 #pragma unroll
           for (int k = 0; k < V; ++k) v[u][k] = fi ? v[u][k] + t[k] : t[k];
@@ -106,26 +128,35 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
 #pragma unroll
     for (int u = 0; u < kPbmAhead; ++u) {
       if (i + u < n) {
-        uint32_t g = laneBit(relMask<M>(v[u][V - 1], thr)) ? 1u : 0u;
+        uint32_t wq[NQ];
 #pragma unroll
-        for (int k = V - 2; k >= 0; --k) g = dblAddLaneBit(g, relMask<M>(v[u][k], thr));
-        const uint32_t w32 = orQuadXor1((g & keepBits) << sh);
-        if (writer) pW[wo[u]] = w32;
-        if constexpr (MOM) {
-          const uint32_t in = g & keepBits;
+        for (int q = 0; q < NQ; ++q) {
+          uint32_t nib = laneBit(relMask<M>(v[u][E * q + E - 1], thr)) ? 1u : 0u;
 #pragma unroll
-          for (int k = 0; k < V; ++k) {
-            const double f = (double)v[u][k];
-            const double fi = ((in >> k) & 1u) ? f : 0.0;
-            const double fo = ((keepBits & ~in) >> k) & 1u ? f : 0.0;
-            s1 += fi;
-            q1 = __builtin_fma(fi, fi, q1);
-            s0 += fo;
-            q0 = __builtin_fma(fo, fo, q0);
+          for (int e = E - 2; e >= 0; --e) nib = dblAddLaneBit(nib, relMask<M>(v[u][E * q + e], thr));
+          nib &= kq[q];
+          uint32_t pw = orHalfMirror(orQuadXor2(orQuadXor1(nib << sh)));
+          if constexpr (LPW == 16) pw = orRowMirror(pw);
+          wq[q] = pw;
+          if constexpr (MOM) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+              const double f = (double)v[u][E * q + e];
+              const double fi = ((nib >> e) & 1u) ? f : 0.0;
+              const double fo = ((kq[q] & ~nib) >> e) & 1u ? f : 0.0;
+              s1 += fi;
+              q1 = __builtin_fma(fi, fi, q1);
+              s0 += fo;
+              q0 = __builtin_fma(fo, fo, q0);
+            }
+            n1 += (uint32_t)__builtin_popcount(nib);
+            nAll += (uint32_t)__builtin_popcount(kq[q]);
           }
-          n1 += (uint32_t)__builtin_popcount(in);
-          nAll += (uint32_t)__builtin_popcount(keepBits);
         }
+        uint32_t w32 = wq[0];
+#pragma unroll
+        for (int q = 1; q < NQ; ++q) w32 = (jl == q) ? wq[q] : w32;
+        if (writer) pW[wo[u]] = w32;
       }
     }
   }
@@ -451,8 +482,10 @@ Tiling chooseBpfdTiling(int X, int Y, int Z) {
   return tl;
 }
 
-// About 32K waves in all (four times what 256 CUs hold at 32 waves each),
-// each streaming a run of rows of one 1024-voxel segment.
+// About 16K waves in all (twice what 256 CUs hold at 32 waves each), each
+// streaming a run of rows of one 1024-voxel segment. (At 1024^3, 32K and 64K
+// waves, and 3 or 4 rows in flight instead of 2, were within 1 % for the map
+// and 2-6 % slower with moments.)
 struct PbmGrid {
   int nSegX;
   long long nRows, rowsPerWave, nBlocks;
