@@ -538,6 +538,41 @@ def main():
         f"BPFD {result['dropin']['bpfd_ms']:.3f} ms, same histogram: {dropin_ok}")
     del pbm
 
+    # Phase moments of the same field: one fused pass (histogram + moments)
+    # against the histogram-only pass above plus a moments-only map pass.
+    hm = torch.zeros(256, dtype=torch.int64, device=dev)
+    mom_f = torch.zeros(M.MKF_MOM_N, dtype=torch.float64, device=dev)
+    mom_m = torch.zeros(M.MKF_MOM_N, dtype=torch.float64, device=dev)
+    fm_ms, mo_ms = [], []
+    for i in range(max(3, K // 4) + 2):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        hm.zero_()
+        mom_f.zero_()
+        mom_m.zero_()
+        e[0].record(stream)
+        M.check(lib.mkfFusedBPFDMoments(hm.data_ptr(), mom_f.data_ptr(), None, fi.ref, C.byref(bm), None, sptr),
+                "mkfFusedBPFDMoments")
+        e[1].record(stream)
+        M.check(lib.mkfPBMMomentsAsync(None, mom_m.data_ptr(), C.byref(org), fi.ref, C.byref(bm), sptr),
+                "mkfPBMMomentsAsync")
+        e[2].record(stream)
+        torch.cuda.synchronize()
+        if i >= 2:
+            fm_ms.append(e[0].elapsed_time(e[1]))
+            mo_ms.append(e[1].elapsed_time(e[2]))
+    mf, mm = mom_f.cpu().numpy(), mom_m.cpu().numpy()
+    fm, mo = sum(fm_ms) / len(fm_ms), sum(mo_ms) / len(mo_ms)
+    result["phase_moments"] = {
+        "fused_ms": round(fm, 4), "moments_only_ms": round(mo, 4), "two_pass_ms": round(kern_avg + mo, 4),
+        "same_histogram_as_fused": bool(np.array_equal(hm.cpu().numpy().view(np.uint64), gpu_hist)),
+        "moments_agree": bool(mf[4] == mm[4] and mf[5] == mm[5] and np.allclose(mf[:4], mm[:4], rtol=1e-12)),
+        "phase_means": [round(float(mf[0] / max(mf[4], 1)), 6), round(float(mf[2] / max(mf[5], 1)), 6)],
+    }
+    log(f"[bench] phase moments: fused histogram+moments {fm:.3f} ms vs two passes {kern_avg + mo:.3f} ms "
+        f"(moments-only map pass {mo:.3f} ms); same histogram {result['phase_moments']['same_histogram_as_fused']}, "
+        f"moments agree {result['phase_moments']['moments_agree']}")
+    del hm, mom_f, mom_m
+
     # The host-field path (PCIe-inclusive): the same field from pinned host
     # memory, streamed through the fused kernel in z-slabs.
     hostf = torch.empty(field.shape, dtype=field.dtype, pin_memory=True)

@@ -140,6 +140,8 @@ _SIGNATURES = {
                                C.POINTER(BinMapF64), C.POINTER(MkfTiling), _P]),
     "mkfPBMAsync": (C.c_int, [_P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo), C.POINTER(BinMapF64), _P]),
     "mkfPBMMomentsAsync": (C.c_int, [_P, _P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo), C.POINTER(BinMapF64), _P]),
+    "mkfFusedBPFDMoments": (C.c_int, [_P, _P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo), C.POINTER(BinMapF64),
+                                      C.POINTER(MkfTiling), _P]),
     "mkfBPFDAsync": (C.c_int, [_P, C.POINTER(BMOrg), _P, C.POINTER(MkfTiling), _P]),
     "mkfDefaultTiling": (MkfTiling, [C.POINTER(C.c_int), C.c_int]),
     "mkfBatchCreate": (_P, [C.POINTER(MkfBatchField), C.c_int, C.POINTER(MkfBatchShape), C.c_int, _P]),

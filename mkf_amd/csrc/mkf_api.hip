@@ -253,6 +253,39 @@ int mkfFusedBPFD(KernInfo* pK, size_t* pBPFD, BMOrg* pO, const BMFieldInfo* pI, 
   return e == hipSuccess ? MKF_OK : hipFail(e, "fused launch");
 }
 
+int mkfFusedBPFDMoments(size_t* pBPFD, double* pMom, BMOrg* pO, const BMFieldInfo* pI, const BinMapF64* pM,
+                        const MkfTiling* pT, void* stream) {
+  if (!pBPFD || !pMom || !pM) return fail(MKF_ERR_ARG, "NULL histogram, moments or map");
+  FieldView fv;
+  FieldDef def[3];
+  int rc = resolveField(pI, &fv, def);
+  if (rc) return rc;
+  BMOrg org;
+  setBMO(&org, def, 0);
+  if (pO) *pO = org;
+  // The plain kernel's autotuned tiling does not carry over: with moments
+  // the kernel wants more voxels per lane and more tiles.
+  const Tiling tl = pT ? Tiling{pT->planePairs, pT->waves, pT->rowPairs, pT->voxelsPerLane}
+                       : chooseMomTiling(fv.X, fv.Y, fv.Z, fv.enc & ENC_MASK_NUM);
+  rc = checkTiling(tl, fv.enc & ENC_MASK_NUM);
+  if (rc) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  // A field without cells adds nothing to the histogram; its moments come
+  // from the map kernel (no map written).
+  const long long nParts = fusedMomentParts(fv, tl);
+  const size_t bytes = (size_t)(nParts ? nParts : pbmMomentParts(fv)) * 6 * sizeof(double);
+  double* part = nullptr;
+  hipError_t e = hipMallocAsync((void**)&part, bytes, s);
+  if (e != hipSuccess) return hipFail(e, "moments scratch");
+  if (nParts)
+    e = launchFusedMoments(fv, *pM, (unsigned long long*)pBPFD, part, pMom, tl, s);
+  else
+    e = launchPBMMoments(fv, *pM, nullptr, org.rowWS, org.planeWS, part, pMom, s);
+  const hipError_t ef = hipFreeAsync(part, s);
+  if (e != hipSuccess) return hipFail(e, "fused moments launch");
+  return ef == hipSuccess ? MKF_OK : hipFail(ef, "moments scratch free");
+}
+
 int mkfFusedAutotune(const BMFieldInfo* pI, const BinMapF64* pM, MkfTiling* pChosen, void* stream) {
   if (!pM) return fail(MKF_ERR_ARG, "NULL map");
   FieldView fv;

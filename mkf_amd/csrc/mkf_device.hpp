@@ -206,6 +206,8 @@ struct VecLoad<double, 2> {
 
 // Kernel argument: up to 4 fields of identical dims and element strides,
 // thresholded as their sum in table order.
+constexpr int kMaxFieldsDev = 4;  // fields summed at most (BMCU_FIELD_MAX)
+
 template <typename T>
 struct FieldArgs {
   const T* p[4];
@@ -225,6 +227,7 @@ struct FieldArgs {
 template <typename T, int M, int V, bool G = false>
 struct FieldRows {
   static constexpr bool kSelfHalo = false;
+  static constexpr int kM = M;
   const T* fw[4];    // fields + xb * sx, xb = min(x0, X - 1) (uniform)
   int nF;            // fields summed (1..4)
   long long sx;      // element stride of x
@@ -267,10 +270,14 @@ struct FieldRows {
   __device__ __forceinline__ Raw load(long long off) const {
     Raw r = loadOne<NT>(fw[0] + off);
     if constexpr (G) {
-      for (int f = 1; f < nF; ++f) {
-        const Raw q = loadOne(fw[f] + off);
+      // unrolled with guards so the pointers stay in registers
 #pragma unroll
-        for (int k = 0; k < V; ++k) r.v[k] += q.v[k];
+      for (int f = 1; f < kMaxFieldsDev; ++f) {
+        if (f < nF) {
+          const Raw q = loadOne(fw[f] + off);
+#pragma unroll
+          for (int k = 0; k < V; ++k) r.v[k] += q.v[k];
+        }
       }
     }
     return r;
@@ -290,14 +297,26 @@ struct FieldRows {
     const uint32_t right = laneBit(R0 >> 1) ? 1u : 0u;
     return packPairs<V>(b, right) & keep;
   }
-  // Bit of the voxel right of the wave, read from memory (wave-uniform).
-  __device__ __forceinline__ uint32_t haloBit(long long off) const {
+  // Value and bit of the voxel right of the wave, read from memory
+  // (wave-uniform).
+  __device__ __forceinline__ T haloVal(long long off) const {
     const long long o = off + (G ? (long long)hxo * sx : (long long)hxo);
     T h = uniformLoad(fw[0] + o);
     if constexpr (G) {
-      for (int f = 1; f < nF; ++f) h += uniformLoad(fw[f] + o);
+#pragma unroll
+      for (int f = 1; f < kMaxFieldsDev; ++f)
+        if (f < nF) h += uniformLoad(fw[f] + o);
     }
-    return (uint32_t)(relMask<M>(h, thr) & 1ull);
+    return h;
+  }
+  __device__ __forceinline__ uint32_t haloBit(long long off) const {
+    return (uint32_t)(relMask<M>(haloVal(off), thr) & 1ull);
+  }
+  // Phase moments of the lane's voxels selected by own (bit k: voxel k).
+  template <class Acc>
+  __device__ __forceinline__ void moments(const Raw& r, uint32_t own, Acc& acc) const {
+#pragma unroll
+    for (int k = 0; k < V; ++k) acc.add(r.v[k], laneBit(relMask<M>(r.v[k], thr)), ((own >> k) & 1u) != 0);
   }
 };
 
@@ -362,6 +381,67 @@ struct PbmRows4 {
   __device__ __forceinline__ uint32_t haloBit(long long) const { return 0u; }
 };
 
+// Per-lane phase moments: sums in double of the values and their squares per
+// phase (map bit 0 or 1) and the counts. Selects, not differences, split the
+// phases, so a NaN stays in its own phase.
+struct MomAcc {
+  double s0 = 0, q0 = 0, s1 = 0, q1 = 0;
+  uint32_t n1 = 0, nAll = 0;
+
+  template <typename T>
+  __device__ __forceinline__ void add(T v, bool in, bool own) {
+    const double f = (double)v;
+    const double fi = (own && in) ? f : 0.0;
+    const double fo = (own && !in) ? f : 0.0;
+    s1 += fi;
+    q1 = __builtin_fma(fi, fi, q1);
+    s0 += fo;
+    q0 = __builtin_fma(fo, fo, q0);
+    n1 += (own && in) ? 1u : 0u;
+    nAll += own ? 1u : 0u;
+  }
+};
+
+constexpr int kMomN = 6;  // {sum0, sumsq0, sum1, sumsq1, n0, n1}
+
+// Writes the workgroup's moments to part[blockIdx.x * kMomN ..] after a
+// fixed-order reduction (lanes by xor shuffles, then waves in order).
+// Every thread of the workgroup must call it.
+__device__ __forceinline__ void momBlockPartial(const MomAcc& a, double* __restrict__ part) {
+  __shared__ double red[16][kMomN];
+  double t[kMomN] = {a.s0, a.q0, a.s1, a.q1, (double)(a.nAll - a.n1), (double)a.n1};
+#pragma unroll
+  for (int j = 0; j < kMomN; ++j) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) t[j] += __shfl_xor(t[j], o);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+    for (int j = 0; j < kMomN; ++j) red[w][j] = t[j];
+  }
+  __syncthreads();
+  if (threadIdx.x < kMomN) {
+    const int nw = blockDim.x >> 6;
+    double s = red[0][threadIdx.x];
+    for (int k = 1; k < nw; ++k) s += red[k][threadIdx.x];
+    part[(size_t)blockIdx.x * kMomN + threadIdx.x] = s;
+  }
+}
+
+// Moments off: every call compiles away.
+struct NoMom {
+  static constexpr bool kOn = false;
+};
+struct OnMom {
+  static constexpr bool kOn = true;
+  MomAcc acc;
+  uint32_t vkeep;     // the lane's voxels inside the field (bit k: voxel k)
+  bool lastRow;       // the stack's row nY is the field's last row
+  bool lastPlane;     // the slab's last plane is the field's last plane
+  bool haloLast;      // the voxel right of the wave is the field's last column, read from memory
+};
+
 // Per-wave tally. Cells that do not exist have code 0 (their pair bits are
 // cleared), so no per-cell validity test is needed. All-one cells are counted
 // from ballots in scalar registers; mixed cells go to the wave's LDS
@@ -403,9 +483,16 @@ __device__ __forceinline__ void loadRows(const Src& src, typename Src::Raw* out,
   ((out[R] = src.template load<ntRow<TY>(R)>(yo[R] + zo)), ...);
 }
 
-template <int TY, int V, class Src>
+// With Mom::kOn the walker also adds the phase moments of the voxels this
+// wave owns: its lanes' voxels inside the field (and, for the wave left of
+// the field's last column when that column is read from memory, lane 63 adds
+// it), in rows 0 .. nY-1 (and nY when that is the field's last row) and
+// planes z0 .. z0+nZ-1 (and z0+nZ when that is the field's last plane), so
+// every voxel is counted once over the grid.
+template <int TY, int V, class Src, class Mom = NoMom>
 __device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1][16], int w, HaloFrom halo,
-                                          int y0, int nY, int z0, int nZ, uint32_t* hist, Tally& tally) {
+                                          int y0, int nY, int z0, int nZ, uint32_t* hist, Tally& tally,
+                                          Mom* mom = nullptr) {
   const int lane = threadIdx.x & (kWave - 1);
   const int nb = min(w + 1, 15);  // neighbour slot, clamped inside the array
   const uint32_t ldsKeep = (halo == kHaloLds) ? 1u : 0u;
@@ -430,12 +517,31 @@ __device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1
       P[r] = src.pairs(cur[r], R0);
       if (!Src::kSelfHalo && lane == 0) xch[par][r][w] = (uint32_t)R0;
     }
+    if constexpr (Mom::kOn) {
+      if (z < z0 + nZ || mom->lastPlane) {
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) {
+          if (r < nY || (r == nY && mom->lastRow)) src.moments(cur[r], mom->vkeep, mom->acc);
+        }
+      }
+    }
     if constexpr (!Src::kSelfHalo) {
       __syncthreads();
       if (halo == kHaloMem) {
         const long long zo = (long long)z * src.sz;
 #pragma unroll
-        for (int r = 0; r <= TY; ++r) P[r] |= (src.haloBit(yo[r] + zo) & lane63) << kRightBit<V>;
+        for (int r = 0; r <= TY; ++r) {
+          if constexpr (Mom::kOn) {
+            const auto h = src.haloVal(yo[r] + zo);
+            const uint32_t hb = (uint32_t)(relMask<Src::kM>(h, src.thr) & 1ull);
+            P[r] |= (hb & lane63) << kRightBit<V>;
+            const bool own = mom->haloLast && lane63 && (z < z0 + nZ || mom->lastPlane) &&
+                             (r < nY || (r == nY && mom->lastRow));
+            mom->acc.add(h, hb != 0, own);
+          } else {
+            P[r] |= (src.haloBit(yo[r] + zo) & lane63) << kRightBit<V>;
+          }
+        }
       } else {
 #pragma unroll
         for (int r = 0; r <= TY; ++r) {

@@ -51,6 +51,7 @@ int setError(int code, const char* fmt, ...);
 
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes);  // fused kernel
 Tiling chooseBpfdTiling(int X, int Y, int Z);              // histogram of a PBM
+Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes);  // fused kernel with moments
 bool tilingSupported(const Tiling& tl, int elemBytes);
 
 // Thresholds fv into the PBM pW (rowWS / planeWS word strides).
@@ -69,6 +70,14 @@ hipError_t launchPBMMoments(const FieldView& fv, const BinMapF64& m, uint32_t* p
 // the field once; no PBM is written.
 hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, const Tiling& tl,
                        hipStream_t s);
+
+// As launchFused, and ADDS the phase moments of fv (as launchPBMMoments) to
+// mom, reading the field once. Needs X, Y, Z >= 2 (hipErrorInvalidValue
+// otherwise); the tiling's row pairs are set to 4. part is device scratch of
+// fusedMomentParts(fv, tl) x 6 doubles.
+long long fusedMomentParts(const FieldView& fv, const Tiling& tl);
+hipError_t launchFusedMoments(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, double* part,
+                              double* mom, const Tiling& tl, hipStream_t s);
 
 // Adds the BPFD of the PBM pW (dims X, Y, Z) to bpfd.
 hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,

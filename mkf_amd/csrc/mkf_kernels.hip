@@ -40,13 +40,11 @@ constexpr int kPbmAhead = MKF_PBM_AHEAD;
 constexpr int kPbmVoxels = 16;  // per lane
 
 // Phase moments (MOM): each lane also sums, in double, the field values of
-// its voxels and their squares per phase (map bit 0 or 1) and counts them.
-// Selects, not differences, split the phases, so a NaN stays in its own
-// phase. Each workgroup writes its totals {s0, q0, s1, q1, n0, n1} to
+// its voxels and their squares per phase (map bit 0 or 1) and counts them
+// (MomAcc). Each workgroup writes its totals {s0, q0, s1, q1, n0, n1} to
 // part[blockIdx.x] after a fixed-order reduction; kMomentsReduce adds the
 // partials in a fixed order, so the result does not depend on scheduling.
 // With pW == nullptr no map is written.
-constexpr int kMomN = 6;
 
 template <typename T, int M, bool G, bool MOM>
 __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, T thr, int X, int Y, int nSegX,
@@ -85,8 +83,7 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   const uint32_t word = (uint32_t)(x0 >> 5) + (uint32_t)(2 * E * jl + lane / LPW);
   const bool writer = pW != nullptr && jl < NQ && word < rowWS;
   const uint32_t sh = (uint32_t)(E * jl);
-  double s0 = 0, q0 = 0, s1 = 0, q1 = 0;
-  uint32_t n1 = 0, nAll = 0;
+  MomAcc acc;
   for (long long i = 0; i < n; i += kPbmAhead) {
     T v[kPbmAhead][V];
     size_t wo[kPbmAhead];
@@ -140,17 +137,7 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
           wq[q] = pw;
           if constexpr (MOM) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-              const double f = (double)v[u][E * q + e];
-              const double fi = ((nib >> e) & 1u) ? f : 0.0;
-              const double fo = ((kq[q] & ~nib) >> e) & 1u ? f : 0.0;
-              s1 += fi;
-              q1 = __builtin_fma(fi, fi, q1);
-              s0 += fo;
-              q0 = __builtin_fma(fo, fo, q0);
-            }
-            n1 += (uint32_t)__builtin_popcount(nib);
-            nAll += (uint32_t)__builtin_popcount(kq[q]);
+            for (int e = 0; e < E; ++e) acc.add(v[u][E * q + e], ((nib >> e) & 1u) != 0, ((kq[q] >> e) & 1u) != 0);
           }
         }
         uint32_t w32 = wq[0];
@@ -160,26 +147,7 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
       }
     }
   }
-  if constexpr (MOM) {
-    __shared__ double red[kPbmWaves][kMomN];
-    double t[kMomN] = {s0, q0, s1, q1, (double)(nAll - n1), (double)n1};
-#pragma unroll
-    for (int j = 0; j < kMomN; ++j) {
-#pragma unroll
-      for (int o = 1; o < kWave; o <<= 1) t[j] += __shfl_xor(t[j], o);
-    }
-    const int w = threadIdx.x >> 6;
-    if (lane == 0) {
-#pragma unroll
-      for (int j = 0; j < kMomN; ++j) red[w][j] = t[j];
-    }
-    __syncthreads();
-    if (threadIdx.x < kMomN) {
-      double a = red[0][threadIdx.x];
-      for (int k = 1; k < kPbmWaves; ++k) a += red[k][threadIdx.x];
-      part[(size_t)blockIdx.x * kMomN + threadIdx.x] = a;
-    }
-  }
+  if constexpr (MOM) momBlockPartial(acc, part);
 }
 
 // Adds the per-workgroup moment partials to out[kMomN] in a fixed order:
@@ -301,13 +269,20 @@ __device__ __forceinline__ uint32_t cellKeep(int x0, int X, uint32_t& cellsPerRo
   return keep;
 }
 
-template <typename T, int M, int V, int TY, bool G>
+// With MOM the kernel also writes the workgroup's phase moments of the
+// voxels it owns (see walkStack) to part[blockIdx.x]; blocks past the last
+// tile write zeros.
+template <typename T, int M, int V, int TY, bool G, bool MOM = false>
 __global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr, int X, int Y, int Z, TileGrid g,
-                                                    unsigned long long* __restrict__ bpfd) {
+                                                    unsigned long long* __restrict__ bpfd,
+                                                    double* __restrict__ part = nullptr) {
   __shared__ uint32_t hist[16][256];
   __shared__ uint32_t xch[2][TY + 1][16];
   int xt, yr, zr;
-  if (!tileOf(g, xt, yr, zr)) return;
+  if (!tileOf(g, xt, yr, zr)) {
+    if (MOM && threadIdx.x < kMomN) part[(size_t)blockIdx.x * kMomN + threadIdx.x] = 0.0;
+    return;
+  }
   const int lane = threadIdx.x & (kWave - 1);
   const WaveGeom q = waveGeom(g, xt, yr, zr, TY, V, X, Y, Z);
   for (int i = lane; i < 256; i += kWave) hist[q.w][i] = 0;
@@ -315,7 +290,8 @@ __global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr,
   const int xb = min(q.x0, X - 1);
   FieldRows<T, M, V, G> src;
   if constexpr (G) {
-    for (int i = 0; i < 4; ++i) src.fw[i] = fa.p[i] + (long long)xb * fa.sx;
+#pragma unroll
+    for (int i = 0; i < kMaxFieldsDev; ++i) src.fw[i] = fa.p[i] + (long long)xb * fa.sx;
     src.nF = fa.nF;
     src.sx = fa.sx;
   } else {
@@ -333,8 +309,22 @@ __global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr,
   uint32_t cellsPerRow;
   src.keep = cellKeep<V>(q.x0, X, cellsPerRow);
   Tally tally;
-  walkStack<TY, V>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist[q.w], tally);
-  flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
+  if constexpr (MOM) {
+    OnMom mom;
+    mom.vkeep = 0;
+#pragma unroll
+    for (int k = 0; k < V; ++k) mom.vkeep |= (q.x0 + V * lane + k < X) ? (1u << k) : 0u;
+    mom.lastRow = q.y0 + q.nY == Y - 1;
+    mom.lastPlane = q.z0 + q.nZ == Z - 1;
+    mom.haloLast = q.halo == kHaloMem && q.x0 + q.W == X - 1;
+    walkStack<TY, V, FieldRows<T, M, V, G>, OnMom>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist[q.w], tally,
+                                                   &mom);
+    flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
+    momBlockPartial(mom.acc, part);
+  } else {
+    walkStack<TY, V>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist[q.w], tally);
+    flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
+  }
 }
 
 template <int TY>
@@ -460,6 +450,28 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   while (tl.TP > 8) {
     const TileGrid g = makeGrid(X, Y, Z, tl);
     if ((long long)g.nTiles * g.waves >= targetWaves) break;
+    tl.TP >>= 1;
+  }
+  return tl;
+}
+
+// Fused kernel with phase moments: the moments add ~9 instructions per voxel,
+// so the kernel needs more voxels per lane and more tiles in flight than the
+// plain one. Measured on MI355X at 1024^3 (best of 3 rounds of 20 launches):
+//   f32: 4 voxels/lane, 64 plane pairs  0.963 ms (128: 1.015, 256: 1.123;
+//        1 voxel/lane, 128: 1.523); plain histogram + moments-only map
+//        pass: 0.715 + 0.670 ms
+//   f64: 2 voxels/lane, 128 plane pairs 1.595 ms (512: 1.609); two passes
+//        1.173 + 1.245 ms
+Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
+  Tiling tl;
+  tl.V = elemBytes >= 8 ? 2 : 4;
+  tl.Ty = 4;
+  tl.TP = elemBytes >= 8 ? 128 : 64;
+  tl.ZW = 16;
+  while (tl.TP > 8) {
+    const TileGrid g = makeGrid(X, Y, Z, tl);
+    if ((long long)g.nTiles * g.waves >= 16384) break;
     tl.TP >>= 1;
   }
   return tl;
@@ -623,6 +635,75 @@ hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long lo
   if (fv.enc == ENC_F32) return dispatchFused<float, 4>(fv, (float)m.t[0], m.m, bpfd, tl, s);
   if (fv.enc == ENC_F64) return dispatchFused<double, 2>(fv, m.t[0], m.m, bpfd, tl, s);
   return hipErrorInvalidValue;
+}
+
+// Fused histogram + phase moments: 4 row pairs per wave (the tiling's slab
+// depth, wave count and voxels per lane are kept).
+template <typename T, int M, int V>
+static hipError_t launchFusedMomT(const FieldView& fv, T thr, unsigned long long* bpfd, double* part,
+                                  const TileGrid& g, hipStream_t s) {
+  const dim3 grid((unsigned)g.chunk * 8u), block(g.waves * kWave);
+  if (fv.nF > 1 || fv.sx != 1) {
+    if constexpr (V == 1)
+      hipLaunchKernelGGL((kFusedBPFD<T, M, 1, 4, true, true>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X, fv.Y,
+                         fv.Z, g, bpfd, part);
+    else
+      return hipErrorInvalidValue;
+  } else {
+    hipLaunchKernelGGL((kFusedBPFD<T, M, V, 4, false, true>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X, fv.Y,
+                       fv.Z, g, bpfd, part);
+  }
+  return hipGetLastError();
+}
+
+template <typename T, int V>
+static hipError_t dispatchFusedMomM(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd, double* part,
+                                    const TileGrid& g, hipStream_t s) {
+  switch (m & 7u) {
+    case 0: return launchFusedMomT<T, 0, V>(fv, thr, bpfd, part, g, s);
+    case 1: return launchFusedMomT<T, 1, V>(fv, thr, bpfd, part, g, s);
+    case 2: return launchFusedMomT<T, 2, V>(fv, thr, bpfd, part, g, s);
+    case 3: return launchFusedMomT<T, 3, V>(fv, thr, bpfd, part, g, s);
+    case 4: return launchFusedMomT<T, 4, V>(fv, thr, bpfd, part, g, s);
+    case 5: return launchFusedMomT<T, 5, V>(fv, thr, bpfd, part, g, s);
+    case 6: return launchFusedMomT<T, 6, V>(fv, thr, bpfd, part, g, s);
+    default: return launchFusedMomT<T, 7, V>(fv, thr, bpfd, part, g, s);
+  }
+}
+
+static Tiling fusedMomTiling(const FieldView& fv, const Tiling& tl0) {
+  Tiling tl = tl0;
+  tl.Ty = 4;
+  if (fv.nF > 1 || fv.sx != 1) tl.V = 1;
+  return tl;
+}
+
+long long fusedMomentParts(const FieldView& fv, const Tiling& tl) {
+  if (fv.X < 2 || fv.Y < 2 || fv.Z < 2) return 0;
+  const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, fusedMomTiling(fv, tl));
+  return g.nTiles > 0 ? (long long)g.chunk * 8 : 0;
+}
+
+hipError_t launchFusedMoments(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, double* part,
+                              double* mom, const Tiling& tl0, hipStream_t s) {
+  const int eb = fv.enc & ENC_MASK_NUM;
+  if (!tilingSupported(tl0, eb)) return hipErrorInvalidValue;
+  const long long nParts = fusedMomentParts(fv, tl0);
+  if (nParts == 0) return hipErrorInvalidValue;  // no cells: the caller takes the map path
+  const Tiling tl = fusedMomTiling(fv, tl0);
+  const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, tl);
+  hipError_t e;
+  if (fv.enc == ENC_F32)
+    e = tl.V == 1 ? dispatchFusedMomM<float, 1>(fv, (float)m.t[0], m.m, bpfd, part, g, s)
+                  : dispatchFusedMomM<float, 4>(fv, (float)m.t[0], m.m, bpfd, part, g, s);
+  else if (fv.enc == ENC_F64)
+    e = tl.V == 1 ? dispatchFusedMomM<double, 1>(fv, m.t[0], m.m, bpfd, part, g, s)
+                  : dispatchFusedMomM<double, 2>(fv, m.t[0], m.m, bpfd, part, g, s);
+  else
+    return hipErrorInvalidValue;
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kMomentsReduce, dim3(1), dim3(256), 0, s, part, nParts, mom);
+  return hipGetLastError();
 }
 
 hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,

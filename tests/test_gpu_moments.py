@@ -150,3 +150,91 @@ def test_moments_uniform_256(lib, cuda):
     _check(got, host_np(f), ">=", 0.5)
     assert got[0] / got[4] == pytest.approx(0.25, abs=1e-3)
     assert got[2] / got[5] == pytest.approx(0.75, abs=1e-3)
+
+
+def _fused_moments(lib, fi, X, Y, Z, relop, thr, cuda, tiling=None):
+    bm = M.bin_map(relop, thr)
+    h = torch.zeros(256, dtype=torch.int64, device=cuda)
+    mom = torch.zeros(M.MKF_MOM_N, dtype=torch.float64, device=cuda)
+    t = M.MkfTiling(*tiling) if tiling is not None else None
+    rc = lib.mkfFusedBPFDMoments(h.data_ptr(), mom.data_ptr(), None, fi.ref, C.byref(bm),
+                                 C.byref(t) if t is not None else None, None)
+    M.check(rc, "mkfFusedBPFDMoments")
+    torch.cuda.synchronize()
+    return h.cpu().numpy().view(np.uint64).copy(), mom.cpu().numpy()
+
+
+# Every voxel must be counted once. The last column x = X - 1 is held by a
+# lane of a real segment (X = 1090, 333), by an extra wave of the workgroup
+# past the last segment (X = 1089), or only read from memory as the halo of
+# the last segment (X = 65, 1025, 2049 at one voxel per lane; 257 at four;
+# 129 at two for doubles). Slabs of 3 plane pairs give several z-tiles.
+FUSED_CASES = [
+    ((5, 7, 65), torch.float32, None),
+    ((5, 7, 65), torch.float32, (3, 16, 4, 1)),
+    ((4, 6, 257), torch.float32, (3, 16, 4, 4)),
+    ((4, 6, 257), torch.float32, (3, 16, 4, 1)),
+    ((3, 4, 1025), torch.float32, (2, 16, 4, 1)),
+    ((3, 3, 1089), torch.float32, (1, 16, 4, 1)),
+    ((3, 3, 1090), torch.float32, (1, 16, 4, 1)),
+    ((2, 2, 2049), torch.float32, None),
+    ((11, 13, 333), torch.float32, (3, 16, 4, 4)),
+    ((4, 5, 129), torch.float64, (3, 16, 4, 2)),
+    ((4, 5, 129), torch.float64, (3, 16, 4, 1)),
+    ((6, 9, 1025), torch.float64, None),
+    ((17, 19, 333), torch.float64, (4, 16, 8, 2)),
+]
+
+
+@pytest.mark.parametrize("shape,dtype,tiling", FUSED_CASES)
+def test_fused_moments_match_oracle(lib, orc, cuda, shape, dtype, tiling):
+    print(f"fused moments {shape} {dtype} {tiling}", flush=True)
+    Z, Y, X = shape
+    f = synth.grf_field(shape, sigma=1.5, seed=X + 7 * Y + 31 * Z, device=cuda, dtype=dtype)
+    fi = M.FieldInfo(f.data_ptr(), (X, Y, Z), enc_of(f))
+    f_np = host_np(f)
+    for relop, thr in ((">=", 0.1), ("<", -0.5)):
+        h, mom = _fused_moments(lib, fi, X, Y, Z, relop, thr, cuda, tiling)
+        bm = M.bin_map(relop, thr)
+        words = orc.pbm_c(f_np, bm.t[0], bm.m)
+        h_ref = orc.bpfd_chunk(words, X) if X >= 32 else orc.bpfd_direct_c(words, X)
+        np.testing.assert_array_equal(h, h_ref)
+        _check(mom, f_np, relop, thr)
+
+
+def test_fused_moments_summed_strided_and_cell_free(lib, orc, cuda):
+    Z, Y, X = 9, 11, 200
+    fs = [synth.grf_field((Z, Y, X), sigma=1.5, seed=50 + i, device=cuda, dtype=torch.float32) for i in range(2)]
+    fi = M.FieldInfo([g.data_ptr() for g in fs], (X, Y, Z), M.ENC_F32)
+    s = host_np(fs[0]) + host_np(fs[1])
+    h, mom = _fused_moments(lib, fi, X, Y, Z, ">", 0.0, cuda)
+    bm = M.bin_map(">", 0.0)
+    np.testing.assert_array_equal(h, orc.bpfd_chunk(orc.pbm_c(s, bm.t[0], bm.m), X))
+    _check(mom, s, ">", 0.0)
+    sx = 2
+    base = synth.grf_field((Z, Y, X * sx), sigma=2.0, seed=58, device=cuda, dtype=torch.float64)
+    fi = M.FieldInfo(base.data_ptr(), (X, Y, Z), M.ENC_F64, strides=(sx, X * sx, X * sx * Y))
+    view = np.ascontiguousarray(host_np(base)[:, :, ::sx])
+    h, mom = _fused_moments(lib, fi, X, Y, Z, "<=", 0.2, cuda)
+    bm = M.bin_map("<=", 0.2)
+    np.testing.assert_array_equal(h, orc.bpfd_chunk(orc.pbm_c(view, bm.t[0], bm.m), X))
+    _check(mom, view, "<=", 0.2)
+    # one plane: no cells, the histogram stays zero, the moments still come
+    flat = synth.grf_field((1, 7, 90), sigma=1.0, seed=59, device=cuda, dtype=torch.float32)
+    fi = M.FieldInfo(flat.data_ptr(), (90, 7, 1), M.ENC_F32)
+    h, mom = _fused_moments(lib, fi, 90, 7, 1, ">=", 0.0, cuda)
+    assert not h.any()
+    _check(mom, host_np(flat), ">=", 0.0)
+    assert mom[4] + mom[5] == 90 * 7
+
+
+def test_fused_moments_agree_with_map_moments(lib, cuda):
+    Z, Y, X = 70, 90, 1030
+    f = synth.grf_field((Z, Y, X), sigma=2.5, seed=77, device=cuda, dtype=torch.float32)
+    fi = M.FieldInfo(f.data_ptr(), (X, Y, Z), M.ENC_F32)
+    _, a = _fused_moments(lib, fi, X, Y, Z, ">=", 0.3, cuda)
+    b, _ = _moments(lib, fi, X, Y, Z, ">=", 0.3, cuda, with_map=False)
+    assert a[4] == b[4] and a[5] == b[5]
+    np.testing.assert_allclose(a[:4], b[:4], rtol=1e-12)
+    _, a2 = _fused_moments(lib, fi, X, Y, Z, ">=", 0.3, cuda)
+    assert a.tobytes() == a2.tobytes()
