@@ -212,9 +212,13 @@ typedef struct MkfServer MkfServer;
 
 /* Starts a server on the current device (cfg may be NULL). NULL on error. */
 MkfServer *mkfServeCreate(const MkfServeConfig *cfg);
-/* Queues a request; thread-safe and non-blocking. MKF_ERR_BUDGET if its
- * field can never fit the budget. The field, and for a device field its
- * memory, must stay valid and unchanged until the result is reported. */
+/* Queues a request; thread-safe and non-blocking. A host field larger than
+ * the budget is served as z-slabs of at most a quarter of the budget that
+ * share one plane (each an internal request admitted, copied and batched like
+ * any other); its result is reported once, after the last slab, with the
+ * summed histograms. MKF_ERR_BUDGET only if two planes exceed the budget.
+ * The field, and for a device field its memory, must stay valid and
+ * unchanged until the result is reported. */
 int mkfServeSubmit(MkfServer *s, const MkfServeRequest *r);
 /* Moves up to maxOut reported results to out, waiting up to waitMicros for
  * the first. Returns the number moved, or < 0 on error. */

@@ -95,8 +95,22 @@ class Arena {
   std::map<size_t, size_t> free_;
 };
 
+// A host request whose field does not fit the budget is served as z-slabs
+// (sharing one plane, so every plane pair is in exactly one slab); the
+// parent sums the slabs' histograms and is reported once, after the last.
+struct Parent {
+  MkfServeRequest r;         // the caller's request (pBPFD: the caller's output)
+  std::vector<size_t> acc;   // nLevels x 256
+  int remaining = 0;
+  int status = MKF_OK;
+  int batchMax = 0;
+  double tSubmit = 0, tAdmit = 1e300, tLaunch = 0;
+};
+
 struct Request {
   MkfServeRequest r;
+  Parent* parent = nullptr;        // set for the slabs of a split request
+  std::vector<size_t> slabOut;     // a slab's histograms (r.pBPFD points here)
   BinMapF64 maps[MKF_BATCH_MAX_LEVELS];
   size_t bytes = 0;  // field bytes
   const void* dField = nullptr;
@@ -162,6 +176,7 @@ hipEvent_t takeEvent(MkfServer* S) {
   return e;
 }
 
+// Worker thread only (parents are touched by no other thread).
 void report(MkfServer* S, Request* q, int status, int batchSize, double tDone) {
   MkfServeResult res;
   res.tag = q->r.tag;
@@ -171,11 +186,33 @@ void report(MkfServer* S, Request* q, int status, int batchSize, double tDone) {
   res.tAdmit = q->tAdmit;
   res.tLaunch = q->tLaunch;
   res.tDone = tDone;
+  Parent* p = q->parent;
+  if (p) {
+    if (status == MKF_OK) {
+      for (size_t i = 0; i < p->acc.size(); ++i) p->acc[i] += q->slabOut[i];
+    } else if (p->status == MKF_OK) {
+      p->status = status;
+    }
+    p->tAdmit = std::min(p->tAdmit, q->tAdmit);
+    p->tLaunch = std::max(p->tLaunch, q->tLaunch);
+    p->batchMax = std::max(p->batchMax, batchSize);
+  }
   if (q->owned) S->arena.free(q->owned, q->bytes);
   if (q->copied) S->eventPool.push_back(q->copied);
   if (q->produced) (void)hipEventDestroy(q->produced);
   delete q;
   S->admitted--;
+  if (p) {
+    if (--p->remaining > 0) return;
+    if (p->status == MKF_OK) memcpy(p->r.pBPFD, p->acc.data(), sizeof(size_t) * p->acc.size());
+    res.tag = p->r.tag;
+    res.status = p->status;
+    res.batchSize = p->batchMax;
+    res.tSubmit = p->tSubmit;
+    res.tAdmit = p->tAdmit;
+    res.tLaunch = p->tLaunch;
+    delete p;
+  }
   std::lock_guard<std::mutex> g(S->mu);
   S->results.push_back(res);
   S->stats.completed++;
@@ -437,22 +474,60 @@ int mkfServeSubmit(MkfServer* S, const MkfServeRequest* r) {
   }
   memcpy(q->maps, r->pM, sizeof(BinMapF64) * r->nLevels);
   q->r.pM = q->maps;
-  q->bytes = (size_t)d[0] * d[1] * d[2] * (r->enc == ENC_F32 ? 4 : 8);
+  const size_t eb = r->enc == ENC_F32 ? 4 : 8;
+  q->bytes = (size_t)d[0] * d[1] * d[2] * eb;
+  std::vector<Request*> parts{q};
   if (!(r->flags & MKF_SERVE_ON_DEVICE) && alignUp(q->bytes) > S->arena.capacity()) {
-    if (q->produced) (void)hipEventDestroy(q->produced);
+    // Split along the slowest non-trivial axis into slabs of at most a
+    // quarter of the budget that share one plane.
+    const int k = nd - 1;
+    size_t planeBytes = eb;
+    for (int i = 0; i < k; ++i) planeBytes *= (size_t)d[i];
+    const long long planes = d[k];
+    const long long sp = std::min<long long>(planes - 1, (long long)(S->arena.capacity() / 4 / planeBytes) - 1);
+    if (planes < 2 || sp < 1 || alignUp(planeBytes * (size_t)(sp + 1)) > S->arena.capacity()) {
+      delete q;
+      return setError(MKF_ERR_BUDGET, "serve: field of %zu bytes exceeds the budget of %zu, and so do two planes",
+                      alignUp(q->bytes), S->arena.capacity());
+    }
+    Parent* par = new Parent();
+    par->r = *r;
+    par->acc.assign((size_t)r->nLevels * 256, 0);
+    parts.clear();
+    for (long long z0 = 0; z0 < planes - 1; z0 += sp) {
+      const long long z1 = std::min<long long>(z0 + sp, planes - 1);
+      Request* sq = new Request();
+      sq->r = q->r;
+      sq->r.pF = (const char*)r->pF + (size_t)z0 * planeBytes;
+      for (int i = 0; i < 3; ++i) sq->r.def[i] = d[i];
+      sq->r.def[k] = (FieldDef)(z1 - z0 + 1);
+      memcpy(sq->maps, q->maps, sizeof(BinMapF64) * r->nLevels);
+      sq->r.pM = sq->maps;
+      sq->slabOut.assign((size_t)r->nLevels * 256, 0);
+      sq->r.pBPFD = sq->slabOut.data();
+      sq->bytes = planeBytes * (size_t)(z1 - z0 + 1);
+      sq->parent = par;
+      parts.push_back(sq);
+    }
+    par->remaining = (int)parts.size();
     delete q;
-    return setError(MKF_ERR_BUDGET, "serve: field of %zu bytes exceeds the budget of %zu", alignUp(q->bytes),
-                    S->arena.capacity());
   }
   {
     std::lock_guard<std::mutex> g(S->mu);
     if (S->stop) {
-      if (q->produced) (void)hipEventDestroy(q->produced);
-      delete q;
+      for (Request* x : parts) {
+        if (x->produced) (void)hipEventDestroy(x->produced);
+        if (x->parent && --x->parent->remaining == 0) delete x->parent;
+        delete x;
+      }
       return setError(MKF_ERR_ARG, "serve: server is stopping");
     }
-    q->tSubmit = S->now();
-    S->pending.push_back(q);
+    const double t = S->now();
+    for (Request* x : parts) {
+      x->tSubmit = t;
+      if (x->parent) x->parent->tSubmit = t;
+      S->pending.push_back(x);
+    }
     S->stats.submitted++;
   }
   S->cvWork.notify_one();

@@ -107,13 +107,49 @@ def test_serve_budget_admission(orc, cuda):
         st = S.stats()
         assert 0 < st["heldMax"] <= 2 * nbytes + 4096 + 512
         assert st["bytesIn"] == 10 * nbytes
+        # a host field larger than the budget is served in z-slabs of at most
+        # a quarter of the budget, reported once with the summed histograms
         too_big = pinned(synth.grf_field_direct((100, 100, 100), 2.0, seed=22, device=cuda, dtype=torch.float64))
+        S.submit(too_big, lv, tag=98)
+        r = wait_all(S, 1)[98]
+        assert r["status"] == 0
+        np.testing.assert_array_equal(r["hist"], oracle(orc, too_big, lv))
+        assert S.stats()["heldMax"] <= 2 * nbytes + 4096 + 512
+        # only a field of which two planes exceed the budget is refused
+        wide = pinned(torch.zeros((3, 600, 600), dtype=torch.float64))
         with pytest.raises(RuntimeError, match="budget"):
-            S.submit(too_big, lv)
+            S.submit(wide, lv)
         # a device-resident field needs no budget
         S.submit(too_big.to(cuda), lv, tag=99)
         r = wait_all(S, 1)[99]
         np.testing.assert_array_equal(r["hist"], oracle(orc, too_big, lv))
+    finally:
+        S.close()
+
+
+def test_serve_split_requests_in_a_mix(orc, cuda):
+    """Oversized host requests (split into slabs) interleaved with ordinary
+    host and device requests: every result equals the oracle."""
+    S = Server(budget=3 << 20, max_in_flight=24, max_batch=6)
+    try:
+        fields, want = {}, {}
+        specs = [((40, 50, 60), torch.float32, 1.5), ((90, 80, 70), torch.float64, 2.5),
+                 ((20, 30, 40), torch.float64, 1.0), ((130, 60, 110), torch.float32, 3.0),
+                 ((64, 64, 64), torch.float32, 2.0), ((300, 40, 33), torch.float64, 1.5)]
+        lv = [(">=", -0.5), ("<", 0.2), (">", 1.0)]
+        for tag, (shape, dt, sigma) in enumerate(specs):
+            f = synth.grf_field_direct(shape, sigma, seed=200 + tag, device=cuda, dtype=dt)
+            fields[tag] = pinned(f) if tag % 3 != 2 else f
+            want[tag] = oracle(orc, f, lv)
+        for tag, f in fields.items():
+            S.submit(f, lv, tag=tag)
+        got = wait_all(S, len(fields))
+        for tag in fields:
+            assert got[tag]["status"] == 0, tag
+            np.testing.assert_array_equal(got[tag]["hist"], want[tag])
+        st = S.stats()
+        assert st["submitted"] == len(fields) and st["completed"] == len(fields)
+        assert st["heldMax"] <= (3 << 20) + 512
     finally:
         S.close()
 
