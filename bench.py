@@ -362,7 +362,8 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}): "
-              "this build has no multi-GPU path; see DESIGN.md for how the field would shard.", file=sys.stderr)
+              "there is no multi-GPU path (a field would shard along z, one slab and one shared plane per GPU, "
+              "the 256-bin histograms summed).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

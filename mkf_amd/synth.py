@@ -2,7 +2,7 @@
 
 Every generator is deterministic for a seed and returns a (Z, Y, X) tensor
 (x fastest, the planar layout the kernels read). Fields are generated once;
-the same bytes go to the GPU kernels and to the CPU checker.
+the same bytes go to the GPU kernels and to the CPU oracle.
 
   uniform_field  i.i.d. U[0,1) float32 from a counter hash: (h >> 40) * 2^-24
   grf_field      Gaussian random field: N(0,1) white noise, periodic Gaussian

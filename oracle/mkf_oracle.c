@@ -1,7 +1,7 @@
 /* mkf_oracle.c - CPU ORACLE for the MKF hot path. TEST INFRASTRUCTURE ONLY.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
- * this library, and only as the checker or as the timed CPU baseline; the
+ * this library, and only to check results or as the timed CPU baseline; the
  * product (libmkf_amd.so) never calls it.
  *
  * It restates the reference's CPU path in plain C:

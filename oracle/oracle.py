@@ -1,7 +1,7 @@
 """CPU ORACLE for the MKF hot path -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
-import this module, and only as the checker (or the timed CPU baseline).
+import this module, and only to check results (or as the timed CPU baseline).
 The product library never calls it.
 
 Two independent restatements of the reference's CPU path are provided:

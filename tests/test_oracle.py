@@ -1,4 +1,4 @@
-"""Pins the CPU oracle before it is trusted as the GPU checker.
+"""Pins the CPU oracle before GPU results are compared with it.
 
 The reference ships no golden vectors, fixtures or tests for this path and
 cannot be built here, so every check below compares the oracle's restatement
