@@ -357,6 +357,8 @@ def main():
     ap.add_argument("--pmc-workload", action="store_true",
                     help="for counter runs: 3 fused launches, then 3 copies of a 1 GiB buffer (the calibration "
                          "kernel for FETCH_SIZE / WRITE_SIZE); with --mode batch, 3 batched launches; prints no JSON")
+    ap.add_argument("--pmc-target", choices=("fused", "bpfd"), default="fused",
+                    help="with --pmc-workload: 'bpfd' makes the PBM once and runs 3 histograms of it instead")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -420,6 +422,19 @@ def main():
         if rc != 0:
             raise RuntimeError(f"mkfFusedBPFD failed: {M.last_error()}")
 
+    if args.pmc_workload and args.pmc_target == "bpfd":
+        # The PBM of the field once, then 3 histogram launches of the PBM.
+        org = M.BMOrg()
+        nwords = lib.setBMO(C.byref(org), M.dims((S, S, S)), b"\x00")
+        pbm = torch.empty(nwords, dtype=torch.int32, device=dev)
+        M.check(lib.mkfPBMAsync(pbm.data_ptr(), C.byref(org), fi.ref, C.byref(bm), sptr), "mkfPBMAsync")
+        del field
+        torch.cuda.synchronize()
+        for _ in range(3):
+            M.check(lib.mkfBPFDAsync(hist.data_ptr(), C.byref(org), pbm.data_ptr(), None, sptr), "mkfBPFDAsync")
+        torch.cuda.synchronize()
+        log("[bench] counter workload done: 1 PBM, 3 histograms of the PBM")
+        return
     if args.pmc_workload:
         for _ in range(3):
             launch()
