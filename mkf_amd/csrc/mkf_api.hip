@@ -80,11 +80,11 @@ struct EventTimer {
   }
 };
 
-// Resolves the first valid field of pI into a row-contiguous view with the
-// dims > 1 compacted to the front (src/gridUtil.cu:30-56 validate, plus
-// src/binMapCUDA.cuh:149-176 for the pointer choice). Several enabled fields
-// (the reference's summed multi-field map) and non-unit x strides are not
-// supported.
+// Resolves pI into a field view: the first (up to) 4 valid entries of the
+// table (summed by the kernels), the dims > 1 compacted to the front, and the
+// element strides (pS, any positive x, y, z strides; planar by default)
+// (src/gridUtil.cu:30-56 validate, src/binMapCUDA.cuh:149-212 for the
+// pointer choice and the multi-field sum).
 int resolveField(const BMFieldInfo* pI, FieldView* fv, FieldDef def[3]) {
   if (!pI) return fail(MKF_ERR_ARG, "NULL field info");
   int nField = 0;
