@@ -83,3 +83,37 @@ def test_slab_against_oracle(lib, orc, big):
     f = big[:zs].cpu().numpy()
     bm = M.bin_map(">=", 0.5)
     np.testing.assert_array_equal(h, orc.bpfd_field(f, bm.t[0], bm.m)[0])
+
+
+def test_fused_moments_and_batch_on_the_large_field(lib, big, cuda):
+    """The one-pass histogram + moments, and the batched kernel (whose
+    per-workgroup cell counter is flushed before it passes 2^31) with the
+    large field next to a small one, give the fused kernel's histograms."""
+    from mkf_amd.batch import Batch
+
+    print("large field: fused moments and batch", flush=True)
+    cells = (X - 1) * (Y - 1) * (Z - 1)
+    ge = fused(lib, big, ">=", 0.5)
+    h = torch.zeros(256, dtype=torch.int64, device=cuda)
+    mom = torch.zeros(M.MKF_MOM_N, dtype=torch.float64, device=cuda)
+    fi = M.FieldInfo(big.data_ptr(), (X, Y, Z), M.ENC_F32)
+    bm = M.bin_map(">=", 0.5)
+    M.check(lib.mkfFusedBPFDMoments(h.data_ptr(), mom.data_ptr(), None, fi.ref, C.byref(bm), None, None),
+            "mkfFusedBPFDMoments")
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(h.cpu().numpy().view(np.uint64), ge)
+    m = mom.cpu().numpy()
+    assert m[4] + m[5] == X * Y * Z
+    small = torch.rand((40, 50, 60), generator=torch.Generator(device=cuda).manual_seed(3), device=cuda)
+    b = Batch([big, small], [[(">=", 0.5), ("<", 0.5)], [(">=", 0.5)]])
+    try:
+        b.run()
+        torch.cuda.synchronize()
+        hb = b.hist(0).cpu().numpy().view(np.uint64)
+        np.testing.assert_array_equal(hb[0], ge)
+        np.testing.assert_array_equal(hb[1], ge[::-1])
+        assert int(hb[0].sum()) == cells
+        np.testing.assert_array_equal(b.hist(1).cpu().numpy().view(np.uint64)[0],
+                                      fused(lib, small, ">=", 0.5, dims=(60, 50, 40)))
+    finally:
+        b.close()
