@@ -141,7 +141,10 @@ Tiling fusedTiling(const MkfTiling* pT, const FieldView& fv) {
   const int eb = fv.enc & ENC_MASK_NUM;
   if (pT) {
     Tiling tl{pT->planePairs, pT->waves, pT->rowPairs, pT->voxelsPerLane};
-    if (tl.V == 0) tl.V = chooseTiling(fv.X, fv.Y, fv.Z, eb).V;
+    if (tl.V == 0) {
+      tl.V = chooseTiling(fv.X, fv.Y, fv.Z, eb).V;
+      if (tl.V == 16 && tl.Ty == 8) tl.V = 4;  // the 16-voxel kernel stacks 2 or 4 row pairs
+    }
     return tl;
   }
   {
@@ -155,7 +158,10 @@ Tiling fusedTiling(const MkfTiling* pT, const FieldView& fv) {
 Tiling toTiling(const MkfTiling* pT, int X, int Y, int Z, int elemBytes) {
   if (pT) {
     Tiling tl{pT->planePairs, pT->waves, pT->rowPairs, pT->voxelsPerLane};
-    if (tl.V == 0) tl.V = chooseTiling(X, Y, Z, elemBytes).V;
+    if (tl.V == 0) {
+      tl.V = chooseTiling(X, Y, Z, elemBytes).V;
+      if (tl.V == 16 && tl.Ty == 8) tl.V = 4;  // the 16-voxel kernel stacks 2 or 4 row pairs
+    }
     return tl;
   }
   return chooseTiling(X, Y, Z, elemBytes);
@@ -298,7 +304,7 @@ int mkfFusedAutotune(const BMFieldInfo* pI, const BinMapF64* pM, MkfTiling* pCho
   std::vector<Tiling> cand{chooseTiling(fv.X, fv.Y, fv.Z, eb)};
   if (eb == 4) {
     cand.push_back(Tiling{256, 16, 4, 4});
-    cand.push_back(Tiling{64, 16, 4, 1});
+    cand.push_back(Tiling{128, 16, 4, 1});
   } else {
     cand.push_back(Tiling{512, 16, 8, 1});
     cand.push_back(Tiling{256, 16, 4, 2});

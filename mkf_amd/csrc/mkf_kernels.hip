@@ -7,6 +7,9 @@
 //                src/bpfdCUDA.cuh:158-206)
 //   kFusedBPFD   field -> histogram, reading the field once (no reference
 //                counterpart; the reference round-trips the PBM through memory)
+#include <map>
+#include <mutex>
+
 #include "mkf_device.hpp"
 #include "mkf_internal.h"
 
@@ -384,6 +387,134 @@ __global__ __launch_bounds__(1024) void kBPFD4(const uint32_t* __restrict__ pW, 
 }
 
 // ---------------------------------------------------------------------------
+// Fused histogram, float fields, 16 voxels per lane (voxelsPerLane 16). The
+// lane layout of kPBM: lane l holds groups q = 0..3 of 4 voxels at
+// x0 + 256 q + 4 l, so each 16-byte load of the wave reads 1 KiB of
+// consecutive bytes and a wave covers a 1024-voxel segment of its rows. The
+// right neighbour of a group's last voxel is the next lane's first voxel of
+// the group: bit l + 1 of that voxel's ballot; lane 63 takes bit 0 of the
+// next group's ballot, or, for the last group, the voxel right of the
+// segment, read by a scalar load issued with the row loads one plane ahead.
+// A group's 4 voxel bits and its right bit enter a 5-bit value by carry-in
+// adds; one multiply spreads it into the byte-pair word of its 4 cells. One
+// wave per workgroup: no LDS exchange and no barrier.
+// ---------------------------------------------------------------------------
+constexpr int kG16 = 16;
+
+template <int M, int TY>
+__global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, float thr, int X, int Y, int Z,
+                                                  long long sy, long long sz, TileGrid g,
+                                                  unsigned long long* __restrict__ bpfd) {
+  typedef float Vec __attribute__((ext_vector_type(4), aligned(4)));
+  typedef const __attribute__((address_space(1))) Vec* GVec;
+  __shared__ uint32_t hist[1][256];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < 256; i += kWave) hist[0][i] = 0;
+  Tally tally;
+  uint32_t cells = 0;
+  // Persistent: block b takes tiles b, b + gridDim.x, ... in the XCD-aware
+  // order (L = (t & 7) * chunk + (t >> 3)), so all of a block's tiles lie in
+  // its XCD's share of the field (the grid is a multiple of 8).
+  for (int t = blockIdx.x; t < g.chunk * 8; t += gridDim.x) {
+    const int L = (t & 7) * g.chunk + (t >> 3);
+    if (L >= g.nTiles) continue;
+    const int xt = L % g.nXT;
+    const int rr = L / g.nXT;
+    const int yr = rr % g.nYR;
+    const int zr = rr / g.nYR;
+    const int x0 = xt * (kWave * kG16);
+    const int y0 = yr * TY;
+    const int nY = min(TY, Y - 1 - y0);
+    const int z0 = zr * g.TZ;
+    const int nZ = min(g.TZ, Z - 1 - z0);
+    const bool vec = x0 + kWave * kG16 <= X;
+    const bool halo = x0 + kWave * kG16 < X;  // the voxel right of the segment exists
+    // cells that exist: x < X - 1
+    uint32_t keep[4], nCells = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      keep[q] = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const bool ok = x0 + 256 * q + 4 * lane + e < X - 1;
+        keep[q] |= ok ? (0xFFu << (8 * e)) : 0u;
+        nCells += ok ? 1u : 0u;
+      }
+    }
+    cells += waveSum(nCells) * (uint32_t)nY * (uint32_t)nZ;
+    long long yo[TY + 1];
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) yo[r] = (long long)(y0 + min(r, nY)) * sy;
+    const float* base = f + x0;
+
+    auto loadPlane = [&](long long zo, float (&v)[TY + 1][kG16], float (&h)[TY + 1]) {
+#pragma unroll
+      for (int r = 0; r <= TY; ++r) {
+        const float* row = base + yo[r] + zo;
+        if (vec) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const Vec w = (r > 0 && r < TY) ? __builtin_nontemporal_load((GVec)(row + 256 * q + 4 * lane))
+                                            : *(const Vec*)(row + 256 * q + 4 * lane);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[r][4 * q + e] = w[e];
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[r][4 * q + e] = row[min(256 * q + 4 * lane + e, X - 1 - x0)];
+        }
+        h[r] = halo ? uniformLoad(row + kWave * kG16) : 0.f;
+      }
+    };
+
+    // One plane of registers: the pair words of plane z are computed first,
+    // then plane z+1 is loaded into the same registers while plane z's cells
+    // are tallied.
+    float cur[TY + 1][kG16];
+    float hc[TY + 1];
+    loadPlane((long long)z0 * sz, cur, hc);
+    uint32_t Hp[TY][4];
+    for (int z = z0; z <= z0 + nZ; ++z) {
+      uint32_t P[TY + 1][4];
+#pragma unroll
+      for (int r = 0; r <= TY; ++r) {
+        uint64_t m[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) m[q][e] = relMask<M>(cur[r][4 * q + e], thr);
+        const uint64_t hb = (halo && (relMask<M>(hc[r], thr) & 1ull)) ? 1ull : 0ull;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint64_t nextFirst = q < 3 ? (m[q + 1][0] & 1ull) : hb;
+          uint32_t v5 = laneBit((m[q][0] >> 1) | (nextFirst << 63)) ? 1u : 0u;
+#pragma unroll
+          for (int e = 3; e >= 0; --e) v5 = dblAddLaneBit(v5, m[q][e]);
+          P[r][q] = ((v5 * 0x204081u) & 0x03030303u) & keep[q];
+        }
+      }
+      if (z < z0 + nZ) loadPlane((long long)(z + 1) * sz, cur, hc);
+#pragma unroll
+      for (int r = 0; r < TY; ++r) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t Hc = P[r][q] | (P[r + 1][q] << 2);
+          if (z > z0 && r < nY) {
+            const uint32_t C = Hp[r][q] | (Hc << 4);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) tally.add((C >> (8 * k)) & 255u, hist[0]);
+          }
+          Hp[r][q] = Hc;
+        }
+      }
+    }
+  }
+  flushHistogram(hist, tally, cells, bpfd);
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 static inline long long ceilDiv(long long a, long long b) { return (a + b - 1) / b; }
@@ -414,8 +545,11 @@ static TileGrid makeGrid(int X, int Y, int Z, const Tiling& tl) {
 }
 
 bool tilingSupported(const Tiling& tl, int elemBytes) {
-  const bool vOk = tl.V == 1 || (elemBytes == 4 && tl.V == 4) || (elemBytes == 8 && tl.V == 2);
-  return vOk && (tl.Ty == 2 || tl.Ty == 4 || tl.Ty == 8) && tl.TP >= 1 && tl.ZW >= 1 && tl.ZW <= 16;
+  const bool vOk = tl.V == 1 || (elemBytes == 4 && (tl.V == 4 || tl.V == 16)) || (elemBytes == 8 && tl.V == 2);
+  const bool tyOk = tl.Ty == 2 || tl.Ty == 4 || (tl.Ty == 8 && tl.V != 16);
+  // A workgroup counts its cells in 32 bits: a tile holds at most
+  // 4096 voxels x 8 rows x 65536 planes = 2^31 cells.
+  return vOk && tyOk && tl.TP >= 1 && tl.TP <= 65536 && tl.ZW >= 1 && tl.ZW <= 16;
 }
 
 // Fused kernel: workgroups as wide as the row (up to 16 waves) and the
@@ -432,9 +566,43 @@ bool tilingSupported(const Tiling& tl, int elemBytes) {
 //        1 voxel/lane, 8 row pairs, 256 plane pairs   1.293 ms
 // The f32 leaders are within 1 % and swap between machines; the library's
 // autotuner times them on the caller's device.
+static unsigned fused16Slots(const void* kernel);
+
+// Slab depth of the persistent 16-voxel kernel: its waves take tiles round
+// robin, so the time goes as the most tiles any wave takes times the planes a
+// tile reads (TP + 1, its first plane read twice). The depth with the lowest
+// ceil(tiles / slots) * (TP + 1) wins (ties: the deeper slab). At 1024^3 on
+// MI355X (2048 slots) that is 128 plane pairs (one tile per wave): 0.651 ms;
+// 64: 0.653, 32: 0.666, 96: 0.833 (some waves take two tiles, most one).
+static int fused16Depth(int X, int Y, int Z, int TY) {
+  const long long slots = fused16Slots((const void*)kFused16<6, 4>);
+  const long long cols = ceilDiv(X - 1, (long long)kWave * kG16) * ceilDiv(Y - 1, TY);
+  int best = 4;
+  long long bestCost = -1;
+  for (int tp = 4; tp <= 512; ++tp) {
+    const long long tiles = cols * ceilDiv(Z - 1, tp);
+    const long long cost = ceilDiv(tiles, slots) * (tp + 1);
+    if (bestCost < 0 || cost <= bestCost) {
+      bestCost = cost;
+      best = tp;
+    }
+  }
+  return best;
+}
+
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   Tiling tl;
   long long targetWaves;
+  // Float fields whose rows fill the 1024-voxel segments of the 16-voxel
+  // kernel at least 3/4 use it (1024^3: 0.651 ms, 6.6 TB/s, against 0.69-0.70
+  // for the 4- and 1-voxel tilings below).
+  if (elemBytes == 4 && X >= 2 && (double)(X - 1) >= 0.75 * 1024.0 * ceilDiv(X - 1, (long long)kWave * kG16)) {
+    tl.V = 16;
+    tl.Ty = 4;
+    tl.ZW = 16;
+    tl.TP = fused16Depth(X, Y, Z, 4);
+    return tl;
+  }
   if (elemBytes >= 8) {
     tl.V = 2;
     tl.Ty = 4;
@@ -618,11 +786,76 @@ template <typename T, int VW>
 static hipError_t dispatchFused(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl0,
                                 hipStream_t s) {
   Tiling tl = tl0;
-  if (fv.nF > 1 || fv.sx != 1) tl.V = 1;  // the general form gathers one voxel per lane
+  if (fv.nF > 1 || fv.sx != 1 || tl.V == 16) tl.V = 1;  // the general form gathers one voxel per lane
   const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, tl);
   if (g.nTiles <= 0) return hipSuccess;
   if (tl.V == 1) return dispatchFusedTY<T, 1>(fv, thr, m, bpfd, g, tl.Ty, s);
   return dispatchFusedTY<T, VW>(fv, thr, m, bpfd, g, tl.Ty, s);
+}
+
+// Resident one-wave workgroups of a kernel on the current device, a multiple
+// of 8 (computed once per kernel and device).
+static unsigned fused16Slots(const void* kernel) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, unsigned> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find({kernel, dev});
+  if (it != cache.end()) return it->second;
+  int perCU = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, kWave, 0) != hipSuccess || perCU < 1) perCU = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 8;
+  const unsigned n = std::max(8u, ((unsigned)(perCU * cus)) / 8u * 8u);
+  cache[{kernel, dev}] = n;
+  return n;
+}
+
+template <int M>
+static hipError_t launchFused16M(const FieldView& fv, float thr, unsigned long long* bpfd, const Tiling& tl,
+                                 hipStream_t s) {
+  TileGrid g{};
+  g.nSegX = (int)ceilDiv(fv.X - 1, (long long)kWave * kG16);
+  g.waves = 1;
+  g.nXT = g.nSegX;
+  g.TZ = tl.TP;
+  g.nYR = (int)ceilDiv(fv.Y - 1, tl.Ty);
+  g.nZR = (int)ceilDiv(fv.Z - 1, tl.TP);
+  g.nTiles = g.nXT * g.nYR * g.nZR;
+  g.chunk = (int)ceilDiv(g.nTiles, 8);
+  if (g.nTiles <= 0) return hipSuccess;
+  // One wave per resident slot (rounded down to a multiple of 8 XCDs), no
+  // more than there are tiles (rounded up to 8).
+  const unsigned slots = fused16Slots(tl.Ty == 2 ? (const void*)kFused16<M, 2> : (const void*)kFused16<M, 4>);
+  unsigned nBlocks = std::min<unsigned>(slots, (unsigned)g.chunk * 8u);
+  // A workgroup's histogram and cell count are 32-bit: no workgroup takes
+  // more than 2^31 cells (only fields of far more than 2^40 cells need more
+  // workgroups than slots for that).
+  const long long tileCells = (long long)kWave * kG16 * tl.Ty * tl.TP;
+  const long long maxTiles = std::max(1LL, (1LL << 31) / tileCells);
+  const long long need = ceilDiv(ceilDiv((long long)g.chunk * 8, maxTiles), 8) * 8;
+  if (need > (long long)nBlocks) nBlocks = (unsigned)std::min<long long>(need, (long long)g.chunk * 8);
+  const dim3 grid(nBlocks), block(kWave);
+  const float* p = (const float*)fv.p[0];
+  switch (tl.Ty) {
+    case 2: hipLaunchKernelGGL((kFused16<M, 2>), grid, block, 0, s, p, thr, fv.X, fv.Y, fv.Z, fv.sy, fv.sz, g, bpfd); break;
+    default: hipLaunchKernelGGL((kFused16<M, 4>), grid, block, 0, s, p, thr, fv.X, fv.Y, fv.Z, fv.sy, fv.sz, g, bpfd); break;
+  }
+  return hipGetLastError();
+}
+
+static hipError_t launchFused16(const FieldView& fv, float thr, uint32_t m, unsigned long long* bpfd,
+                                const Tiling& tl, hipStream_t s) {
+  switch (m & 7u) {
+    case 0: return launchFused16M<0>(fv, thr, bpfd, tl, s);
+    case 1: return launchFused16M<1>(fv, thr, bpfd, tl, s);
+    case 2: return launchFused16M<2>(fv, thr, bpfd, tl, s);
+    case 3: return launchFused16M<3>(fv, thr, bpfd, tl, s);
+    case 4: return launchFused16M<4>(fv, thr, bpfd, tl, s);
+    case 5: return launchFused16M<5>(fv, thr, bpfd, tl, s);
+    case 6: return launchFused16M<6>(fv, thr, bpfd, tl, s);
+    default: return launchFused16M<7>(fv, thr, bpfd, tl, s);
+  }
 }
 
 hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, const Tiling& tl,
@@ -630,6 +863,8 @@ hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long lo
   if (fv.X < 2 || fv.Y < 2 || fv.Z < 2) return hipSuccess;  // no cells
   const int eb = fv.enc & ENC_MASK_NUM;
   if (!tilingSupported(tl, eb)) return hipErrorInvalidValue;
+  if (tl.V == 16 && fv.enc == ENC_F32 && fv.nF == 1 && fv.sx == 1)
+    return launchFused16(fv, (float)m.t[0], m.m, bpfd, tl, s);
   // f32 fields compare against the threshold rounded to float
   // (src/binMapCUDA.cuh:138-139).
   if (fv.enc == ENC_F32) return dispatchFused<float, 4>(fv, (float)m.t[0], m.m, bpfd, tl, s);
@@ -674,6 +909,7 @@ static hipError_t dispatchFusedMomM(const FieldView& fv, T thr, uint32_t m, unsi
 static Tiling fusedMomTiling(const FieldView& fv, const Tiling& tl0) {
   Tiling tl = tl0;
   tl.Ty = 4;
+  if (tl.V == 16) tl.V = 4;  // the 16-voxel kernel has no moments form
   if (fv.nF > 1 || fv.sx != 1) tl.V = 1;
   return tl;
 }
