@@ -183,6 +183,16 @@ FUSED_CASES = [
     ((4, 5, 129), torch.float64, (3, 16, 4, 1)),
     ((6, 9, 1025), torch.float64, None),
     ((17, 19, 333), torch.float64, (4, 16, 8, 2)),
+    # voxelsPerLane 16 (the plain kernel's float default) with moments: the
+    # moments kernel takes such a tiling with 4 voxels per lane and 4 row
+    # pairs; ragged and exact rows, one to three segments.
+    ((3, 4, 1025), torch.float32, (2, 16, 4, 16)),
+    ((3, 4, 1024), torch.float32, (2, 16, 4, 16)),
+    ((5, 6, 1030), torch.float32, (3, 16, 4, 16)),
+    ((4, 5, 2049), torch.float32, (1, 16, 2, 16)),
+    ((7, 9, 300), torch.float32, (3, 16, 4, 16)),
+    ((6, 9, 1000), torch.float32, None),
+    ((9, 6, 2100), torch.float32, (4, 16, 2, 16)),
 ]
 
 
