@@ -224,9 +224,15 @@ int mkfPBMMomentsAsync(BMPackWord* pW, double* pMom, BMOrg* pO, const BMFieldInf
 int mkfBPFDAsync(size_t* pBPFD, const BMOrg* pO, const BMPackWord* pW, const MkfTiling* pT, void* stream) {
   if (!pBPFD || !pO || !pW) return fail(MKF_ERR_ARG, "NULL argument");
   const int X = (int)pO->rowElem, Y = (int)pO->rowPairs + 1, Z = (int)pO->planePairs + 1;
-  // 4 voxels per lane (nibbles of the PBM words), or 1 when asked for.
+  // 16 voxels per lane (half words), 4 (nibbles of the PBM words) or 1, as
+  // asked; 0 takes the built-in choice's (4 where it is 16 and the rows
+  // asked for are 8).
   Tiling tl = chooseBpfdTiling(X, Y, Z);
-  if (pT) tl = Tiling{pT->planePairs, pT->waves, pT->rowPairs, pT->voxelsPerLane == 1 ? 1 : 4};
+  if (pT) {
+    const int v = pT->voxelsPerLane;
+    tl = Tiling{pT->planePairs, pT->waves, pT->rowPairs, v == 1 || v == 16 ? v : v == 0 ? tl.V : 4};
+    if (v == 0 && tl.V == 16 && tl.Ty == 8) tl.V = 4;
+  }
   int rc = checkTiling(tl, 4);
   if (rc) return rc;
   const hipError_t e = launchBPFD(pW, X, Y, Z, pO->rowWS, pO->planeWS, (unsigned long long*)pBPFD, tl,

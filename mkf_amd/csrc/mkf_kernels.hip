@@ -515,6 +515,96 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
 }
 
 // ---------------------------------------------------------------------------
+// Histogram of a PBM, 16 voxels per lane (voxelsPerLane 16). Lane l holds
+// voxels x0 + 16 l .. x0 + 16 l + 15 of a 1024-voxel segment: half of PBM
+// word (x0 >> 5) + (l >> 1). It reads that word and the next one (both
+// clamped to the row) and one funnel shift leaves its 16 voxels and the voxel
+// right of them in bits 0..16. Each group of 4 cells takes a 5-bit window;
+// one multiply spreads it into the group's byte-pair word, as in kFused16.
+// One wave per workgroup, persistent over the tiles: no LDS exchange and no
+// barrier.
+// ---------------------------------------------------------------------------
+template <int TY>
+__global__ __launch_bounds__(kWave) void kBPFD16(const uint32_t* __restrict__ pW, int X, int Y, int Z,
+                                                 uint32_t rowWS, uint32_t planeWS, TileGrid g,
+                                                 unsigned long long* __restrict__ bpfd) {
+  __shared__ uint32_t hist[1][256];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < 256; i += kWave) hist[0][i] = 0;
+  Tally tally;
+  uint32_t cells = 0;
+  for (int t = blockIdx.x; t < g.chunk * 8; t += gridDim.x) {
+    const int L = (t & 7) * g.chunk + (t >> 3);
+    if (L >= g.nTiles) continue;
+    const int xt = L % g.nXT;
+    const int rr = L / g.nXT;
+    const int yr = rr % g.nYR;
+    const int zr = rr / g.nYR;
+    const int x0 = xt * (kWave * kG16);
+    const int y0 = yr * TY;
+    const int nY = min(TY, Y - 1 - y0);
+    const int z0 = zr * g.TZ;
+    const int nZ = min(g.TZ, Z - 1 - z0);
+    // cells that exist: x < X - 1 (cell x0 + 16 lane + 4 q + k is byte k of group q)
+    uint32_t keep[4], nCells = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      keep[q] = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const bool ok = x0 + 16 * lane + 4 * q + k < X - 1;
+        keep[q] |= ok ? (0xFFu << (8 * k)) : 0u;
+        nCells += ok ? 1u : 0u;
+      }
+    }
+    cells += waveSum(nCells) * (uint32_t)nY * (uint32_t)nZ;
+    // The next word is needed only when the lane's last cell exists, and
+    // then it lies inside the row; clamping keeps every read in the row.
+    const uint32_t wlo = min((uint32_t)(x0 >> 5) + (uint32_t)(lane >> 1), rowWS - 1u);
+    const uint32_t whi = min(wlo + 1u, rowWS - 1u);
+    const uint32_t sh = 16u * (uint32_t)(lane & 1);
+    size_t yo[TY + 1];
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) yo[r] = (size_t)(y0 + min(r, nY)) * rowWS;
+    auto loadPlane = [&](size_t zo, uint32_t (&lo)[TY + 1], uint32_t (&hi)[TY + 1]) {
+#pragma unroll
+      for (int r = 0; r <= TY; ++r) {
+        const uint32_t* row = pW + zo + yo[r];
+        lo[r] = row[wlo];
+        hi[r] = row[whi];
+      }
+    };
+    uint32_t lo[TY + 1], hi[TY + 1];
+    loadPlane((size_t)z0 * planeWS, lo, hi);
+    uint32_t Hp[TY][4];
+    for (int z = z0; z <= z0 + nZ; ++z) {
+      uint32_t P[TY + 1][4];
+#pragma unroll
+      for (int r = 0; r <= TY; ++r) {
+        const uint32_t b = __builtin_amdgcn_alignbit(hi[r], lo[r], sh);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) P[r][q] = (((b >> (4 * q)) & 31u) * 0x204081u & 0x03030303u) & keep[q];
+      }
+      if (z < z0 + nZ) loadPlane((size_t)(z + 1) * planeWS, lo, hi);
+#pragma unroll
+      for (int r = 0; r < TY; ++r) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t Hc = P[r][q] | (P[r + 1][q] << 2);
+          if (z > z0 && r < nY) {
+            const uint32_t C = Hp[r][q] | (Hc << 4);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) tally.add((C >> (8 * k)) & 255u, hist[0]);
+          }
+          Hp[r][q] = Hc;
+        }
+      }
+    }
+  }
+  flushHistogram(hist, tally, cells, bpfd);
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 static inline long long ceilDiv(long long a, long long b) { return (a + b - 1) / b; }
@@ -574,8 +664,13 @@ static unsigned fused16Slots(const void* kernel);
 // ceil(tiles / slots) * (TP + 1) wins (ties: the deeper slab). At 1024^3 on
 // MI355X (2048 slots) that is 128 plane pairs (one tile per wave): 0.651 ms;
 // 64: 0.653, 32: 0.666, 96: 0.833 (some waves take two tiles, most one).
-static int fused16Depth(int X, int Y, int Z, int TY) {
-  const long long slots = fused16Slots((const void*)kFused16<6, 4>);
+// Rows of X voxels fill their 1024-voxel segments at least 3/4.
+static bool segmentsFilled(int X) {
+  return X >= 2 && (double)(X - 1) >= 0.75 * (double)(kWave * kG16) * (double)ceilDiv(X - 1, (long long)kWave * kG16);
+}
+
+static int persistentDepth(int X, int Y, int Z, int TY, const void* kernel) {
+  const long long slots = fused16Slots(kernel);
   const long long cols = ceilDiv(X - 1, (long long)kWave * kG16) * ceilDiv(Y - 1, TY);
   int best = 4;
   long long bestCost = -1;
@@ -596,11 +691,11 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   // Float fields whose rows fill the 1024-voxel segments of the 16-voxel
   // kernel at least 3/4 use it (1024^3: 0.651 ms, 6.6 TB/s, against 0.69-0.70
   // for the 4- and 1-voxel tilings below).
-  if (elemBytes == 4 && X >= 2 && (double)(X - 1) >= 0.75 * 1024.0 * ceilDiv(X - 1, (long long)kWave * kG16)) {
+  if (elemBytes == 4 && segmentsFilled(X)) {
     tl.V = 16;
     tl.Ty = 4;
     tl.ZW = 16;
-    tl.TP = fused16Depth(X, Y, Z, 4);
+    tl.TP = persistentDepth(X, Y, Z, 4, (const void*)kFused16<6, 4>);
     return tl;
   }
   if (elemBytes >= 8) {
@@ -650,6 +745,13 @@ Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
 // wave-lanes' worth of work (1024^3: 0.34 ms).
 Tiling chooseBpfdTiling(int X, int Y, int Z) {
   Tiling tl;
+  if (segmentsFilled(X)) {
+    tl.V = 16;
+    tl.Ty = 4;
+    tl.ZW = 16;
+    tl.TP = persistentDepth(X, Y, Z, 4, (const void*)kBPFD16<4>);
+    return tl;
+  }
   tl.V = 4;
   tl.Ty = 4;
   tl.TP = 256;
@@ -811,23 +913,25 @@ static unsigned fused16Slots(const void* kernel) {
   return n;
 }
 
-template <int M>
-static hipError_t launchFused16M(const FieldView& fv, float thr, unsigned long long* bpfd, const Tiling& tl,
-                                 hipStream_t s) {
+// Tiles of the persistent 16-voxel kernels: 1024 voxels x Ty row pairs x
+// TP plane pairs.
+static TileGrid grid16(int X, int Y, int Z, const Tiling& tl) {
   TileGrid g{};
-  g.nSegX = (int)ceilDiv(fv.X - 1, (long long)kWave * kG16);
+  g.nSegX = (int)ceilDiv(X - 1, (long long)kWave * kG16);
   g.waves = 1;
   g.nXT = g.nSegX;
   g.TZ = tl.TP;
-  g.nYR = (int)ceilDiv(fv.Y - 1, tl.Ty);
-  g.nZR = (int)ceilDiv(fv.Z - 1, tl.TP);
+  g.nYR = (int)ceilDiv(Y - 1, tl.Ty);
+  g.nZR = (int)ceilDiv(Z - 1, tl.TP);
   g.nTiles = g.nXT * g.nYR * g.nZR;
   g.chunk = (int)ceilDiv(g.nTiles, 8);
-  if (g.nTiles <= 0) return hipSuccess;
-  // One wave per resident slot (rounded down to a multiple of 8 XCDs), no
-  // more than there are tiles (rounded up to 8).
-  const unsigned slots = fused16Slots(tl.Ty == 2 ? (const void*)kFused16<M, 2> : (const void*)kFused16<M, 4>);
-  unsigned nBlocks = std::min<unsigned>(slots, (unsigned)g.chunk * 8u);
+  return g;
+}
+
+// One wave per resident slot of the kernel (rounded down to a multiple of 8
+// XCDs), no more than there are tiles (rounded up to 8).
+static unsigned blocks16(const TileGrid& g, const Tiling& tl, const void* kernel) {
+  unsigned nBlocks = std::min<unsigned>(fused16Slots(kernel), (unsigned)g.chunk * 8u);
   // A workgroup's histogram and cell count are 32-bit: no workgroup takes
   // more than 2^31 cells (only fields of far more than 2^40 cells need more
   // workgroups than slots for that).
@@ -835,6 +939,16 @@ static hipError_t launchFused16M(const FieldView& fv, float thr, unsigned long l
   const long long maxTiles = std::max(1LL, (1LL << 31) / tileCells);
   const long long need = ceilDiv(ceilDiv((long long)g.chunk * 8, maxTiles), 8) * 8;
   if (need > (long long)nBlocks) nBlocks = (unsigned)std::min<long long>(need, (long long)g.chunk * 8);
+  return nBlocks;
+}
+
+template <int M>
+static hipError_t launchFused16M(const FieldView& fv, float thr, unsigned long long* bpfd, const Tiling& tl,
+                                 hipStream_t s) {
+  const TileGrid g = grid16(fv.X, fv.Y, fv.Z, tl);
+  if (g.nTiles <= 0) return hipSuccess;
+  const unsigned nBlocks =
+      blocks16(g, tl, tl.Ty == 2 ? (const void*)kFused16<M, 2> : (const void*)kFused16<M, 4>);
   const dim3 grid(nBlocks), block(kWave);
   const float* p = (const float*)fv.p[0];
   switch (tl.Ty) {
@@ -945,9 +1059,19 @@ hipError_t launchFusedMoments(const FieldView& fv, const BinMapF64& m, unsigned 
 hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
                       unsigned long long* bpfd, const Tiling& tl0, hipStream_t s) {
   if (X < 2 || Y < 2 || Z < 2) return hipSuccess;
-  // tl.V is 4 (nibbles of the PBM words) or 1.
+  // tl.V is 16 (half words), 4 (nibbles of the PBM words) or 1.
   const Tiling& tl = tl0;
   if (!tilingSupported(tl, 4)) return hipErrorInvalidValue;
+  if (tl.V == 16) {
+    const TileGrid g = grid16(X, Y, Z, tl);
+    if (g.nTiles <= 0) return hipSuccess;
+    const unsigned nBlocks = blocks16(g, tl, tl.Ty == 2 ? (const void*)kBPFD16<2> : (const void*)kBPFD16<4>);
+    if (tl.Ty == 2)
+      hipLaunchKernelGGL((kBPFD16<2>), dim3(nBlocks), dim3(kWave), 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd);
+    else
+      hipLaunchKernelGGL((kBPFD16<4>), dim3(nBlocks), dim3(kWave), 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd);
+    return hipGetLastError();
+  }
   const TileGrid g = makeGrid(X, Y, Z, tl);
   const dim3 grid((unsigned)g.chunk * 8u), block(g.waves * kWave);
   if (tl.V == 4) {

@@ -119,10 +119,17 @@ def test_fused_16_voxels_per_lane(lib, orc, cuda, X, tiling):
     print(f"16 voxels per lane X={X} tiling {tiling}", flush=True)
     f = synth.grf_field((23, 13, X), sigma=2.0, seed=11, device=cuda, dtype=torch.float32)
     f_np = host_np(f)
+    t = M.MkfTiling(*(tiling + (16,)))
     for relop, thr in ((">=", 0.0), ("<", 0.3), ("!", 0.1), ("=", 0.0)):
         _, h_ref = oracle_all(orc, f_np, relop, thr)
         h, _ = dev_fused(lib, f, relop, thr, tiling=tiling + (16,))
         np.testing.assert_array_equal(h, h_ref)
+        # the histogram of the map, 16 voxels per lane
+        _, dw, org, _ = dev_pbm(lib, f, relop, thr)
+        hb = torch.zeros(256, dtype=torch.int64, device=cuda)
+        M.check(lib.mkfBPFDAsync(hb.data_ptr(), C.byref(org), dw.data_ptr(), C.byref(t), None))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(hb.cpu().numpy().view(np.uint64), h_ref)
 
 
 @pytest.mark.parametrize("relop", RELOPS)
@@ -135,9 +142,15 @@ def test_fused_16_special_values(lib, orc, cuda, relop):
     f = torch.from_numpy(rng.choice(choices, size=(5, 6, 2100))).to(cuda)
     for thr in (0.0, 0.25):
         _, h_ref = oracle_all(orc, host_np(f), relop, thr)
+        _, dw, org, _ = dev_pbm(lib, f, relop, thr)
         for ty in (2, 4):
             h, _ = dev_fused(lib, f, relop, thr, tiling=(3, 1, ty, 16))
             np.testing.assert_array_equal(h, h_ref)
+            hb = torch.zeros(256, dtype=torch.int64, device=cuda)
+            t = M.MkfTiling(3, 1, ty, 16)
+            M.check(lib.mkfBPFDAsync(hb.data_ptr(), C.byref(org), dw.data_ptr(), C.byref(t), None))
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(hb.cpu().numpy().view(np.uint64), h_ref)
 
 
 def test_fused_16_padded_unaligned_rows(lib, orc, cuda):
