@@ -624,8 +624,10 @@ static FieldArgs<T> fieldArgs(const FieldView& fv) {
 static TileGrid makeGrid(int X, int Y, int Z, const Tiling& tl) {
   TileGrid g;
   g.nSegX = (int)ceilDiv(X - 1, (long long)kWave * tl.V);
-  g.waves = (int)std::min<long long>(tl.ZW, g.nSegX);
-  g.nXT = (int)ceilDiv(g.nSegX, g.waves);
+  // As few x-tiles as tl.ZW waves allow, with the segments spread evenly
+  // over them (17 segments: 2 tiles of 9 waves, not 16 + 1).
+  g.nXT = (int)ceilDiv(g.nSegX, std::min<long long>(tl.ZW, g.nSegX));
+  g.waves = (int)ceilDiv(g.nSegX, g.nXT);
   g.TZ = tl.TP;
   g.nYR = (int)ceilDiv(Y - 1, tl.Ty);
   g.nZR = (int)ceilDiv(Z - 1, tl.TP);
@@ -658,17 +660,22 @@ bool tilingSupported(const Tiling& tl, int elemBytes) {
 // autotuner times them on the caller's device.
 static unsigned fused16Slots(const void* kernel);
 
-// Slab depth of the persistent 16-voxel kernel: its waves take tiles round
-// robin, so the time goes as the most tiles any wave takes times the planes a
-// tile reads (TP + 1, its first plane read twice). The depth with the lowest
-// ceil(tiles / slots) * (TP + 1) wins (ties: the deeper slab). At 1024^3 on
-// MI355X (2048 slots) that is 128 plane pairs (one tile per wave): 0.651 ms;
-// 64: 0.653, 32: 0.666, 96: 0.833 (some waves take two tiles, most one).
 // Rows of X voxels fill their 1024-voxel segments at least 3/4.
 static bool segmentsFilled(int X) {
   return X >= 2 && (double)(X - 1) >= 0.75 * (double)(kWave * kG16) * (double)ceilDiv(X - 1, (long long)kWave * kG16);
 }
 
+// Rows of X voxels are whole 1024-voxel segments: every segment's voxels
+// x0 .. x0 + 1023 lie in the row (X = 1024 k or 1024 k + 1).
+static bool segmentsWhole(int X) { return X >= kWave * kG16 && (X % (kWave * kG16)) <= 1; }
+
+// Slab depth of the persistent 16-voxel kernels: their waves take tiles
+// round robin, so the time goes as the most tiles any wave takes times the
+// planes a tile reads (TP + 1, its first plane read twice). The depth with
+// the lowest ceil(tiles / slots) * (TP + 1) wins (ties: the deeper slab). At
+// 1024^3 on MI355X (2048 slots for the fused kernel) that is 128 plane pairs
+// (one tile per wave): 0.651 ms; 64: 0.653, 32: 0.666, 96: 0.833 (some waves
+// take two tiles, most one).
 static int persistentDepth(int X, int Y, int Z, int TY, const void* kernel) {
   const long long slots = fused16Slots(kernel);
   const long long cols = ceilDiv(X - 1, (long long)kWave * kG16) * ceilDiv(Y - 1, TY);
@@ -685,13 +692,45 @@ static int persistentDepth(int X, int Y, int Z, int TY, const void* kernel) {
   return best;
 }
 
+// Workgroup width (waves) for rows of nSeg segments. A row of at most 16
+// segments is one workgroup. Longer rows are split into x-tiles of 16, 8 or
+// 4 waves (spread evenly over the tiles they need); the split with the
+// fewest waves wins, each width weighted by how it fills a CU: waves are
+// dealt to 4 SIMDs (a width that is not a multiple of 4 counts 1.25x) and a
+// CU holds 32 waves of this kernel (12 waves: 2 workgroups, 24 of 32). Ties
+// go to the wider. Measured on MI355X (f32, 1 voxel per lane, 1000 x 1000
+// rows of the given segments):
+//   17: 5 x 4 waves 0.99 ms; 3 x 6 1.09; 2 x 9 1.25; 16 + 1 1.48
+//   24: 3 x 8 0.51; 2 x 12 0.57; 6 x 4 0.58
+//   65: 9 x 8 0.737; 17 x 4 0.739; 5 x 13 0.87
+//   9, 11, 16: one workgroup 0.50, 0.58, 0.72 against 0.57, 0.63, 0.85+ split
+static int pickWaves(long long nSeg) {
+  if (nSeg <= 16) return 16;
+  int best = 16;
+  double bestCost = -1;
+  for (int zw : {16, 8, 4}) {
+    const long long nXT = ceilDiv(nSeg, zw);
+    const long long w = ceilDiv(nSeg, nXT);
+    const double perCU = (double)((32 / w) * w);
+    double cost = (double)(nXT * w) * 32.0 / perCU;
+    if (w % 4 != 0) cost *= 1.25;
+    if (bestCost < 0 || cost < bestCost - 1e-9) {
+      bestCost = cost;
+      best = zw;
+    }
+  }
+  return best;
+}
+
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   Tiling tl;
   long long targetWaves;
-  // Float fields whose rows fill the 1024-voxel segments of the 16-voxel
-  // kernel at least 3/4 use it (1024^3: 0.651 ms, 6.6 TB/s, against 0.69-0.70
-  // for the 4- and 1-voxel tilings below).
-  if (elemBytes == 4 && segmentsFilled(X)) {
+  // Float fields whose rows are whole 1024-voxel segments use the 16-voxel
+  // kernel (1024^3: 0.651 ms, 6.6 TB/s, against 0.69-0.70 for the 4- and
+  // 1-voxel tilings below; 2048 x 600 x 600: 0.50 against 0.65 ms). A partial
+  // segment gathers its voxels one by one and is slower than the 1-voxel
+  // tiling (1000^3: 1.12 against 0.78 ms).
+  if (elemBytes == 4 && segmentsWhole(X)) {
     tl.V = 16;
     tl.Ty = 4;
     tl.ZW = 16;
@@ -709,7 +748,7 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
     tl.TP = 128;
     targetWaves = 16384;
   }
-  tl.ZW = 16;
+  tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * tl.V));
   while (tl.TP > 8) {
     const TileGrid g = makeGrid(X, Y, Z, tl);
     if ((long long)g.nTiles * g.waves >= targetWaves) break;
