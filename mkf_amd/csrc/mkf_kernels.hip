@@ -794,7 +794,7 @@ Tiling chooseBpfdTiling(int X, int Y, int Z) {
   tl.V = 4;
   tl.Ty = 4;
   tl.TP = 256;
-  tl.ZW = 16;
+  tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * tl.V));
   while (tl.TP > 8) {
     const TileGrid g = makeGrid(X, Y, Z, tl);
     if ((long long)g.nTiles * g.waves * tl.V >= 256LL * 16 * 4) break;
