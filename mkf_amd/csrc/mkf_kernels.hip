@@ -9,6 +9,8 @@
 //                counterpart; the reference round-trips the PBM through memory)
 #include <map>
 #include <mutex>
+#include <tuple>
+#include <type_traits>
 
 #include "mkf_device.hpp"
 #include "mkf_internal.h"
@@ -400,6 +402,10 @@ __global__ __launch_bounds__(1024) void kBPFD4(const uint32_t* __restrict__ pW, 
 // wave per workgroup: no LDS exchange and no barrier.
 // ---------------------------------------------------------------------------
 constexpr int kG16 = 16;
+// The fused 16-voxel kernel runs at most 2 waves per SIMD (8 per CU): more
+// requests in flight lowered its throughput (1024^3: 3 waves per SIMD
+// 0.66-0.73 ms against 0.65 at 2).
+constexpr int kFused16PerCU = 8;
 
 template <int M, int TY>
 __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, float thr, int X, int Y, int Z,
@@ -446,70 +452,106 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
 #pragma unroll
     for (int r = 0; r <= TY; ++r) yo[r] = (long long)(y0 + min(r, nY)) * sy;
     const float* base = f + x0;
-
-    auto loadPlane = [&](long long zo, float (&v)[TY + 1][kG16], float (&h)[TY + 1]) {
+    // Partial segment: group q is read from min(256 q + 4 lane, X - x0 - 4)
+    // (X >= 4, checked by the host, so the 4 voxels before the row's end
+    // exist), as an unsigned offset from 4 voxels before the segment, which
+    // keeps the loads in the scalar-base + 32-bit-offset form.
+    uint32_t ga[4];
 #pragma unroll
-      for (int r = 0; r <= TY; ++r) {
-        const float* row = base + yo[r] + zo;
-        if (vec) {
+    for (int q = 0; q < 4; ++q) ga[q] = (uint32_t)(min(256 * q + 4 * lane, X - x0 - 4) + 4);
+
+    // The slab walk, compiled once for whole segments (VEC) and once for the
+    // last, partial segment of a row, and chosen per tile, so that neither
+    // carries the other's loads or branches.
+    auto walk = [&](auto vecTag) {
+      constexpr bool VEC = decltype(vecTag)::value;
+      auto loadPlane = [&](long long zo, float (&v)[TY + 1][kG16], float (&h)[TY + 1]) {
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) {
+          const float* row = base + yo[r] + zo;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            const Vec w = (r > 0 && r < TY) ? __builtin_nontemporal_load((GVec)(row + 256 * q + 4 * lane))
-                                            : *(const Vec*)(row + 256 * q + 4 * lane);
+            Vec w;
+            if constexpr (VEC) {
+              w = (r > 0 && r < TY) ? __builtin_nontemporal_load((GVec)(row + 256 * q + 4 * lane))
+                                    : *(const Vec*)(row + 256 * q + 4 * lane);
+            } else {
+              // Partial segment: group q's 16 bytes are read from at most 4
+              // voxels before the row's end (ga[q], counted from 4 voxels
+              // before the segment); the walk shifts them into place when it
+              // uses them, so the loads stay in flight during the tally.
+              const char* rowb = (const char*)(row - 4);
+              const uint32_t byteOff = ga[q] * 4u;
+              w = (r > 0 && r < TY) ? __builtin_nontemporal_load((GVec)(rowb + byteOff))
+                                    : *(const Vec*)(rowb + byteOff);
+            }
 #pragma unroll
             for (int e = 0; e < 4; ++e) v[r][4 * q + e] = w[e];
           }
-        } else {
+          h[r] = halo ? uniformLoad(row + kWave * kG16) : 0.f;
+        }
+      };
+
+      // One plane of registers: the pair words of plane z are computed
+      // first, then plane z+1 is loaded into the same registers while plane
+      // z's cells are tallied.
+      float cur[TY + 1][kG16];
+      float hc[TY + 1];
+      loadPlane((long long)z0 * sz, cur, hc);
+      uint32_t Hp[TY][4];
+      for (int z = z0; z <= z0 + nZ; ++z) {
+        uint32_t P[TY + 1][4];
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) {
+          if constexpr (!VEC) {
+            // Shift each group's voxels back by sh = 0..3 (voxels past the
+            // row's end only meet cells that keep[] drops).
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int sh = 256 * q + 4 * lane + 4 - (int)ga[q];
+              const bool s1 = (sh & 1) != 0, s2 = (sh & 2) != 0;
+              float* g4 = &cur[r][4 * q];
+              const float t0 = s1 ? g4[1] : g4[0], t1 = s1 ? g4[2] : g4[1], t2 = s1 ? g4[3] : g4[2];
+              g4[0] = s2 ? t2 : t0;
+              g4[1] = s2 ? g4[3] : t1;
+              g4[2] = t2;
+            }
+          }
+          uint64_t m[4][4];
 #pragma unroll
           for (int q = 0; q < 4; ++q)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) v[r][4 * q + e] = row[min(256 * q + 4 * lane + e, X - 1 - x0)];
+            for (int e = 0; e < 4; ++e) m[q][e] = relMask<M>(cur[r][4 * q + e], thr);
+          const uint64_t hb = (halo && (relMask<M>(hc[r], thr) & 1ull)) ? 1ull : 0ull;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const uint64_t nextFirst = q < 3 ? (m[q + 1][0] & 1ull) : hb;
+            uint32_t v5 = laneBit((m[q][0] >> 1) | (nextFirst << 63)) ? 1u : 0u;
+#pragma unroll
+            for (int e = 3; e >= 0; --e) v5 = dblAddLaneBit(v5, m[q][e]);
+            P[r][q] = ((v5 * 0x204081u) & 0x03030303u) & keep[q];
+          }
         }
-        h[r] = halo ? uniformLoad(row + kWave * kG16) : 0.f;
+        if (z < z0 + nZ) loadPlane((long long)(z + 1) * sz, cur, hc);
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const uint32_t Hc = P[r][q] | (P[r + 1][q] << 2);
+            if (z > z0 && r < nY) {
+              const uint32_t C = Hp[r][q] | (Hc << 4);
+#pragma unroll
+              for (int k = 0; k < 4; ++k) tally.add((C >> (8 * k)) & 255u, hist[0]);
+            }
+            Hp[r][q] = Hc;
+          }
+        }
       }
     };
-
-    // One plane of registers: the pair words of plane z are computed first,
-    // then plane z+1 is loaded into the same registers while plane z's cells
-    // are tallied.
-    float cur[TY + 1][kG16];
-    float hc[TY + 1];
-    loadPlane((long long)z0 * sz, cur, hc);
-    uint32_t Hp[TY][4];
-    for (int z = z0; z <= z0 + nZ; ++z) {
-      uint32_t P[TY + 1][4];
-#pragma unroll
-      for (int r = 0; r <= TY; ++r) {
-        uint64_t m[4][4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int e = 0; e < 4; ++e) m[q][e] = relMask<M>(cur[r][4 * q + e], thr);
-        const uint64_t hb = (halo && (relMask<M>(hc[r], thr) & 1ull)) ? 1ull : 0ull;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const uint64_t nextFirst = q < 3 ? (m[q + 1][0] & 1ull) : hb;
-          uint32_t v5 = laneBit((m[q][0] >> 1) | (nextFirst << 63)) ? 1u : 0u;
-#pragma unroll
-          for (int e = 3; e >= 0; --e) v5 = dblAddLaneBit(v5, m[q][e]);
-          P[r][q] = ((v5 * 0x204081u) & 0x03030303u) & keep[q];
-        }
-      }
-      if (z < z0 + nZ) loadPlane((long long)(z + 1) * sz, cur, hc);
-#pragma unroll
-      for (int r = 0; r < TY; ++r) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const uint32_t Hc = P[r][q] | (P[r + 1][q] << 2);
-          if (z > z0 && r < nY) {
-            const uint32_t C = Hp[r][q] | (Hc << 4);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) tally.add((C >> (8 * k)) & 255u, hist[0]);
-          }
-          Hp[r][q] = Hc;
-        }
-      }
-    }
+    if (vec)
+      walk(std::true_type{});
+    else
+      walk(std::false_type{});
   }
   flushHistogram(hist, tally, cells, bpfd);
 }
@@ -658,16 +700,12 @@ bool tilingSupported(const Tiling& tl, int elemBytes) {
 //        1 voxel/lane, 8 row pairs, 256 plane pairs   1.293 ms
 // The f32 leaders are within 1 % and swap between machines; the library's
 // autotuner times them on the caller's device.
-static unsigned fused16Slots(const void* kernel);
+static unsigned fused16Slots(const void* kernel, int capPerCU = 0);
 
 // Rows of X voxels fill their 1024-voxel segments at least 3/4.
 static bool segmentsFilled(int X) {
   return X >= 2 && (double)(X - 1) >= 0.75 * (double)(kWave * kG16) * (double)ceilDiv(X - 1, (long long)kWave * kG16);
 }
-
-// Rows of X voxels are whole 1024-voxel segments: every segment's voxels
-// x0 .. x0 + 1023 lie in the row (X = 1024 k or 1024 k + 1).
-static bool segmentsWhole(int X) { return X >= kWave * kG16 && (X % (kWave * kG16)) <= 1; }
 
 // Slab depth of the persistent 16-voxel kernels: their waves take tiles
 // round robin, so the time goes as the most tiles any wave takes times the
@@ -676,8 +714,8 @@ static bool segmentsWhole(int X) { return X >= kWave * kG16 && (X % (kWave * kG1
 // 1024^3 on MI355X (2048 slots for the fused kernel) that is 128 plane pairs
 // (one tile per wave): 0.651 ms; 64: 0.653, 32: 0.666, 96: 0.833 (some waves
 // take two tiles, most one).
-static int persistentDepth(int X, int Y, int Z, int TY, const void* kernel) {
-  const long long slots = fused16Slots(kernel);
+static int persistentDepth(int X, int Y, int Z, int TY, const void* kernel, int capPerCU) {
+  const long long slots = fused16Slots(kernel, capPerCU);
   const long long cols = ceilDiv(X - 1, (long long)kWave * kG16) * ceilDiv(Y - 1, TY);
   int best = 4;
   long long bestCost = -1;
@@ -725,16 +763,16 @@ static int pickWaves(long long nSeg) {
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   Tiling tl;
   long long targetWaves;
-  // Float fields whose rows are whole 1024-voxel segments use the 16-voxel
-  // kernel (1024^3: 0.651 ms, 6.6 TB/s, against 0.69-0.70 for the 4- and
-  // 1-voxel tilings below; 2048 x 600 x 600: 0.50 against 0.65 ms). A partial
-  // segment gathers its voxels one by one and is slower than the 1-voxel
-  // tiling (1000^3: 1.12 against 0.78 ms).
-  if (elemBytes == 4 && segmentsWhole(X)) {
+  // Float fields whose rows fill their 1024-voxel segments at least 3/4 use
+  // the 16-voxel kernel. Against the 1-voxel tiling below: 1024^3 0.651 ms
+  // (6.6 TB/s) against 0.72; 1000^3 0.644 against 0.78; 800 x 1000 x 1000
+  // 0.563 against 0.753; 2000 x 600 x 600 0.496 against 0.646. At half
+  // filled segments it loses (1030 x 1000 x 1000: 1.15 against 1.05).
+  if (elemBytes == 4 && segmentsFilled(X)) {
     tl.V = 16;
     tl.Ty = 4;
     tl.ZW = 16;
-    tl.TP = persistentDepth(X, Y, Z, 4, (const void*)kFused16<6, 4>);
+    tl.TP = persistentDepth(X, Y, Z, 4, (const void*)kFused16<6, 4>, kFused16PerCU);
     return tl;
   }
   if (elemBytes >= 8) {
@@ -788,7 +826,7 @@ Tiling chooseBpfdTiling(int X, int Y, int Z) {
     tl.V = 16;
     tl.Ty = 4;
     tl.ZW = 16;
-    tl.TP = persistentDepth(X, Y, Z, 4, (const void*)kBPFD16<4>);
+    tl.TP = persistentDepth(X, Y, Z, 4, (const void*)kBPFD16<4>, 0);
     return tl;
   }
   tl.V = 4;
@@ -936,19 +974,20 @@ static hipError_t dispatchFused(const FieldView& fv, T thr, uint32_t m, unsigned
 
 // Resident one-wave workgroups of a kernel on the current device, a multiple
 // of 8 (computed once per kernel and device).
-static unsigned fused16Slots(const void* kernel) {
+static unsigned fused16Slots(const void* kernel, int capPerCU) {
   static std::mutex mu;
-  static std::map<std::pair<const void*, int>, unsigned> cache;
+  static std::map<std::tuple<const void*, int, int>, unsigned> cache;
   int dev = 0;
   (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lk(mu);
-  auto it = cache.find({kernel, dev});
+  auto it = cache.find({kernel, dev, capPerCU});
   if (it != cache.end()) return it->second;
   int perCU = 0, cus = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, kWave, 0) != hipSuccess || perCU < 1) perCU = 1;
+  if (capPerCU > 0) perCU = std::min(perCU, capPerCU);
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 8;
   const unsigned n = std::max(8u, ((unsigned)(perCU * cus)) / 8u * 8u);
-  cache[{kernel, dev}] = n;
+  cache[{kernel, dev, capPerCU}] = n;
   return n;
 }
 
@@ -969,8 +1008,8 @@ static TileGrid grid16(int X, int Y, int Z, const Tiling& tl) {
 
 // One wave per resident slot of the kernel (rounded down to a multiple of 8
 // XCDs), no more than there are tiles (rounded up to 8).
-static unsigned blocks16(const TileGrid& g, const Tiling& tl, const void* kernel) {
-  unsigned nBlocks = std::min<unsigned>(fused16Slots(kernel), (unsigned)g.chunk * 8u);
+static unsigned blocks16(const TileGrid& g, const Tiling& tl, const void* kernel, int capPerCU) {
+  unsigned nBlocks = std::min<unsigned>(fused16Slots(kernel, capPerCU), (unsigned)g.chunk * 8u);
   // A workgroup's histogram and cell count are 32-bit: no workgroup takes
   // more than 2^31 cells (only fields of far more than 2^40 cells need more
   // workgroups than slots for that).
@@ -987,7 +1026,7 @@ static hipError_t launchFused16M(const FieldView& fv, float thr, unsigned long l
   const TileGrid g = grid16(fv.X, fv.Y, fv.Z, tl);
   if (g.nTiles <= 0) return hipSuccess;
   const unsigned nBlocks =
-      blocks16(g, tl, tl.Ty == 2 ? (const void*)kFused16<M, 2> : (const void*)kFused16<M, 4>);
+      blocks16(g, tl, tl.Ty == 2 ? (const void*)kFused16<M, 2> : (const void*)kFused16<M, 4>, kFused16PerCU);
   const dim3 grid(nBlocks), block(kWave);
   const float* p = (const float*)fv.p[0];
   switch (tl.Ty) {
@@ -1016,7 +1055,7 @@ hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long lo
   if (fv.X < 2 || fv.Y < 2 || fv.Z < 2) return hipSuccess;  // no cells
   const int eb = fv.enc & ENC_MASK_NUM;
   if (!tilingSupported(tl, eb)) return hipErrorInvalidValue;
-  if (tl.V == 16 && fv.enc == ENC_F32 && fv.nF == 1 && fv.sx == 1)
+  if (tl.V == 16 && fv.enc == ENC_F32 && fv.nF == 1 && fv.sx == 1 && fv.X >= 4)
     return launchFused16(fv, (float)m.t[0], m.m, bpfd, tl, s);
   // f32 fields compare against the threshold rounded to float
   // (src/binMapCUDA.cuh:138-139).
@@ -1104,7 +1143,7 @@ hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, u
   if (tl.V == 16) {
     const TileGrid g = grid16(X, Y, Z, tl);
     if (g.nTiles <= 0) return hipSuccess;
-    const unsigned nBlocks = blocks16(g, tl, tl.Ty == 2 ? (const void*)kBPFD16<2> : (const void*)kBPFD16<4>);
+    const unsigned nBlocks = blocks16(g, tl, tl.Ty == 2 ? (const void*)kBPFD16<2> : (const void*)kBPFD16<4>, 0);
     if (tl.Ty == 2)
       hipLaunchKernelGGL((kBPFD16<2>), dim3(nBlocks), dim3(kWave), 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd);
     else
