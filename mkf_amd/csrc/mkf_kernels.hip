@@ -700,7 +700,7 @@ bool tilingSupported(const Tiling& tl, int elemBytes) {
 //        1 voxel/lane, 8 row pairs, 256 plane pairs   1.293 ms
 // The f32 leaders are within 1 % and swap between machines; the library's
 // autotuner times them on the caller's device.
-static unsigned fused16Slots(const void* kernel, int capPerCU = 0);
+static unsigned persistentSlots(const void* kernel, int capPerCU = 0);
 
 // Rows of X voxels fill their 1024-voxel segments at least 3/4.
 static bool segmentsFilled(int X) {
@@ -715,7 +715,7 @@ static bool segmentsFilled(int X) {
 // (one tile per wave): 0.651 ms; 64: 0.653, 32: 0.666, 96: 0.833 (some waves
 // take two tiles, most one).
 static int persistentDepth(int X, int Y, int Z, int TY, const void* kernel, int capPerCU) {
-  const long long slots = fused16Slots(kernel, capPerCU);
+  const long long slots = persistentSlots(kernel, capPerCU);
   const long long cols = ceilDiv(X - 1, (long long)kWave * kG16) * ceilDiv(Y - 1, TY);
   int best = 4;
   long long bestCost = -1;
@@ -972,9 +972,10 @@ static hipError_t dispatchFused(const FieldView& fv, T thr, uint32_t m, unsigned
   return dispatchFusedTY<T, VW>(fv, thr, m, bpfd, g, tl.Ty, s);
 }
 
-// Resident one-wave workgroups of a kernel on the current device, a multiple
-// of 8 (computed once per kernel and device).
-static unsigned fused16Slots(const void* kernel, int capPerCU) {
+// Resident one-wave workgroups of a kernel on the current device, at most
+// capPerCU per CU when capPerCU > 0, rounded down to a multiple of 8
+// (computed once per kernel, device and cap).
+static unsigned persistentSlots(const void* kernel, int capPerCU) {
   static std::mutex mu;
   static std::map<std::tuple<const void*, int, int>, unsigned> cache;
   int dev = 0;
@@ -1009,7 +1010,7 @@ static TileGrid grid16(int X, int Y, int Z, const Tiling& tl) {
 // One wave per resident slot of the kernel (rounded down to a multiple of 8
 // XCDs), no more than there are tiles (rounded up to 8).
 static unsigned blocks16(const TileGrid& g, const Tiling& tl, const void* kernel, int capPerCU) {
-  unsigned nBlocks = std::min<unsigned>(fused16Slots(kernel, capPerCU), (unsigned)g.chunk * 8u);
+  unsigned nBlocks = std::min<unsigned>(persistentSlots(kernel, capPerCU), (unsigned)g.chunk * 8u);
   // A workgroup's histogram and cell count are 32-bit: no workgroup takes
   // more than 2^31 cells (only fields of far more than 2^40 cells need more
   // workgroups than slots for that).
