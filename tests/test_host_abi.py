@@ -240,3 +240,23 @@ def test_volFrac(lib):
     h[1::2] = 3
     h[0::2] = 1
     assert lib.volFrac(h.ctypes.data_as(C.POINTER(C.c_size_t))) == np.float32(384) / np.float32(512)
+
+
+@pytest.mark.parametrize("dims,eb,want", [
+    ((1024, 1024, 1024), 4, (16, 4, None)),  # whole 1024-voxel segments: 16 voxels per lane
+    ((1000, 1000, 1000), 4, (16, 4, None)),  # rows fill 98 % of their segment
+    ((769, 20, 20), 4, (16, 4, None)),       # 75 %
+    ((700, 20, 20), 4, (1, 4, 16)),          # 68 %: 1 voxel per lane, 11 segments in one workgroup
+    ((1030, 100, 100), 4, (1, 4, 4)),        # 17 segments: 5 x-tiles of 4 waves
+    ((1536, 100, 100), 4, (1, 4, 8)),        # 24 segments: 3 x-tiles of 8 waves
+    ((1024, 1024, 1024), 8, (2, 4, 16)),     # doubles: 2 voxels per lane
+])
+def test_default_tiling_choice(lib, dims, eb, want):
+    """The built-in fused tiling (host logic only; the slab depth depends on
+    the device, so only its range is checked)."""
+    t = lib.mkfDefaultTiling(M.dims(dims), eb)
+    v, ty, waves = want
+    assert t.voxelsPerLane == v and t.rowPairs == ty
+    if waves is not None:
+        assert t.waves == waves
+    assert 1 <= t.planePairs <= 65536
