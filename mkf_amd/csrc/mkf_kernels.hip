@@ -702,9 +702,9 @@ bool tilingSupported(const Tiling& tl, int elemBytes) {
 // autotuner times them on the caller's device.
 static unsigned persistentSlots(const void* kernel, int capPerCU = 0);
 
-// Rows of X voxels fill their 1024-voxel segments at least 3/4.
-static bool segmentsFilled(int X) {
-  return X >= 2 && (double)(X - 1) >= 0.75 * (double)(kWave * kG16) * (double)ceilDiv(X - 1, (long long)kWave * kG16);
+// Rows of X voxels fill their 1024-voxel segments at least to frac.
+static bool segmentsFilled(int X, double frac) {
+  return X >= 2 && (double)(X - 1) >= frac * (double)(kWave * kG16) * (double)ceilDiv(X - 1, (long long)kWave * kG16);
 }
 
 // Slab depth of the persistent 16-voxel kernels: their waves take tiles
@@ -763,12 +763,13 @@ static int pickWaves(long long nSeg) {
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   Tiling tl;
   long long targetWaves;
-  // Float fields whose rows fill their 1024-voxel segments at least 3/4 use
-  // the 16-voxel kernel. Against the 1-voxel tiling below: 1024^3 0.651 ms
-  // (6.6 TB/s) against 0.72; 1000^3 0.644 against 0.78; 800 x 1000 x 1000
-  // 0.563 against 0.753; 2000 x 600 x 600 0.496 against 0.646. At half
-  // filled segments it loses (1030 x 1000 x 1000: 1.15 against 1.05).
-  if (elemBytes == 4 && segmentsFilled(X)) {
+  // Float fields whose rows fill their 1024-voxel segments at least 60 % use
+  // the 16-voxel kernel. Against the 1-voxel tiling below (x 1000 x 1000):
+  // X = 1024 0.651 ms (6.6 TB/s) against 0.72; 1000: 0.644 against 0.78;
+  // 800: 0.563 against 0.753; 700: 0.548 against 0.569; 620: 0.551 against
+  // 0.585; 1400: 1.14 against 1.19. Below that it loses: 576 (56 %) 0.548
+  // against 0.510, 1030 (50 %) 1.15 against 1.05.
+  if (elemBytes == 4 && segmentsFilled(X, 0.6)) {
     tl.V = 16;
     tl.Ty = 4;
     tl.ZW = 16;
@@ -822,7 +823,9 @@ Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
 // wave-lanes' worth of work (1024^3: 0.34 ms).
 Tiling chooseBpfdTiling(int X, int Y, int Z) {
   Tiling tl;
-  if (segmentsFilled(X)) {
+  // The PBM histogram is bound by instruction issue, so lanes past the row
+  // end cost in full: the 16-voxel kernel from 3/4 filled segments.
+  if (segmentsFilled(X, 0.75)) {
     tl.V = 16;
     tl.Ty = 4;
     tl.ZW = 16;

@@ -246,9 +246,11 @@ def test_volFrac(lib):
     ((1024, 1024, 1024), 4, (16, 4, None)),  # whole 1024-voxel segments: 16 voxels per lane
     ((1000, 1000, 1000), 4, (16, 4, None)),  # rows fill 98 % of their segment
     ((769, 20, 20), 4, (16, 4, None)),       # 75 %
-    ((700, 20, 20), 4, (1, 4, 16)),          # 68 %: 1 voxel per lane, 11 segments in one workgroup
+    ((620, 20, 20), 4, (16, 4, None)),       # 60.5 %
+    ((576, 20, 20), 4, (1, 4, 16)),          # 56 %: 1 voxel per lane, 9 segments in one workgroup
     ((1030, 100, 100), 4, (1, 4, 4)),        # 17 segments: 5 x-tiles of 4 waves
-    ((1536, 100, 100), 4, (1, 4, 8)),        # 24 segments: 3 x-tiles of 8 waves
+    ((1536, 100, 100), 4, (16, 4, None)),    # a whole and a half 1024-voxel segment (75 %)
+    ((1200, 100, 100), 4, (1, 4, 4)),        # 59 %: 19 segments as 5 x-tiles of 4 waves
     ((1024, 1024, 1024), 8, (2, 4, 16)),     # doubles: 2 voxels per lane
 ])
 def test_default_tiling_choice(lib, dims, eb, want):
