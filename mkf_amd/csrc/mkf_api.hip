@@ -307,8 +307,10 @@ int mkfFusedAutotune(const BMFieldInfo* pI, const BinMapF64* pM, MkfTiling* pCho
   const int eb = fv.enc & ENC_MASK_NUM;
   // The built-in choice and the shapes that came out ahead of it on some
   // MI355X at 1024^3; several fields or an x stride run one voxel per lane.
+  // Float rows near the 16-voxel kernel's fill threshold get both kernels.
   std::vector<Tiling> cand{chooseTiling(fv.X, fv.Y, fv.Z, eb)};
   if (eb == 4) {
+    if (cand[0].V != 16 && fv.X >= 4 && fv.nF == 1 && fv.sx == 1) cand.push_back(choose16Tiling(fv.X, fv.Y, fv.Z));
     cand.push_back(Tiling{256, 16, 4, 4});
     cand.push_back(Tiling{128, 16, 4, 1});
   } else {

@@ -760,6 +760,17 @@ static int pickWaves(long long nSeg) {
   return best;
 }
 
+// The 16-voxel fused kernel with 4 row pairs and the slab depth of the cost
+// model (any float field with X >= 4).
+Tiling choose16Tiling(int X, int Y, int Z) {
+  Tiling tl;
+  tl.V = 16;
+  tl.Ty = 4;
+  tl.ZW = 16;
+  tl.TP = persistentDepth(X, Y, Z, 4, (const void*)kFused16<6, 4>, kFused16PerCU);
+  return tl;
+}
+
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   Tiling tl;
   long long targetWaves;
@@ -769,13 +780,7 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   // 800: 0.563 against 0.753; 700: 0.548 against 0.569; 620: 0.551 against
   // 0.585; 1400: 1.14 against 1.19. Below that it loses: 576 (56 %) 0.548
   // against 0.510, 1030 (50 %) 1.15 against 1.05.
-  if (elemBytes == 4 && segmentsFilled(X, 0.6)) {
-    tl.V = 16;
-    tl.Ty = 4;
-    tl.ZW = 16;
-    tl.TP = persistentDepth(X, Y, Z, 4, (const void*)kFused16<6, 4>, kFused16PerCU);
-    return tl;
-  }
+  if (elemBytes == 4 && segmentsFilled(X, 0.6)) return choose16Tiling(X, Y, Z);
   if (elemBytes >= 8) {
     tl.V = 2;
     tl.Ty = 4;
