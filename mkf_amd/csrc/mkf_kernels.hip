@@ -89,69 +89,118 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   const bool writer = pW != nullptr && jl < NQ && word < rowWS;
   const uint32_t sh = (uint32_t)(E * jl);
   MomAcc acc;
-  for (long long i = 0; i < n; i += kPbmAhead) {
-    T v[kPbmAhead][V];
-    size_t wo[kPbmAhead];
+  // Partial segment (not the general form): load q of a row is read from
+  // min(64 E q + E lane, X - x0 - E), so it ends inside the row (the host
+  // sends rows shorter than E voxels to the general form), as an unsigned
+  // offset from E voxels before the segment; shq[q] (0..E-1) shifts the
+  // voxels back into place where they are used.
+  uint32_t offq[NQ];
+  int shq[NQ];
 #pragma unroll
-    for (int u = 0; u < kPbmAhead; ++u) {
-      if (i + u < n) {
-        // Several fields are summed in table order (general form only).
-        const long long ro = (long long)y * fa.sy + z * fa.sz;
-        for (int fi = 0; fi < (G ? fa.nF : 1); ++fi) {
-          const T* row = fa.p[fi] + ro;
-          T t[V];
-          if (vec) {
+  for (int q = 0; q < NQ; ++q) {
+    const int start = kWave * E * q + E * lane;
+    offq[q] = (uint32_t)(min(start, X - x0 - E) + E);
+    shq[q] = start + E - (int)offq[q];
+  }
+  // The row loop, compiled once for whole segments (VEC) and once for the
+  // partial or general ones, and chosen per wave, so that neither carries
+  // the other's loads or branches.
+  auto walk = [&](auto vecTag) {
+    constexpr bool VEC = decltype(vecTag)::value;
+    for (long long i = 0; i < n; i += kPbmAhead) {
+      T v[kPbmAhead][V];
+      size_t wo[kPbmAhead];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              const Vec w4 = __builtin_nontemporal_load((GVec)(row + x0 + kWave * E * q + E * lane));
+      for (int u = 0; u < kPbmAhead; ++u) {
+        if (i + u < n) {
+          // Several fields are summed in table order (general form only).
+          const long long ro = (long long)y * fa.sy + z * fa.sz;
+          for (int fi = 0; fi < (G ? fa.nF : 1); ++fi) {
+            const T* row = fa.p[fi] + ro;
+            T t[V];
+            if constexpr (VEC) {
 #pragma unroll
-              for (int e = 0; e < E; ++e) t[E * q + e] = w4[e];
+              for (int q = 0; q < NQ; ++q) {
+                const Vec w4 = __builtin_nontemporal_load((GVec)(row + x0 + kWave * E * q + E * lane));
+#pragma unroll
+                for (int e = 0; e < E; ++e) t[E * q + e] = w4[e];
+              }
+            } else if constexpr (G) {
+#pragma unroll
+              for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                  const int x = min(x0 + kWave * E * q + E * lane + e, X - 1);
+                  t[E * q + e] = row[(long long)x * fa.sx];
+                }
+              }
+            } else {
+              const char* rb = (const char*)(row + x0 - E);
+#pragma unroll
+              for (int q = 0; q < NQ; ++q) {
+                const Vec w4 = __builtin_nontemporal_load((GVec)(rb + offq[q] * (uint32_t)sizeof(T)));
+#pragma unroll
+                for (int e = 0; e < E; ++e) t[E * q + e] = w4[e];
+              }
             }
-          } else {
+#pragma unroll
+            for (int k = 0; k < V; ++k) v[u][k] = fi ? v[u][k] + t[k] : t[k];
+          }
+          wo[u] = (size_t)y * rowWS + (size_t)z * planeWS + word;
+          if (++y == Y) {
+            y = 0;
+            ++z;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kPbmAhead; ++u) {
+        if (i + u < n) {
+          if constexpr (!VEC && !G) {
+            // Shift each load's voxels back by shq (voxels past the row's
+            // end are cleared by kq and owned by nobody).
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-#pragma unroll
-              for (int e = 0; e < E; ++e) {
-                const int x = min(x0 + kWave * E * q + E * lane + e, X - 1);
-                t[E * q + e] = row[G ? (long long)x * fa.sx : x];
+              const bool s1 = (shq[q] & 1) != 0;
+              if constexpr (E == 4) {
+                const bool s2 = (shq[q] & 2) != 0;
+                const T a0 = v[u][4 * q], a1 = v[u][4 * q + 1], a2 = v[u][4 * q + 2], a3 = v[u][4 * q + 3];
+                const T t0 = s1 ? a1 : a0, t1 = s1 ? a2 : a1, t2 = s1 ? a3 : a2;
+                v[u][4 * q] = s2 ? t2 : t0;
+                v[u][4 * q + 1] = s2 ? a3 : t1;
+                v[u][4 * q + 2] = t2;
+              } else {
+                v[u][2 * q] = s1 ? v[u][2 * q + 1] : v[u][2 * q];
               }
             }
           }
+          uint32_t wq[NQ];
 #pragma unroll
-          for (int k = 0; k < V; ++k) v[u][k] = fi ? v[u][k] + t[k] : t[k];
-        }
-        wo[u] = (size_t)y * rowWS + (size_t)z * planeWS + word;
-        if (++y == Y) {
-          y = 0;
-          ++z;
-        }
-      }
-    }
+          for (int q = 0; q < NQ; ++q) {
+            uint32_t nib = laneBit(relMask<M>(v[u][E * q + E - 1], thr)) ? 1u : 0u;
 #pragma unroll
-    for (int u = 0; u < kPbmAhead; ++u) {
-      if (i + u < n) {
-        uint32_t wq[NQ];
+            for (int e = E - 2; e >= 0; --e) nib = dblAddLaneBit(nib, relMask<M>(v[u][E * q + e], thr));
+            nib &= kq[q];
+            uint32_t pw = orHalfMirror(orQuadXor2(orQuadXor1(nib << sh)));
+            if constexpr (LPW == 16) pw = orRowMirror(pw);
+            wq[q] = pw;
+            if constexpr (MOM) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          uint32_t nib = laneBit(relMask<M>(v[u][E * q + E - 1], thr)) ? 1u : 0u;
-#pragma unroll
-          for (int e = E - 2; e >= 0; --e) nib = dblAddLaneBit(nib, relMask<M>(v[u][E * q + e], thr));
-          nib &= kq[q];
-          uint32_t pw = orHalfMirror(orQuadXor2(orQuadXor1(nib << sh)));
-          if constexpr (LPW == 16) pw = orRowMirror(pw);
-          wq[q] = pw;
-          if constexpr (MOM) {
-#pragma unroll
-            for (int e = 0; e < E; ++e) acc.add(v[u][E * q + e], ((nib >> e) & 1u) != 0, ((kq[q] >> e) & 1u) != 0);
+              for (int e = 0; e < E; ++e) acc.add(v[u][E * q + e], ((nib >> e) & 1u) != 0, ((kq[q] >> e) & 1u) != 0);
+            }
           }
-        }
-        uint32_t w32 = wq[0];
+          uint32_t w32 = wq[0];
 #pragma unroll
-        for (int q = 1; q < NQ; ++q) w32 = (jl == q) ? wq[q] : w32;
-        if (writer) pW[wo[u]] = w32;
+          for (int q = 1; q < NQ; ++q) w32 = (jl == q) ? wq[q] : w32;
+          if (writer) pW[wo[u]] = w32;
+        }
       }
     }
-  }
+  };
+  if (vec)
+    walk(std::true_type{});
+  else
+    walk(std::false_type{});
   if constexpr (MOM) momBlockPartial(acc, part);
 }
 
@@ -873,7 +922,8 @@ template <typename T, int M, bool MOM>
 static hipError_t launchPBMT(const FieldView& fv, T thr, uint32_t* pW, uint32_t rowWS, uint32_t planeWS,
                              double* part, hipStream_t s) {
   const PbmGrid g = pbmGrid(fv);
-  if (fv.nF > 1 || fv.sx != 1)
+  // The general form also takes rows shorter than one 16-byte load.
+  if (fv.nF > 1 || fv.sx != 1 || fv.X < (int)(16 / sizeof(T)))
     hipLaunchKernelGGL((kPBM<T, M, true, MOM>), dim3((unsigned)g.nBlocks), dim3(kPbmWaves * kWave), 0, s,
                        fieldArgs<T>(fv), thr, fv.X, fv.Y, g.nSegX, g.nRows, g.rowsPerWave, pW, rowWS, planeWS, part);
   else
