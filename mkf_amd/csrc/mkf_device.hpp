@@ -224,7 +224,7 @@ struct FieldArgs {
 // plain form compiles to the single-field, unit-stride loads only. The bit
 // right of the wave's last voxel comes from the next wave through LDS
 // (walker), or from memory when that wave belongs to another workgroup.
-template <typename T, int M, int V, bool G = false>
+template <typename T, int M, int V, bool G = false, bool PART = false>
 struct FieldRows {
   static constexpr bool kSelfHalo = false;
   static constexpr int kM = M;
@@ -238,6 +238,14 @@ struct FieldRows {
   bool vec;          // unit x stride and every voxel of every lane exists
   int hxo;           // offset (in voxels) from xb of the next wave's first voxel, clamped
   uint32_t keep;     // pair-word bytes of cells that exist (per lane)
+  // PART (partial waves: unit stride, V > 1, some voxels past the row's
+  // end): the lane's V voxels are read by one vector load from vs (offset
+  // from xb, ending inside the row; X >= V is checked by the host) and
+  // shifted back by shv (0..V-1 for lanes whose first voxel is in the row)
+  // where they are used. The kernel compiles the walk once per form, so
+  // whole waves carry no partial-wave code.
+  int vs;
+  uint32_t shv;
 
   struct Raw {
     T v[V];
@@ -246,7 +254,7 @@ struct FieldRows {
   template <bool NT = false>
   __device__ __forceinline__ Raw loadOne(const T* row) const {
     Raw r;
-    if (!G && vec) {
+    if (!G && !PART && vec) {
       typedef typename VecLoad<T, V>::type VT;
       typedef const __attribute__((address_space(1))) VT* GVT;
       VT x;
@@ -260,9 +268,39 @@ struct FieldRows {
 #pragma unroll
         for (int k = 0; k < V; ++k) r.v[k] = x[k];
       }
+    } else if constexpr (PART) {
+      typedef typename VecLoad<T, V>::type VT;
+      typedef const __attribute__((address_space(1))) VT* GVT;
+      VT x;
+      if constexpr (NT)
+        x = __builtin_nontemporal_load((GVT)(row + vs));
+      else
+        x = *(const VT*)(row + vs);
+#pragma unroll
+      for (int k = 0; k < V; ++k) r.v[k] = x[k];
     } else {
 #pragma unroll
       for (int k = 0; k < V; ++k) r.v[k] = row[G ? (long long)min((int)lx + k, last) * sx : min((int)lx + k, last)];
+    }
+    return r;
+  }
+  // The lane's voxels in place: partial waves shift the vector back by shv.
+  __device__ __forceinline__ Raw placed(const Raw& r) const {
+    if constexpr (PART) {
+      {
+        Raw o = r;
+        const bool s1 = (shv & 1u) != 0;
+        if constexpr (V == 4) {
+          const bool s2 = (shv & 2u) != 0;
+          const T t0 = s1 ? r.v[1] : r.v[0], t1 = s1 ? r.v[2] : r.v[1], t2 = s1 ? r.v[3] : r.v[2];
+          o.v[0] = s2 ? t2 : t0;
+          o.v[1] = s2 ? r.v[3] : t1;
+          o.v[2] = t2;
+        } else {
+          o.v[0] = s1 ? r.v[1] : r.v[0];
+        }
+        return o;
+      }
     }
     return r;
   }
@@ -286,7 +324,8 @@ struct FieldRows {
   // mask of voxel 0 (the right neighbours of the lanes' last voxels). Bits of
   // voxels that do not exist (clamped loads) only reach cells that do not
   // exist, whose bytes `keep` clears.
-  __device__ __forceinline__ uint32_t pairs(const Raw& r, uint64_t& R0) const {
+  __device__ __forceinline__ uint32_t pairs(const Raw& r0, uint64_t& R0) const {
+    const Raw r = placed(r0);
     uint32_t b[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) {
@@ -314,7 +353,8 @@ struct FieldRows {
   }
   // Phase moments of the lane's voxels selected by own (bit k: voxel k).
   template <class Acc>
-  __device__ __forceinline__ void moments(const Raw& r, uint32_t own, Acc& acc) const {
+  __device__ __forceinline__ void moments(const Raw& r0, uint32_t own, Acc& acc) const {
+    const Raw r = placed(r0);
 #pragma unroll
     for (int k = 0; k < V; ++k) acc.add(r.v[k], laneBit(relMask<M>(r.v[k], thr)), ((own >> k) & 1u) != 0);
   }

@@ -342,42 +342,62 @@ __global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr,
   for (int i = lane; i < 256; i += kWave) hist[q.w][i] = 0;
 
   const int xb = min(q.x0, X - 1);
-  FieldRows<T, M, V, G> src;
-  if constexpr (G) {
-#pragma unroll
-    for (int i = 0; i < kMaxFieldsDev; ++i) src.fw[i] = fa.p[i] + (long long)xb * fa.sx;
-    src.nF = fa.nF;
-    src.sx = fa.sx;
-  } else {
-    src.fw[0] = fa.p[0] + xb;  // one field, unit x stride
-    src.nF = 1;
-    src.sx = 1;
-  }
-  src.thr = thr;
-  src.sy = fa.sy;
-  src.sz = fa.sz;
-  src.lx = (uint32_t)(min(q.x0 + V * lane, X - 1) - xb);
-  src.last = X - 1 - xb;
-  src.vec = !G && (V == 1 || q.x0 + q.W <= X);
-  src.hxo = min(q.x0 + q.W, X - 1) - xb;
+  const bool vec = !G && (V == 1 || q.x0 + q.W <= X);
   uint32_t cellsPerRow;
-  src.keep = cellKeep<V>(q.x0, X, cellsPerRow);
-  Tally tally;
-  if constexpr (MOM) {
-    OnMom mom;
-    mom.vkeep = 0;
+  const uint32_t keep = cellKeep<V>(q.x0, X, cellsPerRow);
+  // The walk, compiled for whole waves (and the general form) and, for the
+  // moments forms and 4 voxels per lane, once more for partial waves; each
+  // wave runs one, and both pass the same barriers. (The plain 2-voxel double
+  // kernel keeps one walk: the second copy cost it 4 % at 1024^3, and double
+  // rows with a partial segment take 1 voxel per lane instead.)
+  auto run = [&](auto partTag) {
+    constexpr bool PART = decltype(partTag)::value;
+    FieldRows<T, M, V, G, PART> src;
+    if constexpr (G) {
 #pragma unroll
-    for (int k = 0; k < V; ++k) mom.vkeep |= (q.x0 + V * lane + k < X) ? (1u << k) : 0u;
-    mom.lastRow = q.y0 + q.nY == Y - 1;
-    mom.lastPlane = q.z0 + q.nZ == Z - 1;
-    mom.haloLast = q.halo == kHaloMem && q.x0 + q.W == X - 1;
-    walkStack<TY, V, FieldRows<T, M, V, G>, OnMom>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist[q.w], tally,
-                                                   &mom);
-    flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
-    momBlockPartial(mom.acc, part);
+      for (int i = 0; i < kMaxFieldsDev; ++i) src.fw[i] = fa.p[i] + (long long)xb * fa.sx;
+      src.nF = fa.nF;
+      src.sx = fa.sx;
+    } else {
+      src.fw[0] = fa.p[0] + xb;  // one field, unit x stride
+      src.nF = 1;
+      src.sx = 1;
+    }
+    src.thr = thr;
+    src.sy = fa.sy;
+    src.sz = fa.sz;
+    src.lx = (uint32_t)(min(q.x0 + V * lane, X - 1) - xb);
+    src.last = X - 1 - xb;
+    src.vec = vec;
+    src.hxo = min(q.x0 + q.W, X - 1) - xb;
+    src.vs = min(q.x0 + V * lane, X - V) - xb;
+    src.shv = (uint32_t)(q.x0 + V * lane - min(q.x0 + V * lane, X - V));
+    src.keep = keep;
+    Tally tally;
+    if constexpr (MOM) {
+      OnMom mom;
+      mom.vkeep = 0;
+#pragma unroll
+      for (int k = 0; k < V; ++k) mom.vkeep |= (q.x0 + V * lane + k < X) ? (1u << k) : 0u;
+      mom.lastRow = q.y0 + q.nY == Y - 1;
+      mom.lastPlane = q.z0 + q.nZ == Z - 1;
+      mom.haloLast = q.halo == kHaloMem && q.x0 + q.W == X - 1;
+      walkStack<TY, V, FieldRows<T, M, V, G, PART>, OnMom>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist[q.w],
+                                                           tally, &mom);
+      flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
+      momBlockPartial(mom.acc, part);
+    } else {
+      walkStack<TY, V>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist[q.w], tally);
+      flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
+    }
+  };
+  if constexpr (!G && V > 1 && (MOM || V == 4)) {
+    if (vec)
+      run(std::false_type{});
+    else
+      run(std::true_type{});
   } else {
-    walkStack<TY, V>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist[q.w], tally);
-    flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
+    run(std::false_type{});
   }
 }
 
@@ -831,7 +851,10 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   // against 0.510, 1030 (50 %) 1.15 against 1.05.
   if (elemBytes == 4 && segmentsFilled(X, 0.6)) return choose16Tiling(X, Y, Z);
   if (elemBytes >= 8) {
-    tl.V = 2;
+    // Rows of whole 128-voxel segments: 2 voxels per lane (one 16-byte load
+    // per row); otherwise 1 (1000^3 f64: 1.26 ms against 2.13 for the 2-voxel
+    // kernel's voxel-by-voxel partial waves).
+    tl.V = (X % (kWave * 2) <= 1) ? 2 : 1;
     tl.Ty = 4;
     tl.TP = 512;
     targetWaves = 4096;
@@ -1023,7 +1046,7 @@ template <typename T, int VW>
 static hipError_t dispatchFused(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl0,
                                 hipStream_t s) {
   Tiling tl = tl0;
-  if (fv.nF > 1 || fv.sx != 1 || tl.V == 16) tl.V = 1;  // the general form gathers one voxel per lane
+  if (fv.nF > 1 || fv.sx != 1 || tl.V == 16 || fv.X < tl.V) tl.V = 1;  // the general form gathers one voxel per lane
   const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, tl);
   if (g.nTiles <= 0) return hipSuccess;
   if (tl.V == 1) return dispatchFusedTY<T, 1>(fv, thr, m, bpfd, g, tl.Ty, s);
@@ -1161,7 +1184,7 @@ static Tiling fusedMomTiling(const FieldView& fv, const Tiling& tl0) {
   Tiling tl = tl0;
   tl.Ty = 4;
   if (tl.V == 16) tl.V = 4;  // the 16-voxel kernel has no moments form
-  if (fv.nF > 1 || fv.sx != 1) tl.V = 1;
+  if (fv.nF > 1 || fv.sx != 1 || fv.X < tl.V) tl.V = 1;
   return tl;
 }
 

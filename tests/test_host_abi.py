@@ -252,6 +252,7 @@ def test_volFrac(lib):
     ((1536, 100, 100), 4, (16, 4, None)),    # a whole and a half 1024-voxel segment (75 %)
     ((1200, 100, 100), 4, (1, 4, 4)),        # 59 %: 19 segments as 5 x-tiles of 4 waves
     ((1024, 1024, 1024), 8, (2, 4, 16)),     # doubles: 2 voxels per lane
+    ((1000, 1000, 1000), 8, (1, 4, 16)),     # doubles with a partial 128-voxel segment: 1 voxel per lane
 ])
 def test_default_tiling_choice(lib, dims, eb, want):
     """The built-in fused tiling (host logic only; the slab depth depends on
