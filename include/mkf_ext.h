@@ -27,10 +27,12 @@ extern "C" {
  * (1, or 4 for float / 2 for double fields, 0 = the library's choice; the
  * histogram of a PBM uses 4 unless 1 is asked for; several fields or an x
  * stride use 1), holds rowPairs (2, 4 or 8) row pairs and marches over
- * planePairs (1..65536) plane pairs. voxelsPerLane 16 (float fields, 2 or 4
- * row pairs) selects a persistent kernel of one-wave workgroups, each lane
- * holding 4 groups of 4 voxels 256 apart (a wave covers 1024 voxels of a
- * row); `waves` is then ignored. */
+ * planePairs (1..65536) plane pairs. voxelsPerLane 16 or 8 (float fields,
+ * 2 or 4 row pairs; the fused histogram only) selects a persistent kernel of
+ * one-wave workgroups, each lane holding 4 or 2 groups of 4 voxels 256 apart
+ * (a wave covers 1024 or 512 voxels of a row); `waves` is then ignored. The
+ * histogram of a PBM also takes 16 (half PBM words per lane); it reads 8 as
+ * 4. */
 typedef struct {
   int32_t planePairs;
   int32_t waves;

@@ -143,7 +143,7 @@ Tiling fusedTiling(const MkfTiling* pT, const FieldView& fv) {
     Tiling tl{pT->planePairs, pT->waves, pT->rowPairs, pT->voxelsPerLane};
     if (tl.V == 0) {
       tl.V = chooseTiling(fv.X, fv.Y, fv.Z, eb).V;
-      if (tl.V == 16 && tl.Ty == 8) tl.V = 4;  // the 16-voxel kernel stacks 2 or 4 row pairs
+      if (tl.V >= 8 && tl.Ty == 8) tl.V = 4;  // the 8- and 16-voxel kernel stacks 2 or 4 row pairs
     }
     return tl;
   }
@@ -160,7 +160,7 @@ Tiling toTiling(const MkfTiling* pT, int X, int Y, int Z, int elemBytes) {
     Tiling tl{pT->planePairs, pT->waves, pT->rowPairs, pT->voxelsPerLane};
     if (tl.V == 0) {
       tl.V = chooseTiling(X, Y, Z, elemBytes).V;
-      if (tl.V == 16 && tl.Ty == 8) tl.V = 4;  // the 16-voxel kernel stacks 2 or 4 row pairs
+      if (tl.V >= 8 && tl.Ty == 8) tl.V = 4;  // the 8- and 16-voxel kernel stacks 2 or 4 row pairs
     }
     return tl;
   }
@@ -307,10 +307,14 @@ int mkfFusedAutotune(const BMFieldInfo* pI, const BinMapF64* pM, MkfTiling* pCho
   const int eb = fv.enc & ENC_MASK_NUM;
   // The built-in choice and the shapes that came out ahead of it on some
   // MI355X at 1024^3; several fields or an x stride run one voxel per lane.
-  // Float rows near the 16-voxel kernel's fill threshold get both kernels.
+  // Float fields also get the persistent 8- and 16-voxel kernels, whichever
+  // the built-in choice is not.
   std::vector<Tiling> cand{chooseTiling(fv.X, fv.Y, fv.Z, eb)};
   if (eb == 4) {
-    if (cand[0].V != 16 && fv.X >= 4 && fv.nF == 1 && fv.sx == 1) cand.push_back(choose16Tiling(fv.X, fv.Y, fv.Z));
+    if (fv.X >= 4 && fv.nF == 1 && fv.sx == 1) {
+      if (cand[0].V != 16) cand.push_back(choose16Tiling(fv.X, fv.Y, fv.Z, 16));
+      if (cand[0].V != 8) cand.push_back(choose16Tiling(fv.X, fv.Y, fv.Z, 8));
+    }
     cand.push_back(Tiling{256, 16, 4, 4});
     cand.push_back(Tiling{128, 16, 4, 1});
   } else {

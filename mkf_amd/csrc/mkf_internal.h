@@ -50,7 +50,7 @@ struct Tiling {
 int setError(int code, const char* fmt, ...);
 
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes);  // fused kernel
-Tiling choose16Tiling(int X, int Y, int Z);                 // fused kernel, 16 voxels per lane (floats)
+Tiling choose16Tiling(int X, int Y, int Z, int V = 16);     // fused kernel, 8 or 16 voxels per lane (floats)
 Tiling chooseBpfdTiling(int X, int Y, int Z);              // histogram of a PBM
 Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes);  // fused kernel with moments
 bool tilingSupported(const Tiling& tl, int elemBytes);

@@ -458,10 +458,11 @@ __global__ __launch_bounds__(1024) void kBPFD4(const uint32_t* __restrict__ pW, 
 }
 
 // ---------------------------------------------------------------------------
-// Fused histogram, float fields, 16 voxels per lane (voxelsPerLane 16). The
-// lane layout of kPBM: lane l holds groups q = 0..3 of 4 voxels at
-// x0 + 256 q + 4 l, so each 16-byte load of the wave reads 1 KiB of
-// consecutive bytes and a wave covers a 1024-voxel segment of its rows. The
+// Fused histogram, float fields, 4 NG voxels per lane (voxelsPerLane 16 for
+// NG = 4, 8 for NG = 2). The lane layout of kPBM: lane l holds groups
+// q = 0..NG-1 of 4 voxels at x0 + 256 q + 4 l, so each 16-byte load of the
+// wave reads 1 KiB of consecutive bytes and a wave covers a 256 NG-voxel
+// segment of its rows. The
 // right neighbour of a group's last voxel is the next lane's first voxel of
 // the group: bit l + 1 of that voxel's ballot; lane 63 takes bit 0 of the
 // next group's ballot, or, for the last group, the voxel right of the
@@ -470,13 +471,12 @@ __global__ __launch_bounds__(1024) void kBPFD4(const uint32_t* __restrict__ pW, 
 // adds; one multiply spreads it into the byte-pair word of its 4 cells. One
 // wave per workgroup: no LDS exchange and no barrier.
 // ---------------------------------------------------------------------------
-constexpr int kG16 = 16;
 // The fused 16-voxel kernel runs at most 2 waves per SIMD (8 per CU): more
 // requests in flight lowered its throughput (1024^3: 3 waves per SIMD
 // 0.66-0.73 ms against 0.65 at 2).
 constexpr int kFused16PerCU = 8;
 
-template <int M, int TY>
+template <int M, int TY, int NG>
 __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, float thr, int X, int Y, int Z,
                                                   long long sy, long long sz, TileGrid g,
                                                   unsigned long long* __restrict__ bpfd) {
@@ -485,6 +485,7 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
   __shared__ uint32_t hist[1][256];
   const int lane = threadIdx.x;
   for (int i = lane; i < 256; i += kWave) hist[0][i] = 0;
+  constexpr int VPL = 4 * NG;  // voxels per lane
   Tally tally;
   uint32_t cells = 0;
   // Persistent: block b takes tiles b, b + gridDim.x, ... in the XCD-aware
@@ -497,17 +498,17 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
     const int rr = L / g.nXT;
     const int yr = rr % g.nYR;
     const int zr = rr / g.nYR;
-    const int x0 = xt * (kWave * kG16);
+    const int x0 = xt * (kWave * VPL);
     const int y0 = yr * TY;
     const int nY = min(TY, Y - 1 - y0);
     const int z0 = zr * g.TZ;
     const int nZ = min(g.TZ, Z - 1 - z0);
-    const bool vec = x0 + kWave * kG16 <= X;
-    const bool halo = x0 + kWave * kG16 < X;  // the voxel right of the segment exists
+    const bool vec = x0 + kWave * VPL <= X;
+    const bool halo = x0 + kWave * VPL < X;  // the voxel right of the segment exists
     // cells that exist: x < X - 1
-    uint32_t keep[4], nCells = 0;
+    uint32_t keep[NG], nCells = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NG; ++q) {
       keep[q] = 0;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -525,21 +526,21 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
     // (X >= 4, checked by the host, so the 4 voxels before the row's end
     // exist), as an unsigned offset from 4 voxels before the segment, which
     // keeps the loads in the scalar-base + 32-bit-offset form.
-    uint32_t ga[4];
+    uint32_t ga[NG];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ga[q] = (uint32_t)(min(256 * q + 4 * lane, X - x0 - 4) + 4);
+    for (int q = 0; q < NG; ++q) ga[q] = (uint32_t)(min(256 * q + 4 * lane, X - x0 - 4) + 4);
 
     // The slab walk, compiled once for whole segments (VEC) and once for the
     // last, partial segment of a row, and chosen per tile, so that neither
     // carries the other's loads or branches.
     auto walk = [&](auto vecTag) {
       constexpr bool VEC = decltype(vecTag)::value;
-      auto loadPlane = [&](long long zo, float (&v)[TY + 1][kG16], float (&h)[TY + 1]) {
+      auto loadPlane = [&](long long zo, float (&v)[TY + 1][VPL], float (&h)[TY + 1]) {
 #pragma unroll
         for (int r = 0; r <= TY; ++r) {
           const float* row = base + yo[r] + zo;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < NG; ++q) {
             Vec w;
             if constexpr (VEC) {
               w = (r > 0 && r < TY) ? __builtin_nontemporal_load((GVec)(row + 256 * q + 4 * lane))
@@ -557,26 +558,26 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
 #pragma unroll
             for (int e = 0; e < 4; ++e) v[r][4 * q + e] = w[e];
           }
-          h[r] = halo ? uniformLoad(row + kWave * kG16) : 0.f;
+          h[r] = halo ? uniformLoad(row + kWave * VPL) : 0.f;
         }
       };
 
       // One plane of registers: the pair words of plane z are computed
       // first, then plane z+1 is loaded into the same registers while plane
       // z's cells are tallied.
-      float cur[TY + 1][kG16];
+      float cur[TY + 1][VPL];
       float hc[TY + 1];
       loadPlane((long long)z0 * sz, cur, hc);
-      uint32_t Hp[TY][4];
+      uint32_t Hp[TY][NG];
       for (int z = z0; z <= z0 + nZ; ++z) {
-        uint32_t P[TY + 1][4];
+        uint32_t P[TY + 1][NG];
 #pragma unroll
         for (int r = 0; r <= TY; ++r) {
           if constexpr (!VEC) {
             // Shift each group's voxels back by sh = 0..3 (voxels past the
             // row's end only meet cells that keep[] drops).
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < NG; ++q) {
               const int sh = 256 * q + 4 * lane + 4 - (int)ga[q];
               const bool s1 = (sh & 1) != 0, s2 = (sh & 2) != 0;
               float* g4 = &cur[r][4 * q];
@@ -586,15 +587,15 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
               g4[2] = t2;
             }
           }
-          uint64_t m[4][4];
+          uint64_t m[NG][4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
+          for (int q = 0; q < NG; ++q)
 #pragma unroll
             for (int e = 0; e < 4; ++e) m[q][e] = relMask<M>(cur[r][4 * q + e], thr);
           const uint64_t hb = (halo && (relMask<M>(hc[r], thr) & 1ull)) ? 1ull : 0ull;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const uint64_t nextFirst = q < 3 ? (m[q + 1][0] & 1ull) : hb;
+          for (int q = 0; q < NG; ++q) {
+            const uint64_t nextFirst = q < NG - 1 ? (m[q + 1][0] & 1ull) : hb;
             uint32_t v5 = laneBit((m[q][0] >> 1) | (nextFirst << 63)) ? 1u : 0u;
 #pragma unroll
             for (int e = 3; e >= 0; --e) v5 = dblAddLaneBit(v5, m[q][e]);
@@ -605,7 +606,7 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
 #pragma unroll
         for (int r = 0; r < TY; ++r) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < NG; ++q) {
             const uint32_t Hc = P[r][q] | (P[r + 1][q] << 2);
             if (z > z0 && r < nY) {
               const uint32_t C = Hp[r][q] | (Hc << 4);
@@ -651,7 +652,7 @@ __global__ __launch_bounds__(kWave) void kBPFD16(const uint32_t* __restrict__ pW
     const int rr = L / g.nXT;
     const int yr = rr % g.nYR;
     const int zr = rr / g.nYR;
-    const int x0 = xt * (kWave * kG16);
+    const int x0 = xt * (kWave * 16);
     const int y0 = yr * TY;
     const int nY = min(TY, Y - 1 - y0);
     const int z0 = zr * g.TZ;
@@ -748,8 +749,9 @@ static TileGrid makeGrid(int X, int Y, int Z, const Tiling& tl) {
 }
 
 bool tilingSupported(const Tiling& tl, int elemBytes) {
-  const bool vOk = tl.V == 1 || (elemBytes == 4 && (tl.V == 4 || tl.V == 16)) || (elemBytes == 8 && tl.V == 2);
-  const bool tyOk = tl.Ty == 2 || tl.Ty == 4 || (tl.Ty == 8 && tl.V != 16);
+  const bool vOk =
+      tl.V == 1 || (elemBytes == 4 && (tl.V == 4 || tl.V == 8 || tl.V == 16)) || (elemBytes == 8 && tl.V == 2);
+  const bool tyOk = tl.Ty == 2 || tl.Ty == 4 || (tl.Ty == 8 && tl.V < 8);
   // A workgroup counts its cells in 32 bits: a tile holds at most
   // 4096 voxels x 8 rows x 65536 planes = 2^31 cells.
   return vOk && tyOk && tl.TP >= 1 && tl.TP <= 65536 && tl.ZW >= 1 && tl.ZW <= 16;
@@ -771,9 +773,19 @@ bool tilingSupported(const Tiling& tl, int elemBytes) {
 // autotuner times them on the caller's device.
 static unsigned persistentSlots(const void* kernel, int capPerCU = 0);
 
-// Rows of X voxels fill their 1024-voxel segments at least to frac.
-static bool segmentsFilled(int X, double frac) {
-  return X >= 2 && (double)(X - 1) >= frac * (double)(kWave * kG16) * (double)ceilDiv(X - 1, (long long)kWave * kG16);
+// Rows of X voxels fill their segments of seg voxels at least to frac.
+static bool segmentsFilled(int X, double frac, int seg = kWave * 16) {
+  return X >= 2 && (double)(X - 1) >= frac * (double)seg * (double)ceilDiv(X - 1, (long long)seg);
+}
+
+// Per-CU cap of the persistent fused kernel with V voxels per lane on rows
+// of X voxels: 2 waves per SIMD, or 4 for 8 voxels per lane on rows that
+// fill less than 90 % of their 512-voxel segments (their waves hold fewer
+// loads in flight). Measured on MI355X, 8 voxels per lane, caps 8 / 16 per CU:
+// 300 x 1000 x 3000 0.828 / 0.654 ms, 400 x 1000 x 2500 0.713 / 0.697,
+// 512 x 1024 x 2048 0.657 / 0.714.
+static int fused16PerCU(int V, int X) {
+  return V == 8 && !segmentsFilled(X, 0.9, kWave * 8) ? 2 * kFused16PerCU : kFused16PerCU;
 }
 
 // Slab depth of the persistent 16-voxel kernels: their waves take tiles
@@ -783,9 +795,9 @@ static bool segmentsFilled(int X, double frac) {
 // 1024^3 on MI355X (2048 slots for the fused kernel) that is 128 plane pairs
 // (one tile per wave): 0.651 ms; 64: 0.653, 32: 0.666, 96: 0.833 (some waves
 // take two tiles, most one).
-static int persistentDepth(int X, int Y, int Z, int TY, const void* kernel, int capPerCU) {
+static int persistentDepth(int X, int Y, int Z, int TY, int seg, const void* kernel, int capPerCU) {
   const long long slots = persistentSlots(kernel, capPerCU);
-  const long long cols = ceilDiv(X - 1, (long long)kWave * kG16) * ceilDiv(Y - 1, TY);
+  const long long cols = ceilDiv(X - 1, (long long)seg) * ceilDiv(Y - 1, TY);
   int best = 4;
   long long bestCost = -1;
   for (int tp = 4; tp <= 512; ++tp) {
@@ -829,27 +841,42 @@ static int pickWaves(long long nSeg) {
   return best;
 }
 
-// The 16-voxel fused kernel with 4 row pairs and the slab depth of the cost
-// model (any float field with X >= 4).
-Tiling choose16Tiling(int X, int Y, int Z) {
+// The persistent fused kernel with V = 8 or 16 voxels per lane, 4 row pairs
+// and the slab depth of the cost model (any float field with X >= 4).
+Tiling choose16Tiling(int X, int Y, int Z, int V) {
   Tiling tl;
-  tl.V = 16;
+  tl.V = V == 8 ? 8 : 16;
   tl.Ty = 4;
   tl.ZW = 16;
-  tl.TP = persistentDepth(X, Y, Z, 4, (const void*)kFused16<6, 4>, kFused16PerCU);
+  const void* k = tl.V == 8 ? (const void*)kFused16<6, 4, 2> : (const void*)kFused16<6, 4, 4>;
+  tl.TP = persistentDepth(X, Y, Z, 4, kWave * tl.V, k, fused16PerCU(tl.V, X));
   return tl;
 }
 
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   Tiling tl;
   long long targetWaves;
-  // Float fields whose rows fill their 1024-voxel segments at least 60 % use
-  // the 16-voxel kernel. Against the 1-voxel tiling below (x 1000 x 1000):
-  // X = 1024 0.651 ms (6.6 TB/s) against 0.72; 1000: 0.644 against 0.78;
-  // 800: 0.563 against 0.753; 700: 0.548 against 0.569; 620: 0.551 against
-  // 0.585; 1400: 1.14 against 1.19. Below that it loses: 576 (56 %) 0.548
-  // against 0.510, 1030 (50 %) 1.15 against 1.05.
-  if (elemBytes == 4 && segmentsFilled(X, 0.6)) return choose16Tiling(X, Y, Z);
+  // Float fields (HIP events, x 1000 x 1000 unless given, best of 3 rounds
+  // of 10 launches on MI355X):
+  // * rows that fill their 1024-voxel segments at least 80 %: the persistent
+  //   16-voxel kernel. X = 1024 (x 1024 x 1024): 0.669 ms against 0.714 for 4
+  //   voxels per lane; 1000: 0.686 against 0.714; 900: 0.591 against 0.633;
+  //   1900 x 1000 x 560: 0.722 against 0.888. At 800 (78 %) 4 voxels per lane
+  //   are as fast (0.547 against 0.555).
+  // * rows that fill their 512-voxel segments at least 90 %: the 8-voxel
+  //   kernel. 1536 x 1000 x 700: 0.695 against 0.762 (16) and 0.759 (4).
+  // * rows of 257 to 513 voxels: the 8-voxel kernel. 280 x 1000 x 3600:
+  //   0.817 against 0.891 (1) and 1.027 (4); 300 x 1000 x 3000: 0.653 against
+  //   0.750 and 0.897; 400 x 1000 x 2500: 0.697 against 0.859 and 0.849.
+  // * other rows longer than 257 voxels: 4 voxels per lane. 1030: 0.726
+  //   against 0.998 (1 voxel), 1.060 (16) and 0.779 (8); 1240: 0.854 against
+  //   1.097 (16) and 0.871 (8); 640: 0.444 against 0.555 (16); 576: 0.432
+  //   against 0.511 (1); 2100 x 1000 x 500: 0.697 against 0.818 (16).
+  // * rows of at most 257 voxels: 1 voxel per lane. 200 x 1000 x 5000: 0.797
+  //   against 0.977 (4) and 1.028 (8); 256 x 1000 x 4000: 0.721 against 0.841.
+  if (elemBytes == 4 && segmentsFilled(X, 0.8)) return choose16Tiling(X, Y, Z, 16);
+  if (elemBytes == 4 && (segmentsFilled(X, 0.9, kWave * 8) || (X - 1 > kWave * 4 && X - 1 <= kWave * 8)))
+    return choose16Tiling(X, Y, Z, 8);
   if (elemBytes >= 8) {
     // Rows of whole 128-voxel segments: 2 voxels per lane (one 16-byte load
     // per row); otherwise 1 (1000^3 f64: 1.26 ms against 2.13 for the 2-voxel
@@ -858,6 +885,11 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
     tl.Ty = 4;
     tl.TP = 512;
     targetWaves = 4096;
+  } else if (X - 1 > kWave * 4) {
+    tl.V = 4;
+    tl.Ty = 4;
+    tl.TP = 256;
+    targetWaves = 2048;
   } else {
     tl.V = 1;
     tl.Ty = 4;
@@ -906,7 +938,7 @@ Tiling chooseBpfdTiling(int X, int Y, int Z) {
     tl.V = 16;
     tl.Ty = 4;
     tl.ZW = 16;
-    tl.TP = persistentDepth(X, Y, Z, 4, (const void*)kBPFD16<4>, 0);
+    tl.TP = persistentDepth(X, Y, Z, 4, kWave * 16, (const void*)kBPFD16<4>, 0);
     return tl;
   }
   tl.V = 4;
@@ -1046,7 +1078,7 @@ template <typename T, int VW>
 static hipError_t dispatchFused(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl0,
                                 hipStream_t s) {
   Tiling tl = tl0;
-  if (fv.nF > 1 || fv.sx != 1 || tl.V == 16 || fv.X < tl.V) tl.V = 1;  // the general form gathers one voxel per lane
+  if (fv.nF > 1 || fv.sx != 1 || tl.V >= 8 || fv.X < tl.V) tl.V = 1;  // the general form gathers one voxel per lane
   const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, tl);
   if (g.nTiles <= 0) return hipSuccess;
   if (tl.V == 1) return dispatchFusedTY<T, 1>(fv, thr, m, bpfd, g, tl.Ty, s);
@@ -1073,11 +1105,11 @@ static unsigned persistentSlots(const void* kernel, int capPerCU) {
   return n;
 }
 
-// Tiles of the persistent 16-voxel kernels: 1024 voxels x Ty row pairs x
-// TP plane pairs.
+// Tiles of the persistent 8- and 16-voxel kernels: 64 V voxels x Ty row
+// pairs x TP plane pairs.
 static TileGrid grid16(int X, int Y, int Z, const Tiling& tl) {
   TileGrid g{};
-  g.nSegX = (int)ceilDiv(X - 1, (long long)kWave * kG16);
+  g.nSegX = (int)ceilDiv(X - 1, (long long)kWave * tl.V);
   g.waves = 1;
   g.nXT = g.nSegX;
   g.TZ = tl.TP;
@@ -1095,27 +1127,30 @@ static unsigned blocks16(const TileGrid& g, const Tiling& tl, const void* kernel
   // A workgroup's histogram and cell count are 32-bit: no workgroup takes
   // more than 2^31 cells (only fields of far more than 2^40 cells need more
   // workgroups than slots for that).
-  const long long tileCells = (long long)kWave * kG16 * tl.Ty * tl.TP;
+  const long long tileCells = (long long)kWave * tl.V * tl.Ty * tl.TP;
   const long long maxTiles = std::max(1LL, (1LL << 31) / tileCells);
   const long long need = ceilDiv(ceilDiv((long long)g.chunk * 8, maxTiles), 8) * 8;
   if (need > (long long)nBlocks) nBlocks = (unsigned)std::min<long long>(need, (long long)g.chunk * 8);
   return nBlocks;
 }
 
-template <int M>
-static hipError_t launchFused16M(const FieldView& fv, float thr, unsigned long long* bpfd, const Tiling& tl,
+template <int M, int TY, int NG>
+static hipError_t launchFused16K(const FieldView& fv, float thr, unsigned long long* bpfd, const Tiling& tl,
                                  hipStream_t s) {
   const TileGrid g = grid16(fv.X, fv.Y, fv.Z, tl);
   if (g.nTiles <= 0) return hipSuccess;
-  const unsigned nBlocks =
-      blocks16(g, tl, tl.Ty == 2 ? (const void*)kFused16<M, 2> : (const void*)kFused16<M, 4>, kFused16PerCU);
-  const dim3 grid(nBlocks), block(kWave);
-  const float* p = (const float*)fv.p[0];
-  switch (tl.Ty) {
-    case 2: hipLaunchKernelGGL((kFused16<M, 2>), grid, block, 0, s, p, thr, fv.X, fv.Y, fv.Z, fv.sy, fv.sz, g, bpfd); break;
-    default: hipLaunchKernelGGL((kFused16<M, 4>), grid, block, 0, s, p, thr, fv.X, fv.Y, fv.Z, fv.sy, fv.sz, g, bpfd); break;
-  }
+  const unsigned nBlocks = blocks16(g, tl, (const void*)kFused16<M, TY, NG>, fused16PerCU(4 * NG, fv.X));
+  hipLaunchKernelGGL((kFused16<M, TY, NG>), dim3(nBlocks), dim3(kWave), 0, s, (const float*)fv.p[0], thr, fv.X,
+                     fv.Y, fv.Z, fv.sy, fv.sz, g, bpfd);
   return hipGetLastError();
+}
+
+template <int M>
+static hipError_t launchFused16M(const FieldView& fv, float thr, unsigned long long* bpfd, const Tiling& tl,
+                                 hipStream_t s) {
+  if (tl.V == 8)
+    return tl.Ty == 2 ? launchFused16K<M, 2, 2>(fv, thr, bpfd, tl, s) : launchFused16K<M, 4, 2>(fv, thr, bpfd, tl, s);
+  return tl.Ty == 2 ? launchFused16K<M, 2, 4>(fv, thr, bpfd, tl, s) : launchFused16K<M, 4, 4>(fv, thr, bpfd, tl, s);
 }
 
 static hipError_t launchFused16(const FieldView& fv, float thr, uint32_t m, unsigned long long* bpfd,
@@ -1137,7 +1172,7 @@ hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long lo
   if (fv.X < 2 || fv.Y < 2 || fv.Z < 2) return hipSuccess;  // no cells
   const int eb = fv.enc & ENC_MASK_NUM;
   if (!tilingSupported(tl, eb)) return hipErrorInvalidValue;
-  if (tl.V == 16 && fv.enc == ENC_F32 && fv.nF == 1 && fv.sx == 1 && fv.X >= 4)
+  if ((tl.V == 16 || tl.V == 8) && fv.enc == ENC_F32 && fv.nF == 1 && fv.sx == 1 && fv.X >= 4)
     return launchFused16(fv, (float)m.t[0], m.m, bpfd, tl, s);
   // f32 fields compare against the threshold rounded to float
   // (src/binMapCUDA.cuh:138-139).
@@ -1183,7 +1218,7 @@ static hipError_t dispatchFusedMomM(const FieldView& fv, T thr, uint32_t m, unsi
 static Tiling fusedMomTiling(const FieldView& fv, const Tiling& tl0) {
   Tiling tl = tl0;
   tl.Ty = 4;
-  if (tl.V == 16) tl.V = 4;  // the 16-voxel kernel has no moments form
+  if (tl.V >= 8) tl.V = 4;  // the 8- and 16-voxel kernel has no moments form
   if (fv.nF > 1 || fv.sx != 1 || fv.X < tl.V) tl.V = 1;
   return tl;
 }

@@ -110,21 +110,25 @@ def test_fused_and_pbm_histogram_tilings(lib, orc, cuda, tiling, vpl, dtype, X):
 # segments, exact and ragged (the last segment gathered, the voxel right of a
 # segment read when it exists), 2 and 4 row pairs, slab depths that do and do
 # not divide the planes, slabs deeper than the field.
-V16_CASES = [(300, (5, 1, 4)), (1024, (7, 1, 4)), (1025, (3, 1, 2)), (1030, (1, 1, 4)), (2048, (4, 1, 4)),
-             (2049, (128, 1, 2)), (2100, (9, 16, 4)), (3071, (2, 1, 2))]
+V16_CASES = [(5, (2, 1, 4)), (300, (5, 1, 4)), (512, (2, 1, 4)), (513, (6, 1, 2)), (1024, (7, 1, 4)),
+             (1025, (3, 1, 2)), (1030, (1, 1, 4)), (1537, (5, 1, 4)), (2048, (4, 1, 4)), (2049, (128, 1, 2)),
+             (2100, (9, 16, 4)), (3071, (2, 1, 2))]
 
 
+@pytest.mark.parametrize("V", [16, 8])
 @pytest.mark.parametrize("X,tiling", V16_CASES)
-def test_fused_16_voxels_per_lane(lib, orc, cuda, X, tiling):
-    print(f"16 voxels per lane X={X} tiling {tiling}", flush=True)
+def test_fused_16_voxels_per_lane(lib, orc, cuda, X, tiling, V):
+    """The persistent fused kernel with 16 or 8 voxels per lane; the map's
+    histogram with the same tiling (16 voxels per lane, or 4 for 8)."""
+    print(f"{V} voxels per lane X={X} tiling {tiling}", flush=True)
     f = synth.grf_field((23, 13, X), sigma=2.0, seed=11, device=cuda, dtype=torch.float32)
     f_np = host_np(f)
-    t = M.MkfTiling(*(tiling + (16,)))
+    t = M.MkfTiling(*(tiling + (V,)))
     for relop, thr in ((">=", 0.0), ("<", 0.3), ("!", 0.1), ("=", 0.0)):
         _, h_ref = oracle_all(orc, f_np, relop, thr)
-        h, _ = dev_fused(lib, f, relop, thr, tiling=tiling + (16,))
+        h, _ = dev_fused(lib, f, relop, thr, tiling=tiling + (V,))
         np.testing.assert_array_equal(h, h_ref)
-        # the histogram of the map, 16 voxels per lane
+        # the histogram of the map
         _, dw, org, _ = dev_pbm(lib, f, relop, thr)
         hb = torch.zeros(256, dtype=torch.int64, device=cuda)
         M.check(lib.mkfBPFDAsync(hb.data_ptr(), C.byref(org), dw.data_ptr(), C.byref(t), None))
@@ -136,18 +140,18 @@ def test_fused_16_voxels_per_lane(lib, orc, cuda, X, tiling):
 def test_fused_16_special_values(lib, orc, cuda, relop):
     """NaN, signed zeros, subnormals and infinities at every lane position
     of two segments and the gathered tail."""
-    print(f"16 voxels per lane, special values, relation {relop!r}", flush=True)
+    print(f"16 and 8 voxels per lane, special values, relation {relop!r}", flush=True)
     rng = np.random.default_rng(17)
     choices = np.array([0.0, -0.0, 1e-40, -1e-40, np.nan, np.inf, -np.inf, 0.25, -0.25, 1.0], np.float32)
     f = torch.from_numpy(rng.choice(choices, size=(5, 6, 2100))).to(cuda)
     for thr in (0.0, 0.25):
         _, h_ref = oracle_all(orc, host_np(f), relop, thr)
         _, dw, org, _ = dev_pbm(lib, f, relop, thr)
-        for ty in (2, 4):
-            h, _ = dev_fused(lib, f, relop, thr, tiling=(3, 1, ty, 16))
+        for ty, v in ((2, 16), (4, 16), (2, 8), (4, 8)):
+            h, _ = dev_fused(lib, f, relop, thr, tiling=(3, 1, ty, v))
             np.testing.assert_array_equal(h, h_ref)
             hb = torch.zeros(256, dtype=torch.int64, device=cuda)
-            t = M.MkfTiling(3, 1, ty, 16)
+            t = M.MkfTiling(3, 1, ty, v)
             M.check(lib.mkfBPFDAsync(hb.data_ptr(), C.byref(org), dw.data_ptr(), C.byref(t), None))
             torch.cuda.synchronize()
             np.testing.assert_array_equal(hb.cpu().numpy().view(np.uint64), h_ref)
@@ -162,7 +166,7 @@ def test_fused_16_padded_unaligned_rows(lib, orc, cuda):
     _, h_ref = oracle_all(orc, np.ascontiguousarray(host_np(sub)), ">", 0.05)
     bm = M.bin_map(">", 0.05)
     fi = M.FieldInfo(sub.data_ptr(), (X, Y, Z), M.ENC_F32, strides=(1, 1031, 1031 * 11))
-    for tiling in (None, (4, 1, 4, 16), (2, 1, 2, 16)):
+    for tiling in (None, (4, 1, 4, 16), (2, 1, 2, 16), (3, 1, 4, 8), (5, 1, 2, 8)):
         h = torch.zeros(256, dtype=torch.int64, device=cuda)
         t = M.MkfTiling(*tiling) if tiling else None
         M.check(lib.mkfFusedBPFD(None, h.data_ptr(), None, fi.ref, C.byref(bm), C.byref(t) if t else None, None))
@@ -176,7 +180,7 @@ def test_bad_tiling_is_rejected(lib, cuda):
     h = torch.zeros(256, dtype=torch.int64, device=cuda)
     fi = M.FieldInfo(f.data_ptr(), (64, 8, 8), M.ENC_F32)
     for bad in [(0, 16, 4, 1), (8, 17, 4, 1), (8, 16, 3, 1), (8, 0, 4, 1), (8, 16, 4, 3), (8, 16, 4, 2),
-                (8, 16, 8, 16), (65537, 16, 4, 1)]:
+                (8, 16, 8, 16), (8, 16, 8, 8), (8, 16, 4, 12), (65537, 16, 4, 1)]:
         t = M.MkfTiling(*bad)
         assert lib.mkfFusedBPFD(None, h.data_ptr(), None, fi.ref, C.byref(bm), C.byref(t), None) == -1
 
@@ -490,7 +494,7 @@ def test_fused_autotune(lib, orc, cuda, dtype):
     t = M.MkfTiling()
     M.check(lib.mkfFusedAutotune(fi.ref, C.byref(bm), C.byref(t), None), "autotune")
     assert t.planePairs >= 1 and 1 <= t.waves <= 16 and t.rowPairs in (2, 4, 8)
-    assert t.voxelsPerLane in ((1, 4, 16) if dtype == torch.float32 else (1, 2))
+    assert t.voxelsPerLane in ((1, 4, 8, 16) if dtype == torch.float32 else (1, 2))
     _, h_ref = oracle_all(orc, host_np(f), ">", -0.2)
     h, _ = dev_fused(lib, f, ">", -0.2)
     np.testing.assert_array_equal(h, h_ref)

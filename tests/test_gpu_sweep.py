@@ -1,6 +1,6 @@
 """GPU parity sweep: seeded random shapes, relations, thresholds and tilings.
-Every kernel variant of the fused histogram (1, 4 or 16 voxels per lane for
-floats; 1 or 2 for doubles) and of the map histogram (1, 4 or 16) must equal
+Every kernel variant of the fused histogram (1, 4, 8 or 16 voxels per lane
+for floats; 1 or 2 for doubles) and of the map histogram (1, 4 or 16) must equal
 the oracle bit for bit, and the map must equal the oracle's words.
 
 Row lengths cluster around the 64-, 256- and 1024-voxel segment edges (one
@@ -34,9 +34,9 @@ def _case(seed):
     f32 = bool(rng.random() < 0.7)
     relop = RELOPS[int(rng.integers(0, len(RELOPS)))]
     thr = float(rng.choice([0.0, 0.3, -0.4, 1.5]))
-    vs = [1, 4, 16] if f32 else [1, 2]
+    vs = [1, 4, 8, 16] if f32 else [1, 2]
     V = int(rng.choice(vs))
-    Ty = int(rng.choice([2, 4] if V == 16 else [2, 4, 8]))
+    Ty = int(rng.choice([2, 4] if V >= 8 else [2, 4, 8]))
     TP = int(rng.integers(1, 13))
     waves = int(rng.integers(1, 17))
     vb = int(rng.choice([1, 4, 16]))

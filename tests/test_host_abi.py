@@ -245,12 +245,17 @@ def test_volFrac(lib):
 @pytest.mark.parametrize("dims,eb,want", [
     ((1024, 1024, 1024), 4, (16, 4, None)),  # whole 1024-voxel segments: 16 voxels per lane
     ((1000, 1000, 1000), 4, (16, 4, None)),  # rows fill 98 % of their segment
-    ((769, 20, 20), 4, (16, 4, None)),       # 75 %
-    ((620, 20, 20), 4, (16, 4, None)),       # 60.5 %
-    ((576, 20, 20), 4, (1, 4, 16)),          # 56 %: 1 voxel per lane, 9 segments in one workgroup
-    ((1030, 100, 100), 4, (1, 4, 4)),        # 17 segments: 5 x-tiles of 4 waves
-    ((1536, 100, 100), 4, (16, 4, None)),    # a whole and a half 1024-voxel segment (75 %)
-    ((1200, 100, 100), 4, (1, 4, 4)),        # 59 %: 19 segments as 5 x-tiles of 4 waves
+    ((900, 20, 20), 4, (16, 4, None)),       # 88 %
+    ((1900, 20, 20), 4, (16, 4, None)),      # 93 % of two segments
+    ((769, 20, 20), 4, (4, 4, 16)),          # 75 %: 4 voxels per lane, 3 segments in one workgroup
+    ((620, 20, 20), 4, (4, 4, 16)),          # 60 %
+    ((1030, 100, 100), 4, (4, 4, 16)),       # 50 % of 1024, 67 % of 512: 4 voxels per lane
+    ((1200, 100, 100), 4, (4, 4, 16)),       # 59 % of 1024, 78 % of 512
+    ((1536, 100, 100), 4, (8, 4, None)),     # whole 512-voxel segments: 8 voxels per lane
+    ((512, 100, 100), 4, (8, 4, None)),      # one whole 512-voxel segment
+    ((300, 20, 20), 4, (8, 4, None)),        # rows of 257..513 voxels: 8 voxels per lane
+    ((257, 20, 20), 4, (1, 4, 16)),          # rows of at most 257 voxels: 1 voxel per lane
+    ((200, 20, 20), 4, (1, 4, 16)),
     ((1024, 1024, 1024), 8, (2, 4, 16)),     # doubles: 2 voxels per lane
     ((1000, 1000, 1000), 8, (1, 4, 16)),     # doubles with a partial 128-voxel segment: 1 voxel per lane
 ])
