@@ -841,6 +841,44 @@ static int pickWaves(long long nSeg) {
   return best;
 }
 
+// Slab depth of the fused kernel with 4 voxels per lane: of 256, 128, ...,
+// 8 plane pairs, the depths whose grid holds at least 4096 waves (16 per CU,
+// to hide latency), and of those the one with the least work on the busiest
+// CU. Workgroups are dealt to the CUs, each costing its plane pairs plus 8
+// for its set-up, first plane and flush: ceil(workgroups / CUs) * (TP + 8).
+// 8 when no depth gives 4096 waves. Measured on MI355X: 1030 x 722 x 722 at
+// 256 plane pairs (543 workgroups, 3 on some CUs, 2 on most) 0.570 ms, at 32
+// 0.405; 300 x 669 x 669 at 8 0.320, at 32 0.150; 1030 x 1000 x 1000 and
+// 2100 x 1000 x 500 best at 256.
+static int depth4(int X, int Y, int Z, Tiling tl) {
+  int cus = 0, dev = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+  int best = 8;
+  long long bestCost = -1;
+  for (int tp = 256; tp >= 8; tp >>= 1) {
+    tl.TP = tp;
+    const TileGrid g = makeGrid(X, Y, Z, tl);
+    if ((long long)g.nTiles * g.waves < 4096) continue;
+    const long long cost = ceilDiv(g.nTiles, (long long)cus) * (tp + 8);
+    if (bestCost < 0 || cost < bestCost) {
+      bestCost = cost;
+      best = tp;
+    }
+  }
+  return best;
+}
+
+// Plane pairs each wave of a persistent fused tiling walks: the tiles it
+// takes times their depth.
+static long long persistentWalk(int X, int Y, int Z, const Tiling& tl) {
+  const void* k = tl.V == 8 ? (const void*)kFused16<6, 4, 2> : (const void*)kFused16<6, 4, 4>;
+  const long long slots = persistentSlots(k, fused16PerCU(tl.V, X));
+  const long long tiles =
+      ceilDiv(X - 1, (long long)kWave * tl.V) * ceilDiv(Y - 1, tl.Ty) * ceilDiv(Z - 1, tl.TP);
+  return ceilDiv(tiles, slots) * tl.TP;
+}
+
 // The persistent fused kernel with V = 8 or 16 voxels per lane, 4 row pairs
 // and the slab depth of the cost model (any float field with X >= 4).
 Tiling choose16Tiling(int X, int Y, int Z, int V) {
@@ -857,7 +895,7 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   Tiling tl;
   long long targetWaves;
   // Float fields (HIP events, x 1000 x 1000 unless given, best of 3 rounds
-  // of 10 launches on MI355X):
+  // of 10-20 launches on MI355X):
   // * rows that fill their 1024-voxel segments at least 80 %: the persistent
   //   16-voxel kernel. X = 1024 (x 1024 x 1024): 0.669 ms against 0.714 for 4
   //   voxels per lane; 1000: 0.686 against 0.714; 900: 0.591 against 0.633;
@@ -868,15 +906,31 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   // * rows of 257 to 513 voxels: the 8-voxel kernel. 280 x 1000 x 3600:
   //   0.817 against 0.891 (1) and 1.027 (4); 300 x 1000 x 3000: 0.653 against
   //   0.750 and 0.897; 400 x 1000 x 2500: 0.697 against 0.859 and 0.849.
-  // * other rows longer than 257 voxels: 4 voxels per lane. 1030: 0.726
-  //   against 0.998 (1 voxel), 1.060 (16) and 0.779 (8); 1240: 0.854 against
-  //   1.097 (16) and 0.871 (8); 640: 0.444 against 0.555 (16); 576: 0.432
-  //   against 0.511 (1); 2100 x 1000 x 500: 0.697 against 0.818 (16).
-  // * rows of at most 257 voxels: 1 voxel per lane. 200 x 1000 x 5000: 0.797
-  //   against 0.977 (4) and 1.028 (8); 256 x 1000 x 4000: 0.721 against 0.841.
-  if (elemBytes == 4 && segmentsFilled(X, 0.8)) return choose16Tiling(X, Y, Z, 16);
-  if (elemBytes == 4 && (segmentsFilled(X, 0.9, kWave * 8) || (X - 1 > kWave * 4 && X - 1 <= kWave * 8)))
-    return choose16Tiling(X, Y, Z, 8);
+  // * the persistent kernels only where each wave walks at least 48 plane
+  //   pairs: on smaller fields the pipeline's start and end are not paid for.
+  //   At 2^27 voxels 4 voxels per lane win (1000 x 366 x 366: 0.104 against
+  //   0.119 (16); 1536 x 296 x 296: 0.099 against 0.111 (8); waves walk 17 and
+  //   33 plane pairs), at 2^29 the persistent kernels (1000 x 733 x 733: 0.363
+  //   against 0.402; 1536 x 591 x 591: 0.357 against 0.419; 67 and 132).
+  // * other rows longer than 513 voxels: 4 voxels per lane, slab depth from
+  //   depth4. 1030: 0.736 against 1.008 (1 voxel), 1.105 (16) and 0.792 (8);
+  //   1240: 0.854 against 1.097 (16) and 0.871 (8); 640: 0.454 against 0.549
+  //   (16); 576: 0.420 against 0.510 (1); 2100 x 1000 x 500: 0.694 against
+  //   0.818 (16); 700 x 500 x 500: 0.124 against 0.167 (16); 2000 x 200 x 300:
+  //   0.087 against 0.111 (16).
+  // * other rows: 1 voxel per lane. 200 x 1000 x 5000: 0.797 against 0.977
+  //   (4) and 1.028 (8); 256 x 1000 x 4000: 0.721 against 0.841;
+  //   300 x 669 x 669: 0.133 against 0.149 (4).
+  if (elemBytes == 4) {
+    if (segmentsFilled(X, 0.8)) {
+      const Tiling t16 = choose16Tiling(X, Y, Z, 16);
+      if (persistentWalk(X, Y, Z, t16) >= 48) return t16;
+    }
+    if (segmentsFilled(X, 0.9, kWave * 8) || (X - 1 > kWave * 4 && X - 1 <= kWave * 8)) {
+      const Tiling t8 = choose16Tiling(X, Y, Z, 8);
+      if (persistentWalk(X, Y, Z, t8) >= 48) return t8;
+    }
+  }
   if (elemBytes >= 8) {
     // Rows of whole 128-voxel segments: 2 voxels per lane (one 16-byte load
     // per row); otherwise 1 (1000^3 f64: 1.26 ms against 2.13 for the 2-voxel
@@ -885,11 +939,12 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
     tl.Ty = 4;
     tl.TP = 512;
     targetWaves = 4096;
-  } else if (X - 1 > kWave * 4) {
+  } else if (X - 1 > kWave * 8) {
     tl.V = 4;
     tl.Ty = 4;
-    tl.TP = 256;
-    targetWaves = 2048;
+    tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * 4));
+    tl.TP = depth4(X, Y, Z, tl);
+    return tl;
   } else {
     tl.V = 1;
     tl.Ty = 4;

@@ -245,17 +245,19 @@ def test_volFrac(lib):
 @pytest.mark.parametrize("dims,eb,want", [
     ((1024, 1024, 1024), 4, (16, 4, None)),  # whole 1024-voxel segments: 16 voxels per lane
     ((1000, 1000, 1000), 4, (16, 4, None)),  # rows fill 98 % of their segment
-    ((900, 20, 20), 4, (16, 4, None)),       # 88 %
-    ((1900, 20, 20), 4, (16, 4, None)),      # 93 % of two segments
-    ((769, 20, 20), 4, (4, 4, 16)),          # 75 %: 4 voxels per lane, 3 segments in one workgroup
-    ((620, 20, 20), 4, (4, 4, 16)),          # 60 %
-    ((1030, 100, 100), 4, (4, 4, 16)),       # 50 % of 1024, 67 % of 512: 4 voxels per lane
+    ((900, 1000, 1000), 4, (16, 4, None)),   # 88 %
+    ((1536, 1000, 700), 4, (8, 4, None)),    # whole 512-voxel segments: 8 voxels per lane
+    ((512, 1024, 2048), 4, (8, 4, None)),    # one whole 512-voxel segment
+    ((300, 1000, 3000), 4, (8, 4, None)),    # rows of 257..513 voxels: 8 voxels per lane
+    ((900, 20, 20), 4, (4, 4, 16)),          # a small field: waves would walk too few planes
+    ((1536, 20, 20), 4, (4, 4, 16)),
+    ((300, 20, 20), 4, (1, 4, 16)),          # rows of 257..513 voxels on a small field: 1 voxel per lane
+    ((769, 1000, 1000), 4, (4, 4, 16)),      # 75 %: 4 voxels per lane, 3 segments in one workgroup
+    ((1030, 1000, 1000), 4, (4, 4, 16)),     # 50 % of 1024, 67 % of 512
     ((1200, 100, 100), 4, (4, 4, 16)),       # 59 % of 1024, 78 % of 512
-    ((1536, 100, 100), 4, (8, 4, None)),     # whole 512-voxel segments: 8 voxels per lane
-    ((512, 100, 100), 4, (8, 4, None)),      # one whole 512-voxel segment
-    ((300, 20, 20), 4, (8, 4, None)),        # rows of 257..513 voxels: 8 voxels per lane
     ((257, 20, 20), 4, (1, 4, 16)),          # rows of at most 257 voxels: 1 voxel per lane
-    ((200, 20, 20), 4, (1, 4, 16)),
+    ((514, 20, 20), 4, (4, 4, 16)),          # rows over 513 voxels: 4 voxels per lane
+    ((200, 1000, 5000), 4, (1, 4, 16)),
     ((1024, 1024, 1024), 8, (2, 4, 16)),     # doubles: 2 voxels per lane
     ((1000, 1000, 1000), 8, (1, 4, 16)),     # doubles with a partial 128-voxel segment: 1 voxel per lane
 ])
