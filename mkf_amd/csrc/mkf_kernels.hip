@@ -986,25 +986,26 @@ Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
 // slabs of up to 256 plane pairs halved until the grid has about 16K
 // wave-lanes' worth of work (1024^3: 0.34 ms).
 Tiling chooseBpfdTiling(int X, int Y, int Z) {
+  // 4 voxels per lane, 32 plane pairs per workgroup, or 16 where that gives
+  // fewer than 4 workgroups per CU. Measured on MI355X on GRF maps (HIP
+  // events, best of 3 rounds of 20 launches), against the 16-voxel kernel
+  // (kBPFD16) with its resident-slot slab depth:
+  //   1024^3 0.247 ms against 0.257; 1000 x 733 x 733 0.141 against 0.194;
+  //   1000 x 366 x 366 0.057 against 0.137; 1536 x 1000 x 700 0.294 against
+  //   0.376; 800 x 1000 x 1000 0.248 against 0.248; 2048 x 256 x 256 (16
+  //   plane pairs) 0.044 against 0.138; 1024 x 181 x 181 (16) 0.028 against
+  //   0.056.
+  // 256 plane pairs, the earlier choice, took 0.325 ms at 1024^3, and 64
+  // took 0.254; 16 took 0.41 there.
+  int cus = 0, dev = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
   Tiling tl;
-  // The PBM histogram is bound by instruction issue, so lanes past the row
-  // end cost in full: the 16-voxel kernel from 3/4 filled segments.
-  if (segmentsFilled(X, 0.75)) {
-    tl.V = 16;
-    tl.Ty = 4;
-    tl.ZW = 16;
-    tl.TP = persistentDepth(X, Y, Z, 4, kWave * 16, (const void*)kBPFD16<4>, 0);
-    return tl;
-  }
   tl.V = 4;
   tl.Ty = 4;
-  tl.TP = 256;
+  tl.TP = 32;
   tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * tl.V));
-  while (tl.TP > 8) {
-    const TileGrid g = makeGrid(X, Y, Z, tl);
-    if ((long long)g.nTiles * g.waves * tl.V >= 256LL * 16 * 4) break;
-    tl.TP >>= 1;
-  }
+  if ((long long)makeGrid(X, Y, Z, tl).nTiles < 4LL * cus) tl.TP = 16;
   return tl;
 }
 
