@@ -358,7 +358,8 @@ def main():
                     help="for counter runs: 3 fused launches, then 3 copies of a 1 GiB buffer (the calibration "
                          "kernel for FETCH_SIZE / WRITE_SIZE); with --mode batch, 3 batched launches; prints no JSON")
     ap.add_argument("--pmc-target", choices=("fused", "bpfd"), default="fused",
-                    help="with --pmc-workload: 'bpfd' makes the PBM once and runs 3 histograms of it instead")
+                    help="with --pmc-workload: 'bpfd' makes the PBM once and runs 3 histograms of it instead "
+                         "(with --tiling's tiling, else the library's default)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -430,8 +431,10 @@ def main():
         M.check(lib.mkfPBMAsync(pbm.data_ptr(), C.byref(org), fi.ref, C.byref(bm), sptr), "mkfPBMAsync")
         del field
         torch.cuda.synchronize()
+        # --tiling selects the histogram's tiling; without it, the library's default
+        bt = C.byref(tiling) if args.tiling else None
         for _ in range(3):
-            M.check(lib.mkfBPFDAsync(hist.data_ptr(), C.byref(org), pbm.data_ptr(), None, sptr), "mkfBPFDAsync")
+            M.check(lib.mkfBPFDAsync(hist.data_ptr(), C.byref(org), pbm.data_ptr(), bt, sptr), "mkfBPFDAsync")
         torch.cuda.synchronize()
         log("[bench] counter workload done: 1 PBM, 3 histograms of the PBM")
         return
