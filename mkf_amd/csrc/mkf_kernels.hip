@@ -968,12 +968,26 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
 //        pass: 0.715 + 0.670 ms
 //   f64: 2 voxels/lane, 128 plane pairs 1.595 ms (512: 1.609); two passes
 //        1.173 + 1.245 ms
+// Float fields take 32 plane pairs, or 16 where that gives fewer than 8
+// workgroups per CU (GRF fields, best of 3 rounds of 10 launches): 1024^3
+// 0.872 ms (64: 0.871); 1000 x 733 x 733 0.478 (64: 0.507); 300 x 669 x 669
+// 0.256 (8, the earlier choice: 0.418); 1030 x 722 x 722 0.796 (64: 0.873);
+// 1536 x 1000 x 700 1.232 (64: 1.227); at 16: 1000 x 366 x 366 0.153 (32:
+// 0.166); 700 x 500 x 500 0.216 (0.218); 2000 x 200 x 300 0.144 (0.142).
 Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
   Tiling tl;
   tl.V = elemBytes >= 8 ? 2 : 4;
   tl.Ty = 4;
   tl.TP = elemBytes >= 8 ? 128 : 64;
   tl.ZW = 16;
+  if (elemBytes == 4) {
+    int cus = 0, dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    tl.TP = 32;
+    if ((long long)makeGrid(X, Y, Z, tl).nTiles < 8LL * cus) tl.TP = 16;
+    return tl;
+  }
   while (tl.TP > 8) {
     const TileGrid g = makeGrid(X, Y, Z, tl);
     if ((long long)g.nTiles * g.waves >= 16384) break;
@@ -982,9 +996,7 @@ Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
   return tl;
 }
 
-// Histogram of an existing PBM: 4 voxels per lane (kBPFD4), 4 row pairs,
-// slabs of up to 256 plane pairs halved until the grid has about 16K
-// wave-lanes' worth of work (1024^3: 0.34 ms).
+// Histogram of an existing PBM: kBPFD4 with 4 row pairs.
 Tiling chooseBpfdTiling(int X, int Y, int Z) {
   // 4 voxels per lane, 32 plane pairs per workgroup, or 16 where that gives
   // fewer than 4 workgroups per CU. Measured on MI355X on GRF maps (HIP
