@@ -841,22 +841,26 @@ static int pickWaves(long long nSeg) {
   return best;
 }
 
-// Slab depth of the fused kernel with 4 voxels per lane: of 256, 128, ...,
-// 8 plane pairs, the depths whose grid holds at least 4096 waves (16 per CU,
-// to hide latency), and of those the one with the least work on the busiest
-// CU. Workgroups are dealt to the CUs, each costing its plane pairs plus 8
-// for its set-up, first plane and flush: ceil(workgroups / CUs) * (TP + 8).
-// 8 when no depth gives 4096 waves. Measured on MI355X: 1030 x 722 x 722 at
-// 256 plane pairs (543 workgroups, 3 on some CUs, 2 on most) 0.570 ms, at 32
-// 0.405; 300 x 669 x 669 at 8 0.320, at 32 0.150; 1030 x 1000 x 1000 and
-// 2100 x 1000 x 500 best at 256.
-static int depth4(int X, int Y, int Z, Tiling tl) {
+// Slab depth of the fused kernel (kFusedBPFD) for float fields at 4 voxels
+// per lane and for double fields: of maxTP, maxTP / 2, ..., 8 plane pairs,
+// the depths whose grid holds at least 4096 waves (16 per CU, to hide
+// latency), and of those the one with the least work on the busiest CU.
+// Workgroups are dealt to the CUs, each costing its plane pairs plus 8 for
+// its set-up, first plane and flush: ceil(workgroups / CUs) * (TP + 8). 8
+// when no depth gives 4096 waves. Measured on MI355X, floats: 1030 x 722 x
+// 722 at 256 plane pairs (543 workgroups, 3 on some CUs, 2 on most) 0.570 ms,
+// at 32 0.405; 300 x 669 x 669 at 8 0.320, at 32 0.150; 1030 x 1000 x 1000 and
+// 2100 x 1000 x 500 best at 256. Doubles (the earlier rule halved 512 until
+// the grid held 4096 waves): 2000 x 200 x 300 at 128 0.275, at 32 0.166;
+// 1000 x 366 x 366 at 128 0.268, at 32 0.192; 1024 x 362 x 362 at 64 0.204,
+// at 32 0.187; 1024^3 best at 512 (1.18), 700 x 500 x 500 at 128 (0.221).
+static int balancedDepth(int X, int Y, int Z, Tiling tl, int maxTP) {
   int cus = 0, dev = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
   int best = 8;
   long long bestCost = -1;
-  for (int tp = 256; tp >= 8; tp >>= 1) {
+  for (int tp = maxTP; tp >= 8; tp >>= 1) {
     tl.TP = tp;
     const TileGrid g = makeGrid(X, Y, Z, tl);
     if ((long long)g.nTiles * g.waves < 4096) continue;
@@ -913,7 +917,7 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   //   33 plane pairs), at 2^29 the persistent kernels (1000 x 733 x 733: 0.363
   //   against 0.402; 1536 x 591 x 591: 0.357 against 0.419; 67 and 132).
   // * other rows longer than 513 voxels: 4 voxels per lane, slab depth from
-  //   depth4. 1030: 0.736 against 1.008 (1 voxel), 1.105 (16) and 0.792 (8);
+  //   balancedDepth. 1030: 0.736 against 1.008 (1 voxel), 1.105 (16) and 0.792 (8);
   //   1240: 0.854 against 1.097 (16) and 0.871 (8); 640: 0.454 against 0.549
   //   (16); 576: 0.420 against 0.510 (1); 2100 x 1000 x 500: 0.694 against
   //   0.818 (16); 700 x 500 x 500: 0.124 against 0.167 (16); 2000 x 200 x 300:
@@ -937,13 +941,14 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
     // kernel's voxel-by-voxel partial waves).
     tl.V = (X % (kWave * 2) <= 1) ? 2 : 1;
     tl.Ty = 4;
-    tl.TP = 512;
-    targetWaves = 4096;
+    tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * tl.V));
+    tl.TP = balancedDepth(X, Y, Z, tl, 512);
+    return tl;
   } else if (X - 1 > kWave * 8) {
     tl.V = 4;
     tl.Ty = 4;
     tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * 4));
-    tl.TP = depth4(X, Y, Z, tl);
+    tl.TP = balancedDepth(X, Y, Z, tl, 256);
     return tl;
   } else {
     tl.V = 1;
