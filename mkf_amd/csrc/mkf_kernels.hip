@@ -993,11 +993,12 @@ Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
     if ((long long)makeGrid(X, Y, Z, tl).nTiles < 8LL * cus) tl.TP = 16;
     return tl;
   }
-  while (tl.TP > 8) {
-    const TileGrid g = makeGrid(X, Y, Z, tl);
-    if ((long long)g.nTiles * g.waves >= 16384) break;
-    tl.TP >>= 1;
-  }
+  // Double fields: the busiest-CU model from 128 plane pairs. Against the
+  // earlier rule (128 halved until the grid held 16K waves): 2000 x 200 x 300
+  // 0.190 ms (32) against 0.245; 300 x 669 x 669 0.218 (32) against 0.239;
+  // 700 x 500 x 500 0.294 (64) against 0.323; 1024^3 and 1000^3 unchanged
+  // (128).
+  tl.TP = balancedDepth(X, Y, Z, tl, 128);
   return tl;
 }
 
