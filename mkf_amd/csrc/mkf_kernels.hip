@@ -922,6 +922,8 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   //   (16); 576: 0.420 against 0.510 (1); 2100 x 1000 x 500: 0.694 against
   //   0.818 (16); 700 x 500 x 500: 0.124 against 0.167 (16); 2000 x 200 x 300:
   //   0.087 against 0.111 (16).
+  // * rows of 258..513 voxels filling 72 % of their 256-voxel segments: 4
+  //   voxels per lane at 32 plane pairs (below).
   // * other rows: 1 voxel per lane. 200 x 1000 x 5000: 0.797 against 0.977
   //   (4) and 1.028 (8); 256 x 1000 x 4000: 0.721 against 0.841;
   //   300 x 669 x 669: 0.133 against 0.149 (4).
@@ -949,6 +951,20 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
     tl.Ty = 4;
     tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * 4));
     tl.TP = balancedDepth(X, Y, Z, tl, 256);
+    return tl;
+  } else if (X - 1 > kWave * 4 && segmentsFilled(X, 0.72, kWave * 4)) {
+    // Rows of 258..513 voxels that fill their two 256-voxel segments at
+    // least 72 %: 4 voxels per lane at 32 plane pairs (2-wave workgroups,
+    // whose grids the busiest-CU model sizes poorly). Against 1 voxel per
+    // lane: 400 x 819 x 819 0.214 ms against 0.251; 400 x 290 x 290 0.054
+    // against 0.073; 390 x 1000 x 700 0.219 against 0.258; 500^3 0.104
+    // against 0.117; 383 x 300 x 300 0.054 against 0.074. Below it loses:
+    // 350 x 600 x 600 (68 %) 0.137 against 0.122; 300 x 669 x 669 (58 %)
+    // 0.148 against 0.133.
+    tl.V = 4;
+    tl.Ty = 4;
+    tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * 4));
+    tl.TP = 32;
     return tl;
   } else {
     tl.V = 1;
