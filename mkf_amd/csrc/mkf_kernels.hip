@@ -873,6 +873,16 @@ static int balancedDepth(int X, int Y, int Z, Tiling tl, int maxTP) {
   return best;
 }
 
+// At least 2 workgroups per CU for row stacks of 4 row pairs and slabs of
+// 32 plane pairs (one workgroup per row stack: rows of up to 16 segments).
+static bool grid32Filled(int X, int Y, int Z) {
+  int cus = 0, dev = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+  (void)X;
+  return ceilDiv(Y - 1, 4) * ceilDiv(Z - 1, 32) >= 2LL * cus;
+}
+
 // Plane pairs each wave of a persistent fused tiling walks: the tiles it
 // takes times their depth.
 static long long persistentWalk(int X, int Y, int Z, const Tiling& tl) {
@@ -952,15 +962,18 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
     tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * 4));
     tl.TP = balancedDepth(X, Y, Z, tl, 256);
     return tl;
-  } else if (X - 1 > kWave * 4 && segmentsFilled(X, 0.72, kWave * 4)) {
+  } else if (X - 1 > kWave * 4 && segmentsFilled(X, 0.72, kWave * 4) && grid32Filled(X, Y, Z)) {
     // Rows of 258..513 voxels that fill their two 256-voxel segments at
-    // least 72 %: 4 voxels per lane at 32 plane pairs (2-wave workgroups,
-    // whose grids the busiest-CU model sizes poorly). Against 1 voxel per
-    // lane: 400 x 819 x 819 0.214 ms against 0.251; 400 x 290 x 290 0.054
-    // against 0.073; 390 x 1000 x 700 0.219 against 0.258; 500^3 0.104
-    // against 0.117; 383 x 300 x 300 0.054 against 0.074. Below it loses:
-    // 350 x 600 x 600 (68 %) 0.137 against 0.122; 300 x 669 x 669 (58 %)
-    // 0.148 against 0.133.
+    // least 72 %, on grids of at least 2 workgroups per CU at 32 plane
+    // pairs: 4 voxels per lane at 32 plane pairs (2-wave workgroups, whose
+    // grids the busiest-CU model sizes poorly). Against 1 voxel per lane:
+    // 400 x 819 x 819 0.214 ms against 0.251; 400 x 290 x 290 (730
+    // workgroups) 0.054 against 0.073; 390 x 1000 x 700 0.219 against 0.258;
+    // 500^3 0.104 against 0.117; 376 x 363 x 212 (637) 0.052 against 0.064.
+    // It loses below 72 % (350 x 600 x 600, 68 %: 0.137 against 0.122;
+    // 300 x 669 x 669, 58 %: 0.148 against 0.133) and on smaller grids
+    // (383 x 92 x 375, 276 workgroups: 0.048 against 0.034; 370 x 321 x 57,
+    // 160: 0.040 against 0.022).
     tl.V = 4;
     tl.Ty = 4;
     tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * 4));
