@@ -86,7 +86,9 @@ int mkfFusedBPFDMoments(size_t *pBPFD, double *pMom, BMOrg *pO, const BMFieldInf
 int mkfBPFDAsync(size_t *pBPFD, const BMOrg *pO, const BMPackWord *pW, const MkfTiling *pT, void *stream);
 
 /* Tiling the library chooses for the fused kernel on a field of dims def
- * and element size (the histogram of a PBM has its own default). */
+ * and element size (the histogram of a PBM has its own default). The choice
+ * depends on the current device: its CU count and how many workgroups of the
+ * persistent kernels it holds at once. */
 MkfTiling mkfDefaultTiling(const FieldDef def[3], int elemBytes);
 
 /* Times the fused kernel on this field (the caller's data, a scratch
