@@ -10,6 +10,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "mkf_amd", "csrc")
@@ -35,7 +36,11 @@ HIP_FLAGS = [
 ]
 C_FLAGS = ["-O2", "-std=gnu11", "-fPIC", "-Wall", f"-I{INCLUDE}"]
 
-HIP_SOURCES = ["mkf_kernels.hip", "mkf_api.hip", "mkf_batch.hip", "mkf_serve.hip", "mkf_stream.hip"]
+HIP_SOURCES = ["mkf_api.hip", "mkf_batch.hip", "mkf_serve.hip", "mkf_stream.hip"]
+# mkf_kernels.hip is compiled once per part (-DMKF_KPART=i), each part holding
+# a disjoint set of launchers and their kernel instantiations.
+KERNEL_SOURCE = "mkf_kernels.hip"
+KERNEL_PARTS = 7
 C_SOURCES = ["binmap_util.c", "measures.c"]
 
 
@@ -49,17 +54,26 @@ def _run(cmd):
     return out
 
 
-def build_library(verbose=False):
+def build_library(verbose=False, jobs=None):
+    """Compiles every source (in parallel, one compiler process per source)
+    and links the shared library."""
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
+    for part in range(KERNEL_PARTS):
+        obj = os.path.join(BUILD, f"{KERNEL_SOURCE}.p{part}.o")
+        cmds.append([HIPCC, *HIP_FLAGS, f"-DMKF_KPART={part}", "-c", os.path.join(CSRC, KERNEL_SOURCE), "-o", obj])
+        objs.append(obj)
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in C_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    n = jobs or min(len(cmds), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=n) as ex:
+        list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -15,6 +15,19 @@
 #include "mkf_device.hpp"
 #include "mkf_internal.h"
 
+// The file is compiled once per part (-DMKF_KPART=0..6), each part holding
+// only its own launchers and so instantiating only its own kernels; the
+// parts compile in parallel. Without MKF_KPART one compile holds them all.
+//   0 tiling choice and the fused dispatcher   4 kFused16
+//   1 kPBM and the moment reduction            5 kFusedBPFD with moments
+//   2 kFusedBPFD for float fields              6 kBPFD4 / kBPFD16 / kBPFD
+//   3 kFusedBPFD for double fields
+#ifdef MKF_KPART
+#define MKF_PART(p) (MKF_KPART == (p))
+#else
+#define MKF_PART(p) 1
+#endif
+
 namespace mkf {
 
 // ---------------------------------------------------------------------------
@@ -204,6 +217,7 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   if constexpr (MOM) momBlockPartial(acc, part);
 }
 
+#if MKF_PART(1)
 // Adds the per-workgroup moment partials to out[kMomN] in a fixed order:
 // thread t sums partials t, t + 256, ... and a pairwise tree sums the threads.
 __global__ __launch_bounds__(256) void kMomentsReduce(const double* __restrict__ part, long long nParts,
@@ -222,6 +236,7 @@ __global__ __launch_bounds__(256) void kMomentsReduce(const double* __restrict__
     __syncthreads();
   }
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // Histogram kernels. A tile is (x-tile, y-range, z-range): a workgroup of
@@ -721,6 +736,16 @@ __global__ __launch_bounds__(kWave) void kBPFD16(const uint32_t* __restrict__ pW
 // ---------------------------------------------------------------------------
 static inline long long ceilDiv(long long a, long long b) { return (a + b - 1) / b; }
 
+// Launchers defined in other parts of this file.
+hipError_t launchMomentsReduce(const double* part, long long nParts, double* mom, hipStream_t s);  // part 1
+hipError_t launchFusedF32(const FieldView& fv, float thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
+                          hipStream_t s);  // part 2
+hipError_t launchFusedF64(const FieldView& fv, double thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
+                          hipStream_t s);  // part 3
+hipError_t launchFused16(const FieldView& fv, float thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
+                         hipStream_t s);  // part 4
+unsigned fused16Slots(int V, int X);      // part 4: resident slots of kFused16 with V voxels per lane
+
 // Kernel view of a field description; unused table slots repeat field 0.
 template <typename T>
 static FieldArgs<T> fieldArgs(const FieldView& fv) {
@@ -733,7 +758,7 @@ static FieldArgs<T> fieldArgs(const FieldView& fv) {
   return a;
 }
 
-static TileGrid makeGrid(int X, int Y, int Z, const Tiling& tl) {
+static inline TileGrid makeGrid(int X, int Y, int Z, const Tiling& tl) {
   TileGrid g;
   g.nSegX = (int)ceilDiv(X - 1, (long long)kWave * tl.V);
   // As few x-tiles as tl.ZW waves allow, with the segments spread evenly
@@ -748,6 +773,7 @@ static TileGrid makeGrid(int X, int Y, int Z, const Tiling& tl) {
   return g;
 }
 
+#if MKF_PART(0)
 bool tilingSupported(const Tiling& tl, int elemBytes) {
   const bool vOk =
       tl.V == 1 || (elemBytes == 4 && (tl.V == 4 || tl.V == 8 || tl.V == 16)) || (elemBytes == 8 && tl.V == 2);
@@ -756,6 +782,7 @@ bool tilingSupported(const Tiling& tl, int elemBytes) {
   // 4096 voxels x 8 rows x 65536 planes = 2^31 cells.
   return vOk && tyOk && tl.TP >= 1 && tl.TP <= 65536 && tl.ZW >= 1 && tl.ZW <= 16;
 }
+#endif
 
 // Fused kernel: workgroups as wide as the row (up to 16 waves) and the
 // longest slab that still gives the grid a dtype's target number of waves.
@@ -771,10 +798,29 @@ bool tilingSupported(const Tiling& tl, int elemBytes) {
 //        1 voxel/lane, 8 row pairs, 256 plane pairs   1.293 ms
 // The f32 leaders are within 1 % and swap between machines; the library's
 // autotuner times them on the caller's device.
-static unsigned persistentSlots(const void* kernel, int capPerCU = 0);
+// Resident one-wave workgroups of a kernel on the current device, at most
+// capPerCU per CU when capPerCU > 0, rounded down to a multiple of 8
+// (computed once per kernel, device and cap).
+static inline unsigned persistentSlots(const void* kernel, int capPerCU = 0) {
+  static std::mutex mu;
+  static std::map<std::tuple<const void*, int, int>, unsigned> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find({kernel, dev, capPerCU});
+  if (it != cache.end()) return it->second;
+  int perCU = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, kWave, 0) != hipSuccess || perCU < 1) perCU = 1;
+  if (capPerCU > 0) perCU = std::min(perCU, capPerCU);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 8;
+  const unsigned n = std::max(8u, ((unsigned)(perCU * cus)) / 8u * 8u);
+  cache[{kernel, dev, capPerCU}] = n;
+  return n;
+}
+
 
 // Rows of X voxels fill their segments of seg voxels at least to frac.
-static bool segmentsFilled(int X, double frac, int seg = kWave * 16) {
+static inline bool segmentsFilled(int X, double frac, int seg = kWave * 16) {
   return X >= 2 && (double)(X - 1) >= frac * (double)seg * (double)ceilDiv(X - 1, (long long)seg);
 }
 
@@ -784,10 +830,11 @@ static bool segmentsFilled(int X, double frac, int seg = kWave * 16) {
 // loads in flight). Measured on MI355X, 8 voxels per lane, caps 8 / 16 per CU:
 // 300 x 1000 x 3000 0.828 / 0.654 ms, 400 x 1000 x 2500 0.713 / 0.697,
 // 512 x 1024 x 2048 0.657 / 0.714.
-static int fused16PerCU(int V, int X) {
+static inline int fused16PerCU(int V, int X) {
   return V == 8 && !segmentsFilled(X, 0.9, kWave * 8) ? 2 * kFused16PerCU : kFused16PerCU;
 }
 
+#if MKF_PART(0)
 // Slab depth of the persistent 16-voxel kernels: their waves take tiles
 // round robin, so the time goes as the most tiles any wave takes times the
 // planes a tile reads (TP + 1, its first plane read twice). The depth with
@@ -795,8 +842,7 @@ static int fused16PerCU(int V, int X) {
 // 1024^3 on MI355X (2048 slots for the fused kernel) that is 128 plane pairs
 // (one tile per wave): 0.651 ms; 64: 0.653, 32: 0.666, 96: 0.833 (some waves
 // take two tiles, most one).
-static int persistentDepth(int X, int Y, int Z, int TY, int seg, const void* kernel, int capPerCU) {
-  const long long slots = persistentSlots(kernel, capPerCU);
+static int persistentDepth(int X, int Y, int Z, int TY, int seg, long long slots) {
   const long long cols = ceilDiv(X - 1, (long long)seg) * ceilDiv(Y - 1, TY);
   int best = 4;
   long long bestCost = -1;
@@ -886,8 +932,7 @@ static bool grid32Filled(int X, int Y, int Z) {
 // Plane pairs each wave of a persistent fused tiling walks: the tiles it
 // takes times their depth.
 static long long persistentWalk(int X, int Y, int Z, const Tiling& tl) {
-  const void* k = tl.V == 8 ? (const void*)kFused16<6, 4, 2> : (const void*)kFused16<6, 4, 4>;
-  const long long slots = persistentSlots(k, fused16PerCU(tl.V, X));
+  const long long slots = fused16Slots(tl.V, X);
   const long long tiles =
       ceilDiv(X - 1, (long long)kWave * tl.V) * ceilDiv(Y - 1, tl.Ty) * ceilDiv(Z - 1, tl.TP);
   return ceilDiv(tiles, slots) * tl.TP;
@@ -900,8 +945,7 @@ Tiling choose16Tiling(int X, int Y, int Z, int V) {
   tl.V = V == 8 ? 8 : 16;
   tl.Ty = 4;
   tl.ZW = 16;
-  const void* k = tl.V == 8 ? (const void*)kFused16<6, 4, 2> : (const void*)kFused16<6, 4, 4>;
-  tl.TP = persistentDepth(X, Y, Z, 4, kWave * tl.V, k, fused16PerCU(tl.V, X));
+  tl.TP = persistentDepth(X, Y, Z, 4, kWave * tl.V, fused16Slots(tl.V, X));
   return tl;
 }
 
@@ -1056,6 +1100,9 @@ Tiling chooseBpfdTiling(int X, int Y, int Z) {
   return tl;
 }
 
+#endif
+
+#if MKF_PART(1)
 // About 16K waves in all (twice what 256 CUs hold at 32 waves each), each
 // streaming a run of rows of one 1024-voxel segment. (At 1024^3, 32K and 64K
 // waves, and 3 or 4 rows in flight instead of 2, were within 1 % for the map
@@ -1130,9 +1177,14 @@ hipError_t launchPBMMoments(const FieldView& fv, const BinMapF64& m, uint32_t* p
   else
     return hipErrorInvalidValue;
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kMomentsReduce, dim3(1), dim3(256), 0, s, part, pbmGrid(fv).nBlocks, mom);
+  return launchMomentsReduce(part, pbmGrid(fv).nBlocks, mom, s);
+}
+
+hipError_t launchMomentsReduce(const double* part, long long nParts, double* mom, hipStream_t s) {
+  hipLaunchKernelGGL(kMomentsReduce, dim3(1), dim3(256), 0, s, part, nParts, mom);
   return hipGetLastError();
 }
+#endif
 
 template <typename T, int M, int V, int TY>
 static hipError_t launchFusedT(const FieldView& fv, T thr, unsigned long long* bpfd, const TileGrid& g,
@@ -1188,29 +1240,23 @@ static hipError_t dispatchFused(const FieldView& fv, T thr, uint32_t m, unsigned
   return dispatchFusedTY<T, VW>(fv, thr, m, bpfd, g, tl.Ty, s);
 }
 
-// Resident one-wave workgroups of a kernel on the current device, at most
-// capPerCU per CU when capPerCU > 0, rounded down to a multiple of 8
-// (computed once per kernel, device and cap).
-static unsigned persistentSlots(const void* kernel, int capPerCU) {
-  static std::mutex mu;
-  static std::map<std::tuple<const void*, int, int>, unsigned> cache;
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  std::lock_guard<std::mutex> lk(mu);
-  auto it = cache.find({kernel, dev, capPerCU});
-  if (it != cache.end()) return it->second;
-  int perCU = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, kWave, 0) != hipSuccess || perCU < 1) perCU = 1;
-  if (capPerCU > 0) perCU = std::min(perCU, capPerCU);
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 8;
-  const unsigned n = std::max(8u, ((unsigned)(perCU * cus)) / 8u * 8u);
-  cache[{kernel, dev, capPerCU}] = n;
-  return n;
+#if MKF_PART(2)
+hipError_t launchFusedF32(const FieldView& fv, float thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
+                          hipStream_t s) {
+  return dispatchFused<float, 4>(fv, thr, m, bpfd, tl, s);
 }
+#endif
+
+#if MKF_PART(3)
+hipError_t launchFusedF64(const FieldView& fv, double thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
+                          hipStream_t s) {
+  return dispatchFused<double, 2>(fv, thr, m, bpfd, tl, s);
+}
+#endif
 
 // Tiles of the persistent 8- and 16-voxel kernels: 64 V voxels x Ty row
 // pairs x TP plane pairs.
-static TileGrid grid16(int X, int Y, int Z, const Tiling& tl) {
+static inline TileGrid grid16(int X, int Y, int Z, const Tiling& tl) {
   TileGrid g{};
   g.nSegX = (int)ceilDiv(X - 1, (long long)kWave * tl.V);
   g.waves = 1;
@@ -1225,7 +1271,7 @@ static TileGrid grid16(int X, int Y, int Z, const Tiling& tl) {
 
 // One wave per resident slot of the kernel (rounded down to a multiple of 8
 // XCDs), no more than there are tiles (rounded up to 8).
-static unsigned blocks16(const TileGrid& g, const Tiling& tl, const void* kernel, int capPerCU) {
+static inline unsigned blocks16(const TileGrid& g, const Tiling& tl, const void* kernel, int capPerCU) {
   unsigned nBlocks = std::min<unsigned>(persistentSlots(kernel, capPerCU), (unsigned)g.chunk * 8u);
   // A workgroup's histogram and cell count are 32-bit: no workgroup takes
   // more than 2^31 cells (only fields of far more than 2^40 cells need more
@@ -1235,6 +1281,12 @@ static unsigned blocks16(const TileGrid& g, const Tiling& tl, const void* kernel
   const long long need = ceilDiv(ceilDiv((long long)g.chunk * 8, maxTiles), 8) * 8;
   if (need > (long long)nBlocks) nBlocks = (unsigned)std::min<long long>(need, (long long)g.chunk * 8);
   return nBlocks;
+}
+
+#if MKF_PART(4)
+unsigned fused16Slots(int V, int X) {
+  const void* k = V == 8 ? (const void*)kFused16<6, 4, 2> : (const void*)kFused16<6, 4, 4>;
+  return persistentSlots(k, fused16PerCU(V, X));
 }
 
 template <int M, int TY, int NG>
@@ -1256,8 +1308,8 @@ static hipError_t launchFused16M(const FieldView& fv, float thr, unsigned long l
   return tl.Ty == 2 ? launchFused16K<M, 2, 4>(fv, thr, bpfd, tl, s) : launchFused16K<M, 4, 4>(fv, thr, bpfd, tl, s);
 }
 
-static hipError_t launchFused16(const FieldView& fv, float thr, uint32_t m, unsigned long long* bpfd,
-                                const Tiling& tl, hipStream_t s) {
+hipError_t launchFused16(const FieldView& fv, float thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
+                         hipStream_t s) {
   switch (m & 7u) {
     case 0: return launchFused16M<0>(fv, thr, bpfd, tl, s);
     case 1: return launchFused16M<1>(fv, thr, bpfd, tl, s);
@@ -1270,6 +1322,9 @@ static hipError_t launchFused16(const FieldView& fv, float thr, uint32_t m, unsi
   }
 }
 
+#endif
+
+#if MKF_PART(0)
 hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, const Tiling& tl,
                        hipStream_t s) {
   if (fv.X < 2 || fv.Y < 2 || fv.Z < 2) return hipSuccess;  // no cells
@@ -1279,11 +1334,13 @@ hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long lo
     return launchFused16(fv, (float)m.t[0], m.m, bpfd, tl, s);
   // f32 fields compare against the threshold rounded to float
   // (src/binMapCUDA.cuh:138-139).
-  if (fv.enc == ENC_F32) return dispatchFused<float, 4>(fv, (float)m.t[0], m.m, bpfd, tl, s);
-  if (fv.enc == ENC_F64) return dispatchFused<double, 2>(fv, m.t[0], m.m, bpfd, tl, s);
+  if (fv.enc == ENC_F32) return launchFusedF32(fv, (float)m.t[0], m.m, bpfd, tl, s);
+  if (fv.enc == ENC_F64) return launchFusedF64(fv, m.t[0], m.m, bpfd, tl, s);
   return hipErrorInvalidValue;
 }
+#endif
 
+#if MKF_PART(5)
 // Fused histogram + phase moments: 4 row pairs per wave (the tiling's slab
 // depth, wave count and voxels per lane are kept).
 template <typename T, int M, int V>
@@ -1350,10 +1407,11 @@ hipError_t launchFusedMoments(const FieldView& fv, const BinMapF64& m, unsigned 
   else
     return hipErrorInvalidValue;
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kMomentsReduce, dim3(1), dim3(256), 0, s, part, nParts, mom);
-  return hipGetLastError();
+  return launchMomentsReduce(part, nParts, mom, s);
 }
+#endif
 
+#if MKF_PART(6)
 hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
                       unsigned long long* bpfd, const Tiling& tl0, hipStream_t s) {
   if (X < 2 || Y < 2 || Z < 2) return hipSuccess;
@@ -1387,5 +1445,6 @@ hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, u
   }
   return hipGetLastError();
 }
+#endif
 
 }  // namespace mkf
