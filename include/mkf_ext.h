@@ -43,6 +43,13 @@ typedef struct {
 /* Text of the last error raised by any entry point in this thread. */
 const char *mkfLastError(void);
 
+/* Test hook for error paths (no counterpart in the reference): the nth call
+ * from now (1 = the next) of the named internal step fails as an allocation
+ * or runtime error would, once; nth 0 disarms the point. Points:
+ * "batch.upload" (a batch's work-queue upload) and "batch.capture" (its
+ * graph capture). Returns MKF_OK, or MKF_ERR_ARG for an unknown point. */
+int mkfDebugInjectFault(const char *point, int nth);
+
 /* Fused threshold + histogram: ADDS the BPFD of the field (device pointers in
  * pI, as for binMapCUDA) to pBPFD (device, 256 x uint64), reading each field
  * byte from HBM once and writing no packed map. pO (optional) receives the
@@ -90,6 +97,25 @@ int mkfBPFDAsync(size_t *pBPFD, const BMOrg *pO, const BMPackWord *pW, const Mkf
  * depends on the current device: its CU count and how many workgroups of the
  * persistent kernels it holds at once. */
 MkfTiling mkfDefaultTiling(const FieldDef def[3], int elemBytes);
+
+/* Geometry of a histogram launch: its tiles, the workgroups launched, the
+ * waves of each, and the tiling the kernel runs with (after the adjustments
+ * the library makes, e.g. one voxel per lane for several fields). The
+ * persistent kernels (voxelsPerLane 16 and 8) launch at most one workgroup
+ * per resident slot, and workgroup b walks tiles b, b + workgroups, ... */
+typedef struct {
+  int64_t tiles;
+  int64_t workgroups;
+  int32_t wavesPerGroup;
+  MkfTiling used;
+} MkfLaunchInfo;
+
+/* What mkfFusedBPFD would launch for this field, relation (pM NULL: ">=")
+ * and tiling (pT NULL: the tiling a call without one uses). Launches
+ * nothing; needs the current device for the resident-slot count. */
+int mkfFusedLaunchInfo(const BMFieldInfo *pI, const BinMapF64 *pM, const MkfTiling *pT, MkfLaunchInfo *pInfo);
+/* The same for mkfBPFDAsync on a PBM of organisation pO. */
+int mkfBPFDLaunchInfo(const BMOrg *pO, const MkfTiling *pT, MkfLaunchInfo *pInfo);
 
 /* Times the fused kernel on this field (the caller's data, a scratch
  * histogram) under a few tilings, keeps the fastest for fields of this shape
@@ -220,10 +246,12 @@ typedef struct MkfServer MkfServer;
 /* Starts a server on the current device (cfg may be NULL). NULL on error. */
 MkfServer *mkfServeCreate(const MkfServeConfig *cfg);
 /* Queues a request; thread-safe and non-blocking. A host field larger than
- * the budget is served as z-slabs of at most a quarter of the budget that
- * share one plane (each an internal request admitted, copied and batched like
- * any other); its result is reported once, after the last slab, with the
- * summed histograms. MKF_ERR_BUDGET only if two planes exceed the budget.
+ * the budget is served as z-slabs that share one plane, of at most a quarter
+ * of the budget (or, when two planes exceed a quarter, of as many planes as
+ * the budget holds); each slab is an internal request admitted, copied and
+ * batched like any other, and the result is reported once, after the last
+ * slab, with the summed histograms. MKF_ERR_BUDGET only if two planes exceed
+ * the budget.
  * The field, and for a device field its memory, must stay valid and
  * unchanged until the result is reported. */
 int mkfServeSubmit(MkfServer *s, const MkfServeRequest *r);

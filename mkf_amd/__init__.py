@@ -69,6 +69,11 @@ class MkfTiling(C.Structure):
                 ("voxelsPerLane", C.c_int32)]
 
 
+class MkfLaunchInfo(C.Structure):
+    _fields_ = [("tiles", C.c_int64), ("workgroups", C.c_int64), ("wavesPerGroup", C.c_int32),
+                ("used", MkfTiling)]
+
+
 class MkfBatchField(C.Structure):
     _fields_ = [("pF", C.c_void_p), ("enc", C.c_uint8), ("def_", C.c_int * 3), ("nLevels", C.c_int32),
                 ("pM", C.POINTER(BinMapF64)), ("pBPFD", C.c_void_p)]
@@ -136,6 +141,10 @@ _SIGNATURES = {
     "volFrac": (C.c_float, [_SZP]),
     "mkfCroftonMeasuresBPFD": (C.c_int, [C.POINTER(C.c_double), _SZP, C.c_double]),
     "mkfLastError": (C.c_char_p, []),
+    "mkfDebugInjectFault": (C.c_int, [C.c_char_p, C.c_int]),
+    "mkfFusedLaunchInfo": (C.c_int, [C.POINTER(BMFieldInfo), C.POINTER(BinMapF64), C.POINTER(MkfTiling),
+                                     C.POINTER(MkfLaunchInfo)]),
+    "mkfBPFDLaunchInfo": (C.c_int, [C.POINTER(BMOrg), C.POINTER(MkfTiling), C.POINTER(MkfLaunchInfo)]),
     "mkfFusedBPFD": (C.c_int, [C.POINTER(KernInfo), _P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo),
                                C.POINTER(BinMapF64), C.POINTER(MkfTiling), _P]),
     "mkfPBMAsync": (C.c_int, [_P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo), C.POINTER(BinMapF64), _P]),

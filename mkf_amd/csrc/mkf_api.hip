@@ -12,6 +12,7 @@
 
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -45,6 +46,24 @@ int mkf::setError(int code, const char* fmt, ...) {
   va_end(ap);
   if (getenv("MKF_VERBOSE")) fprintf(stderr, "[mkf] %s\n", gErr);
   return code;
+}
+
+namespace {
+
+// Armed fault points: name -> calls left before the one that fails.
+std::mutex gFaultLock;
+std::map<std::string, int> gFaults;
+
+}  // namespace
+
+bool mkf::faultDue(const char* point) {
+  std::lock_guard<std::mutex> g(gFaultLock);
+  if (gFaults.empty()) return false;
+  auto it = gFaults.find(point);
+  if (it == gFaults.end()) return false;
+  if (--it->second > 0) return false;
+  gFaults.erase(it);
+  return true;
 }
 
 namespace {
@@ -180,6 +199,20 @@ extern "C" {
 
 const char* mkfLastError(void) { return gErr; }
 
+int mkfDebugInjectFault(const char* point, int nth) {
+  if (!point || nth < 0) return fail(MKF_ERR_ARG, "fault injection: NULL point or negative count");
+  static const char* const known[] = {"batch.upload", "batch.capture"};
+  bool ok = false;
+  for (const char* k : known) ok |= strcmp(k, point) == 0;
+  if (!ok) return fail(MKF_ERR_ARG, "fault injection: unknown point '%s'", point);
+  std::lock_guard<std::mutex> g(gFaultLock);
+  if (nth == 0)
+    gFaults.erase(point);
+  else
+    gFaults[point] = nth;
+  return MKF_OK;
+}
+
 MkfTiling mkfDefaultTiling(const FieldDef def[3], int elemBytes) {
   const Tiling tl = chooseTiling(def[0], def[1], def[2], elemBytes);
   return MkfTiling{tl.TP, tl.ZW, tl.Ty, tl.V};
@@ -221,18 +254,34 @@ int mkfPBMMomentsAsync(BMPackWord* pW, double* pMom, BMOrg* pO, const BMFieldInf
   return ef == hipSuccess ? MKF_OK : hipFail(ef, "moments scratch free");
 }
 
-int mkfBPFDAsync(size_t* pBPFD, const BMOrg* pO, const BMPackWord* pW, const MkfTiling* pT, void* stream) {
-  if (!pBPFD || !pO || !pW) return fail(MKF_ERR_ARG, "NULL argument");
-  const int X = (int)pO->rowElem, Y = (int)pO->rowPairs + 1, Z = (int)pO->planePairs + 1;
-  // 16 voxels per lane (half words), 4 (nibbles of the PBM words) or 1, as
-  // asked; 0 takes the built-in choice's (4 where it is 16 and the rows
-  // asked for are 8).
+namespace {
+
+// The PBM histogram's tiling: 16 voxels per lane (half words), 4 (nibbles of
+// the PBM words) or 1, as asked; 0 takes the built-in choice's (4 where it
+// is 16 and the rows asked for are 8).
+Tiling bpfdTiling(const MkfTiling* pT, int X, int Y, int Z) {
   Tiling tl = chooseBpfdTiling(X, Y, Z);
   if (pT) {
     const int v = pT->voxelsPerLane;
     tl = Tiling{pT->planePairs, pT->waves, pT->rowPairs, v == 1 || v == 16 ? v : v == 0 ? tl.V : 4};
     if (v == 0 && tl.V == 16 && tl.Ty == 8) tl.V = 4;
   }
+  return tl;
+}
+
+void toLaunchInfo(const LaunchInfo& li, MkfLaunchInfo* out) {
+  out->tiles = li.tiles;
+  out->workgroups = li.workgroups;
+  out->wavesPerGroup = li.wavesPerGroup;
+  out->used = MkfTiling{li.used.TP, li.used.ZW, li.used.Ty, li.used.V};
+}
+
+}  // namespace
+
+int mkfBPFDAsync(size_t* pBPFD, const BMOrg* pO, const BMPackWord* pW, const MkfTiling* pT, void* stream) {
+  if (!pBPFD || !pO || !pW) return fail(MKF_ERR_ARG, "NULL argument");
+  const int X = (int)pO->rowElem, Y = (int)pO->rowPairs + 1, Z = (int)pO->planePairs + 1;
+  const Tiling tl = bpfdTiling(pT, X, Y, Z);
   int rc = checkTiling(tl, 4);
   if (rc) return rc;
   const hipError_t e = launchBPFD(pW, X, Y, Z, pO->rowWS, pO->planeWS, (unsigned long long*)pBPFD, tl,
@@ -263,6 +312,37 @@ int mkfFusedBPFD(KernInfo* pK, size_t* pBPFD, BMOrg* pO, const BMFieldInfo* pI, 
   }
   const hipError_t e = launchFused(fv, *pM, (unsigned long long*)pBPFD, tl, s);
   return e == hipSuccess ? MKF_OK : hipFail(e, "fused launch");
+}
+
+int mkfFusedLaunchInfo(const BMFieldInfo* pI, const BinMapF64* pM, const MkfTiling* pT, MkfLaunchInfo* pInfo) {
+  if (!pInfo) return fail(MKF_ERR_ARG, "NULL launch info");
+  FieldView fv;
+  FieldDef def[3];
+  int rc = resolveField(pI, &fv, def);
+  if (rc) return rc;
+  const Tiling tl = fusedTiling(pT, fv);
+  rc = checkTiling(tl, fv.enc & ENC_MASK_NUM);
+  if (rc) return rc;
+  BinMapF64 m{};
+  m.m = 6;
+  LaunchInfo li;
+  const hipError_t e = launchFused(fv, pM ? *pM : m, nullptr, tl, 0, &li);
+  if (e != hipSuccess) return hipFail(e, "fused launch geometry");
+  toLaunchInfo(li, pInfo);
+  return MKF_OK;
+}
+
+int mkfBPFDLaunchInfo(const BMOrg* pO, const MkfTiling* pT, MkfLaunchInfo* pInfo) {
+  if (!pO || !pInfo) return fail(MKF_ERR_ARG, "NULL argument");
+  const int X = (int)pO->rowElem, Y = (int)pO->rowPairs + 1, Z = (int)pO->planePairs + 1;
+  const Tiling tl = bpfdTiling(pT, X, Y, Z);
+  int rc = checkTiling(tl, 4);
+  if (rc) return rc;
+  LaunchInfo li;
+  const hipError_t e = launchBPFD(nullptr, X, Y, Z, pO->rowWS, pO->planeWS, nullptr, tl, 0, &li);
+  if (e != hipSuccess) return hipFail(e, "histogram launch geometry");
+  toLaunchInfo(li, pInfo);
+  return MKF_OK;
 }
 
 int mkfFusedBPFDMoments(size_t* pBPFD, double* pMom, BMOrg* pO, const BMFieldInfo* pI, const BinMapF64* pM,

@@ -536,11 +536,15 @@ struct MkfBatch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t cap = nullptr;
+  // Set when a re-plan failed and the previous plan could not be restored:
+  // the device queue and the graph may then disagree, so runs are refused.
+  bool broken = false;
 };
 
 namespace {
 
 int upload(MkfBatch* B, const MkfBatchShape& s, hipStream_t st) {
+  if (faultDue("batch.upload")) return bfail(MKF_ERR_MEM, "work queue allocation (injected fault)");
   for (auto& b : B->host) shapeField(b, s);
   std::vector<BatchItem> items;
   buildItems(B->host, items);
@@ -574,6 +578,7 @@ int capture(MkfBatch* B) {
   if (B->graph) (void)hipGraphDestroy(B->graph);
   B->exec = nullptr;
   B->graph = nullptr;
+  if (faultDue("batch.capture")) return bfail(MKF_ERR_HIP, "stream capture (injected fault)");
   if (hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return bfail(MKF_ERR_HIP, "stream capture begin");
   const hipError_t e = record(B, B->cap);
@@ -635,6 +640,7 @@ MkfBatch* mkfBatchCreate(const MkfBatchField* fields, int nFields, const MkfBatc
 }
 
 int mkfBatchRun(MkfBatch* B, void* stream) {
+  if (B && B->broken) return bfail(MKF_ERR_ARG, "unusable after a failed re-plan; destroy it");
   if (!B || !B->exec) return bfail(MKF_ERR_ARG, "no batch");
   const hipError_t e = hipGraphLaunch(B->exec, (hipStream_t)stream);
   return e == hipSuccess ? MKF_OK : bfail(MKF_ERR_HIP, "graph launch");
@@ -642,13 +648,16 @@ int mkfBatchRun(MkfBatch* B, void* stream) {
 
 int mkfBatchRunDirect(MkfBatch* B, void* stream) {
   if (!B) return bfail(MKF_ERR_ARG, "no batch");
+  if (B->broken) return bfail(MKF_ERR_ARG, "unusable after a failed re-plan; destroy it");
   return record(B, (hipStream_t)stream) == hipSuccess ? MKF_OK : bfail(MKF_ERR_HIP, "batch launch");
 }
 
 int mkfBatchAutotune(MkfBatch* B, const MkfBatchShape* candidates, int nCandidates, int reps,
                      MkfBatchShape* chosen, void* stream) {
   if (!B || !candidates || nCandidates <= 0) return bfail(MKF_ERR_ARG, "no candidates");
+  if (B->broken) return bfail(MKF_ERR_ARG, "unusable after a failed re-plan; destroy it");
   hipStream_t st = (hipStream_t)stream;
+  const MkfBatchShape prev = B->shape;
   // Point every field at a scratch histogram while tuning.
   size_t total = 0;
   for (auto& b : B->host) total += (size_t)b.nLevels * 256;
@@ -693,9 +702,18 @@ int mkfBatchAutotune(MkfBatch* B, const MkfBatchShape* candidates, int nCandidat
   for (size_t f = 0; f < B->host.size(); ++f) B->host[f].out = saved[f];
   B->flags = saveFlags;
   (void)hipFree(scratch);
+  if (rc == MKF_OK && best.planePairs <= 0) rc = bfail(MKF_ERR_ARG, "no usable candidate");
+  // Plan the fastest shape, or, after an error, the previous one again: the
+  // tuning runs replaced the device queue (and may have reallocated it), so
+  // the queue and the graph are rebuilt either way. If that fails too, the
+  // batch refuses further runs.
+  int rp = upload(B, rc == MKF_OK ? best : prev, st);
+  if (rp == MKF_OK) rp = capture(B);
+  if (rp != MKF_OK) {
+    B->broken = true;
+    return rc != MKF_OK ? rc : rp;
+  }
   if (rc != MKF_OK) return rc;
-  if (best.planePairs <= 0) return bfail(MKF_ERR_ARG, "no usable candidate");
-  if ((rc = upload(B, best, st)) != MKF_OK || (rc = capture(B)) != MKF_OK) return rc;
   if (chosen) *chosen = best;
   return MKF_OK;
 }

@@ -45,9 +45,22 @@ struct Tiling {
   int V;   // voxels per lane: 1, or 16 bytes' worth (4 floats / 2 doubles)
 };
 
+// Geometry of a histogram launch: tiles, workgroups launched (for the
+// persistent kernels, each walks tiles b, b + workgroups, ...), waves per
+// workgroup and the tiling the kernel runs with.
+struct LaunchInfo {
+  long long tiles, workgroups;
+  int wavesPerGroup;
+  Tiling used;
+};
+
 // Records the text of an error for mkfLastError() (this thread) and returns
 // code.
 int setError(int code, const char* fmt, ...);
+
+// True when the named fault point, armed by mkfDebugInjectFault, is due to
+// fail on this call (counts the calls of an armed point; thread-safe).
+bool faultDue(const char* point);
 
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes);  // fused kernel
 Tiling choose16Tiling(int X, int Y, int Z, int V = 16);     // fused kernel, 8 or 16 voxels per lane (floats)
@@ -68,9 +81,10 @@ hipError_t launchPBMMoments(const FieldView& fv, const BinMapF64& m, uint32_t* p
                             uint32_t planeWS, double* part, double* mom, hipStream_t s);
 
 // Adds the BPFD of fv thresholded by m to bpfd (device, 256 x u64), reading
-// the field once; no PBM is written.
+// the field once; no PBM is written. With info, nothing is launched and
+// info receives the launch's geometry.
 hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, const Tiling& tl,
-                       hipStream_t s);
+                       hipStream_t s, LaunchInfo* info = nullptr);
 
 // As launchFused, and ADDS the phase moments of fv (as launchPBMMoments) to
 // mom, reading the field once. Needs X, Y, Z >= 2 (hipErrorInvalidValue
@@ -80,9 +94,10 @@ long long fusedMomentParts(const FieldView& fv, const Tiling& tl);
 hipError_t launchFusedMoments(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, double* part,
                               double* mom, const Tiling& tl, hipStream_t s);
 
-// Adds the BPFD of the PBM pW (dims X, Y, Z) to bpfd.
+// Adds the BPFD of the PBM pW (dims X, Y, Z) to bpfd (with info: only the
+// geometry, as launchFused).
 hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
-                      unsigned long long* bpfd, const Tiling& tl, hipStream_t s);
+                      unsigned long long* bpfd, const Tiling& tl, hipStream_t s, LaunchInfo* info = nullptr);
 
 // Copies bytes (a multiple of 16) with a kernel on s; either side may be
 // the device address of mapped pinned host memory.

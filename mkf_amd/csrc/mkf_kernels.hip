@@ -743,7 +743,7 @@ hipError_t launchFusedF32(const FieldView& fv, float thr, uint32_t m, unsigned l
 hipError_t launchFusedF64(const FieldView& fv, double thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
                           hipStream_t s);  // part 3
 hipError_t launchFused16(const FieldView& fv, float thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
-                         hipStream_t s);  // part 4
+                         hipStream_t s, LaunchInfo* info);  // part 4
 unsigned fused16Slots(int V, int X);      // part 4: resident slots of kFused16 with V voxels per lane
 
 // Kernel view of a field description; unused table slots repeat field 0.
@@ -1229,11 +1229,26 @@ static hipError_t dispatchFusedTY(const FieldView& fv, T thr, uint32_t m, unsign
   }
 }
 
+// The effective tiling of the workgroup kernels for fv: several fields, an x
+// stride, V >= 8 (the persistent kernels' widths) or rows shorter than V run
+// one voxel per lane (the general form gathers one voxel per lane).
+static inline Tiling groupTiling(const FieldView& fv, const Tiling& tl0) {
+  Tiling tl = tl0;
+  if (fv.nF > 1 || fv.sx != 1 || tl.V >= 8 || fv.X < tl.V) tl.V = 1;
+  return tl;
+}
+
+static inline void fillInfo(LaunchInfo* info, const TileGrid& g, long long groups, const Tiling& tl) {
+  info->tiles = g.nTiles;
+  info->workgroups = g.nTiles > 0 ? groups : 0;
+  info->wavesPerGroup = g.waves;
+  info->used = tl;
+}
+
 template <typename T, int VW>
 static hipError_t dispatchFused(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl0,
                                 hipStream_t s) {
-  Tiling tl = tl0;
-  if (fv.nF > 1 || fv.sx != 1 || tl.V >= 8 || fv.X < tl.V) tl.V = 1;  // the general form gathers one voxel per lane
+  const Tiling tl = groupTiling(fv, tl0);
   const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, tl);
   if (g.nTiles <= 0) return hipSuccess;
   if (tl.V == 1) return dispatchFusedTY<T, 1>(fv, thr, m, bpfd, g, tl.Ty, s);
@@ -1291,10 +1306,14 @@ unsigned fused16Slots(int V, int X) {
 
 template <int M, int TY, int NG>
 static hipError_t launchFused16K(const FieldView& fv, float thr, unsigned long long* bpfd, const Tiling& tl,
-                                 hipStream_t s) {
+                                 hipStream_t s, LaunchInfo* info) {
   const TileGrid g = grid16(fv.X, fv.Y, fv.Z, tl);
-  if (g.nTiles <= 0) return hipSuccess;
+  if (g.nTiles <= 0 && !info) return hipSuccess;
   const unsigned nBlocks = blocks16(g, tl, (const void*)kFused16<M, TY, NG>, fused16PerCU(4 * NG, fv.X));
+  if (info) {
+    fillInfo(info, g, nBlocks, tl);
+    return hipSuccess;
+  }
   hipLaunchKernelGGL((kFused16<M, TY, NG>), dim3(nBlocks), dim3(kWave), 0, s, (const float*)fv.p[0], thr, fv.X,
                      fv.Y, fv.Z, fv.sy, fv.sz, g, bpfd);
   return hipGetLastError();
@@ -1302,23 +1321,25 @@ static hipError_t launchFused16K(const FieldView& fv, float thr, unsigned long l
 
 template <int M>
 static hipError_t launchFused16M(const FieldView& fv, float thr, unsigned long long* bpfd, const Tiling& tl,
-                                 hipStream_t s) {
+                                 hipStream_t s, LaunchInfo* info) {
   if (tl.V == 8)
-    return tl.Ty == 2 ? launchFused16K<M, 2, 2>(fv, thr, bpfd, tl, s) : launchFused16K<M, 4, 2>(fv, thr, bpfd, tl, s);
-  return tl.Ty == 2 ? launchFused16K<M, 2, 4>(fv, thr, bpfd, tl, s) : launchFused16K<M, 4, 4>(fv, thr, bpfd, tl, s);
+    return tl.Ty == 2 ? launchFused16K<M, 2, 2>(fv, thr, bpfd, tl, s, info)
+                      : launchFused16K<M, 4, 2>(fv, thr, bpfd, tl, s, info);
+  return tl.Ty == 2 ? launchFused16K<M, 2, 4>(fv, thr, bpfd, tl, s, info)
+                    : launchFused16K<M, 4, 4>(fv, thr, bpfd, tl, s, info);
 }
 
 hipError_t launchFused16(const FieldView& fv, float thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
-                         hipStream_t s) {
+                         hipStream_t s, LaunchInfo* info) {
   switch (m & 7u) {
-    case 0: return launchFused16M<0>(fv, thr, bpfd, tl, s);
-    case 1: return launchFused16M<1>(fv, thr, bpfd, tl, s);
-    case 2: return launchFused16M<2>(fv, thr, bpfd, tl, s);
-    case 3: return launchFused16M<3>(fv, thr, bpfd, tl, s);
-    case 4: return launchFused16M<4>(fv, thr, bpfd, tl, s);
-    case 5: return launchFused16M<5>(fv, thr, bpfd, tl, s);
-    case 6: return launchFused16M<6>(fv, thr, bpfd, tl, s);
-    default: return launchFused16M<7>(fv, thr, bpfd, tl, s);
+    case 0: return launchFused16M<0>(fv, thr, bpfd, tl, s, info);
+    case 1: return launchFused16M<1>(fv, thr, bpfd, tl, s, info);
+    case 2: return launchFused16M<2>(fv, thr, bpfd, tl, s, info);
+    case 3: return launchFused16M<3>(fv, thr, bpfd, tl, s, info);
+    case 4: return launchFused16M<4>(fv, thr, bpfd, tl, s, info);
+    case 5: return launchFused16M<5>(fv, thr, bpfd, tl, s, info);
+    case 6: return launchFused16M<6>(fv, thr, bpfd, tl, s, info);
+    default: return launchFused16M<7>(fv, thr, bpfd, tl, s, info);
   }
 }
 
@@ -1326,12 +1347,19 @@ hipError_t launchFused16(const FieldView& fv, float thr, uint32_t m, unsigned lo
 
 #if MKF_PART(0)
 hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, const Tiling& tl,
-                       hipStream_t s) {
+                       hipStream_t s, LaunchInfo* info) {
+  if (info) *info = LaunchInfo{0, 0, 0, tl};
   if (fv.X < 2 || fv.Y < 2 || fv.Z < 2) return hipSuccess;  // no cells
   const int eb = fv.enc & ENC_MASK_NUM;
   if (!tilingSupported(tl, eb)) return hipErrorInvalidValue;
   if ((tl.V == 16 || tl.V == 8) && fv.enc == ENC_F32 && fv.nF == 1 && fv.sx == 1 && fv.X >= 4)
-    return launchFused16(fv, (float)m.t[0], m.m, bpfd, tl, s);
+    return launchFused16(fv, (float)m.t[0], m.m, bpfd, tl, s, info);
+  if (info) {
+    const Tiling gt = groupTiling(fv, tl);
+    const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, gt);
+    fillInfo(info, g, (long long)g.chunk * 8, gt);
+    return hipSuccess;
+  }
   // f32 fields compare against the threshold rounded to float
   // (src/binMapCUDA.cuh:138-139).
   if (fv.enc == ENC_F32) return launchFusedF32(fv, (float)m.t[0], m.m, bpfd, tl, s);
@@ -1413,15 +1441,20 @@ hipError_t launchFusedMoments(const FieldView& fv, const BinMapF64& m, unsigned 
 
 #if MKF_PART(6)
 hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
-                      unsigned long long* bpfd, const Tiling& tl0, hipStream_t s) {
+                      unsigned long long* bpfd, const Tiling& tl0, hipStream_t s, LaunchInfo* info) {
+  if (info) *info = LaunchInfo{0, 0, 0, tl0};
   if (X < 2 || Y < 2 || Z < 2) return hipSuccess;
   // tl.V is 16 (half words), 4 (nibbles of the PBM words) or 1.
   const Tiling& tl = tl0;
   if (!tilingSupported(tl, 4)) return hipErrorInvalidValue;
   if (tl.V == 16) {
     const TileGrid g = grid16(X, Y, Z, tl);
-    if (g.nTiles <= 0) return hipSuccess;
+    if (g.nTiles <= 0 && !info) return hipSuccess;
     const unsigned nBlocks = blocks16(g, tl, tl.Ty == 2 ? (const void*)kBPFD16<2> : (const void*)kBPFD16<4>, 0);
+    if (info) {
+      fillInfo(info, g, nBlocks, tl);
+      return hipSuccess;
+    }
     if (tl.Ty == 2)
       hipLaunchKernelGGL((kBPFD16<2>), dim3(nBlocks), dim3(kWave), 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd);
     else
@@ -1429,6 +1462,10 @@ hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, u
     return hipGetLastError();
   }
   const TileGrid g = makeGrid(X, Y, Z, tl);
+  if (info) {
+    fillInfo(info, g, (long long)g.chunk * 8, tl);
+    return hipSuccess;
+  }
   const dim3 grid((unsigned)g.chunk * 8u), block(g.waves * kWave);
   if (tl.V == 4) {
     switch (tl.Ty) {

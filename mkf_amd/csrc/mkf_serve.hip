@@ -478,13 +478,17 @@ int mkfServeSubmit(MkfServer* S, const MkfServeRequest* r) {
   q->bytes = (size_t)d[0] * d[1] * d[2] * eb;
   std::vector<Request*> parts{q};
   if (!(r->flags & MKF_SERVE_ON_DEVICE) && alignUp(q->bytes) > S->arena.capacity()) {
-    // Split along the slowest non-trivial axis into slabs of at most a
-    // quarter of the budget that share one plane.
+    // Split along the slowest non-trivial axis into slabs that share one
+    // plane: of at most a quarter of the budget, so several slabs are on the
+    // device at once, or, when two planes exceed a quarter, of as many planes
+    // as the whole budget holds.
     const int k = nd - 1;
     size_t planeBytes = eb;
     for (int i = 0; i < k; ++i) planeBytes *= (size_t)d[i];
     const long long planes = d[k];
-    const long long sp = std::min<long long>(planes - 1, (long long)(S->arena.capacity() / 4 / planeBytes) - 1);
+    const long long cap = (long long)S->arena.capacity();
+    long long sp = std::min<long long>(planes - 1, cap / 4 / (long long)planeBytes - 1);
+    if (sp < 1) sp = std::min<long long>(planes - 1, cap / (long long)planeBytes - 1);
     if (planes < 2 || sp < 1 || alignUp(planeBytes * (size_t)(sp + 1)) > S->arena.capacity()) {
       delete q;
       return setError(MKF_ERR_BUDGET, "serve: field of %zu bytes exceeds the budget of %zu, and so do two planes",

@@ -4,7 +4,9 @@ seeded request stream used to exercise them.
 Server.submit() takes a (Z, Y, X) float32/float64 tensor: a CUDA tensor is
 read in place (the request is device-resident), a CPU tensor is copied in by
 the server (pin it for the copy to overlap). The wrapper keeps the field, the
-threshold maps and the output array alive until the result is polled."""
+threshold maps and the output array alive until the result is polled. Each
+request travels under an id of its own, so a caller's tag may repeat while
+earlier requests with that tag are outstanding."""
 
 import ctypes as C
 import math
@@ -27,7 +29,7 @@ class Server:
         self.handle = self.lib.mkfServeCreate(C.byref(cfg))
         if not self.handle:
             raise RuntimeError(f"mkfServeCreate failed: {last_error()}")
-        self._live = {}
+        self._live = {}  # request id -> (caller's tag, field, maps, output)
         self._next = 0
         self._buf = (MkfServeResult * 256)()
 
@@ -41,9 +43,9 @@ class Server:
         for j, lv in enumerate(levels):
             maps[j] = lv if isinstance(lv, BinMapF64) else bin_map(*lv)
         out = np.zeros((len(levels), 256), dtype=np.uint64)
-        if tag is None:
-            tag = self._next
-        self._next = max(self._next, int(tag)) + 1
+        rid = self._next
+        self._next += 1
+        tag = rid if tag is None else int(tag)
         Z, Y, X = field.shape
         r = MkfServeRequest()
         r.pF = field.data_ptr()
@@ -57,13 +59,13 @@ class Server:
         r.nLevels = len(levels)
         r.pM = C.cast(maps, C.POINTER(BinMapF64))
         r.pBPFD = out.ctypes.data
-        r.tag = int(tag)
-        self._live[int(tag)] = (field, maps, out)
+        r.tag = rid
+        self._live[rid] = (tag, field, maps, out)
         rc = self.lib.mkfServeSubmit(self.handle, C.byref(r))
         if rc != 0:
-            del self._live[int(tag)]
+            del self._live[rid]
             raise RuntimeError(f"mkfServeSubmit failed ({rc}): {last_error()}")
-        return int(tag)
+        return tag
 
     def poll(self, wait_us=0):
         """Reported results: a list of dicts (tag, status, hist (levels, 256)
@@ -74,8 +76,8 @@ class Server:
         res = []
         for i in range(n):
             r = self._buf[i]
-            _, _, out = self._live.pop(int(r.tag))
-            res.append({"tag": int(r.tag), "status": int(r.status), "hist": out, "batch": int(r.batchSize),
+            tag, _, _, out = self._live.pop(int(r.tag))
+            res.append({"tag": tag, "status": int(r.status), "hist": out, "batch": int(r.batchSize),
                         "submit": r.tSubmit, "admit": r.tAdmit, "launch": r.tLaunch, "done": r.tDone})
         return res
 

@@ -196,3 +196,40 @@ def test_batch_special_values_every_relation(lib, orc, cuda, dtype):
                 np.testing.assert_array_equal(got(o, len(bms)), ref[i], err_msg=f"segments {segs} field {i}")
         finally:
             lib.mkfBatchDestroy(b)
+
+
+def test_batch_autotune_failure_restores_or_refuses(lib, orc, cuda):
+    """A re-plan that fails part way (forced from the host through the
+    library's fault points) leaves the previous plan in place, so runs still
+    give the oracle's histograms; if restoring the plan fails too, the batch
+    refuses to run instead of reading a stale work queue."""
+    fields, maps, outs, arr = make_batch(cuda, SPECS[2:6], seed=50)
+    ref = oracle_hists(orc, fields, maps)
+    b = lib.mkfBatchCreate(arr, 4, C.byref(bshape(8, 8)), M.MKF_BATCH_ZERO_OUTPUT, None)
+    assert b, M.last_error()
+    cands = (M.MkfBatchShape * 3)(bshape(4, 2), bshape(16, 1), bshape(2, 4))
+    try:
+        M.check(lib.mkfDebugInjectFault(b"batch.upload", 2))  # the second candidate's upload
+        rc = lib.mkfBatchAutotune(b, cands, 3, 1, None, None)
+        assert rc == M.MKF_ERR_MEM and "injected" in M.last_error()
+        n, shp, g = C.c_int(0), M.MkfBatchShape(), C.c_int(0)
+        M.check(lib.mkfBatchInfo(b, C.byref(n), C.byref(shp), C.byref(g)))
+        assert (shp.planePairs, shp.segments) == (8, 8)
+        for graph in (True, False):
+            for o in outs:
+                o.fill_(9)
+            M.check((lib.mkfBatchRun if graph else lib.mkfBatchRunDirect)(b, None), "run after failed tune")
+            torch.cuda.synchronize()
+            for i, (o, bms) in enumerate(zip(outs, maps)):
+                np.testing.assert_array_equal(got(o, len(bms)), ref[i], err_msg=f"graph={graph} field {i}")
+        # first candidate's upload fails, then the restoring capture fails
+        M.check(lib.mkfDebugInjectFault(b"batch.upload", 1))
+        M.check(lib.mkfDebugInjectFault(b"batch.capture", 1))
+        assert lib.mkfBatchAutotune(b, cands, 3, 1, None, None) != M.MKF_OK
+        assert lib.mkfBatchRun(b, None) == M.MKF_ERR_ARG and "unusable" in M.last_error()
+        assert lib.mkfBatchRunDirect(b, None) == M.MKF_ERR_ARG
+        assert lib.mkfBatchAutotune(b, cands, 3, 1, None, None) == M.MKF_ERR_ARG
+    finally:
+        lib.mkfDebugInjectFault(b"batch.upload", 0)
+        lib.mkfDebugInjectFault(b"batch.capture", 0)
+        lib.mkfBatchDestroy(b)

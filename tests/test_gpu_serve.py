@@ -194,3 +194,46 @@ def test_serve_device_snapshots_and_threads(orc, cuda):
             np.testing.assert_array_equal(got[100 + i]["hist"], oracle(orc, h, lv), err_msg=f"host {i}")
     finally:
         S.close()
+
+
+def test_serve_repeated_tags(orc, cuda):
+    """A caller's tag may repeat while requests with it are outstanding:
+    each request keeps its own field, maps and output."""
+    S = Server(budget=64 << 20, max_in_flight=16, max_batch=8)
+    try:
+        fs = [pinned(synth.grf_field_direct((20 + 3 * i, 30, 40), 1.5, seed=60 + i, device=cuda,
+                                            dtype=torch.float32)) for i in range(3)]
+        for i, f in enumerate(fs):
+            S.submit(f, [(">=", 0.1 * k) for k in range(i + 1)], tag=7)  # i + 1 levels identify request i
+        res = []
+        t0 = time.time()
+        while len(res) < 3:
+            res += S.poll(wait_us=20000)
+            assert time.time() - t0 < 60
+        assert S.outstanding == 0
+        for r in res:
+            assert r["tag"] == 7 and r["status"] == 0
+            i = r["hist"].shape[0] - 1
+            np.testing.assert_array_equal(r["hist"], oracle(orc, fs[i], [(">=", 0.1 * k) for k in range(i + 1)]))
+    finally:
+        S.close()
+
+
+def test_serve_split_when_two_planes_exceed_a_quarter(orc, cuda):
+    """Two planes larger than a quarter of the budget but within it: the
+    field is split into slabs of as many planes as the budget holds (here 2),
+    not refused."""
+    budget = 2_000_000
+    f = synth.grf_field_direct((12, 300, 300), 2.0, seed=71, device=cuda, dtype=torch.float64)
+    plane = 300 * 300 * 8
+    assert budget / 4 < 2 * plane <= budget < f.numel() * 8
+    S = Server(budget=budget, max_in_flight=8, max_batch=4)
+    try:
+        lv = [(">=", 0.0), ("<", -0.4)]
+        S.submit(pinned(f), lv, tag=5)
+        r = wait_all(S, 1)[5]
+        assert r["status"] == 0
+        np.testing.assert_array_equal(r["hist"], oracle(orc, f, lv))
+        assert S.stats()["heldMax"] <= budget + 256
+    finally:
+        S.close()

@@ -272,3 +272,12 @@ def test_default_tiling_choice(lib, dims, eb, want):
     if waves is not None:
         assert t.waves == waves
     assert 1 <= t.planePairs <= 65536
+
+
+def test_fault_points_validate_their_arguments(lib):
+    """The error-path test hook accepts only known points (no GPU work)."""
+    assert lib.mkfDebugInjectFault(b"no.such.point", 1) == M.MKF_ERR_ARG
+    assert "unknown point" in M.last_error()
+    assert lib.mkfDebugInjectFault(b"batch.upload", -1) == M.MKF_ERR_ARG
+    assert lib.mkfDebugInjectFault(b"batch.upload", 3) == M.MKF_OK
+    assert lib.mkfDebugInjectFault(b"batch.upload", 0) == M.MKF_OK
