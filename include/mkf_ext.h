@@ -89,6 +89,21 @@ int mkfPBMMomentsAsync(BMPackWord *pW, double *pMom, BMOrg *pO, const BMFieldInf
 int mkfFusedBPFDMoments(size_t *pBPFD, double *pMom, BMOrg *pO, const BMFieldInfo *pI, const BinMapF64 *pM,
                         const MkfTiling *pT, void *stream);
 
+/* As mkfFusedBPFDMoments, but allocating nothing: no hipMalloc, hipFree or
+ * allocating library call. No counterpart in the reference; it is for callers
+ * that cannot allocate (code under HIP graph capture, serving loops and
+ * harnesses that forbid device allocation). The per-workgroup moment
+ * partials live in a static device array of MKF_NOALLOC_SCRATCH_BYTES that
+ * each device holds until the process ends; a launch needing more partials
+ * than fit (MKF_NOALLOC_SCRATCH_BYTES / 48, about 87,000 workgroups; a
+ * 1024^3 float field takes 8,192 with the built-in tiling) fails with
+ * MKF_ERR_BUDGET before any work is queued. Every call on one device shares
+ * that device's array: calls that overlap on one device (other streams or
+ * threads) race, so the caller serialises them. */
+#define MKF_NOALLOC_SCRATCH_BYTES (4 << 20)
+int mkfFusedBPFDMomentsNoAlloc(size_t *pBPFD, double *pMom, BMOrg *pO, const BMFieldInfo *pI, const BinMapF64 *pM,
+                               const MkfTiling *pT, void *stream);
+
 /* Stream-ordered histogram of a packed binary map; adds into pBPFD. */
 int mkfBPFDAsync(size_t *pBPFD, const BMOrg *pO, const BMPackWord *pW, const MkfTiling *pT, void *stream);
 

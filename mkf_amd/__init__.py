@@ -108,6 +108,7 @@ MKF_SERVE_ON_DEVICE = 1
 MKF_SERVE_AFTER_STREAM = 2
 MKF_BATCH_MAX_LEVELS = 16
 MKF_MOM_N = 6
+MKF_NOALLOC_SCRATCH_BYTES = 4 << 20
 MKF_BATCH_ZERO_OUTPUT = 1
 
 _P = C.c_void_p
@@ -151,6 +152,8 @@ _SIGNATURES = {
     "mkfPBMMomentsAsync": (C.c_int, [_P, _P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo), C.POINTER(BinMapF64), _P]),
     "mkfFusedBPFDMoments": (C.c_int, [_P, _P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo), C.POINTER(BinMapF64),
                                       C.POINTER(MkfTiling), _P]),
+    "mkfFusedBPFDMomentsNoAlloc": (C.c_int, [_P, _P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo),
+                                             C.POINTER(BinMapF64), C.POINTER(MkfTiling), _P]),
     "mkfBPFDAsync": (C.c_int, [_P, C.POINTER(BMOrg), _P, C.POINTER(MkfTiling), _P]),
     "mkfDefaultTiling": (MkfTiling, [C.POINTER(C.c_int), C.c_int]),
     "mkfBatchCreate": (_P, [C.POINTER(MkfBatchField), C.c_int, C.POINTER(MkfBatchShape), C.c_int, _P]),

@@ -378,6 +378,71 @@ int mkfFusedBPFDMoments(size_t* pBPFD, double* pMom, BMOrg* pO, const BMFieldInf
   return ef == hipSuccess ? MKF_OK : hipFail(ef, "moments scratch free");
 }
 
+}  // extern "C"
+
+namespace {
+
+// Scratch of the allocation-free moments entry point: the per-workgroup
+// moment partials (6 doubles each). Each device holds its own copy at its own
+// address, looked up once per device with hipGetSymbolAddress; host code
+// passes that address, never the array's name, to kernels.
+constexpr size_t kNoAllocDoubles = MKF_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+std::mutex gScratchLock;
+std::map<int, double*> gScratchPtr;
+
+int scratchOf(double** p) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hipFail(e, "no-alloc scratch: current device");
+  std::lock_guard<std::mutex> g(gScratchLock);
+  auto it = gScratchPtr.find(dev);
+  if (it == gScratchPtr.end()) {
+    double* ptr = nullptr;
+    e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return hipFail(e, "no-alloc scratch: symbol address");
+    it = gScratchPtr.emplace(dev, ptr).first;
+  }
+  *p = it->second;
+  return MKF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mkfFusedBPFDMomentsNoAlloc(size_t* pBPFD, double* pMom, BMOrg* pO, const BMFieldInfo* pI, const BinMapF64* pM,
+                               const MkfTiling* pT, void* stream) {
+  if (!pBPFD || !pMom || !pM) return fail(MKF_ERR_ARG, "NULL histogram, moments or map");
+  FieldView fv;
+  FieldDef def[3];
+  int rc = resolveField(pI, &fv, def);
+  if (rc) return rc;
+  BMOrg org;
+  setBMO(&org, def, 0);
+  if (pO) *pO = org;
+  const Tiling tl = pT ? Tiling{pT->planePairs, pT->waves, pT->rowPairs, pT->voxelsPerLane}
+                       : chooseMomTiling(fv.X, fv.Y, fv.Z, fv.enc & ENC_MASK_NUM);
+  rc = checkTiling(tl, fv.enc & ENC_MASK_NUM);
+  if (rc) return rc;
+  const long long nParts = fusedMomentParts(fv, tl);
+  const long long need = nParts ? nParts : pbmMomentParts(fv);
+  const long long maxParts = (long long)(kNoAllocDoubles / MKF_MOM_N);
+  if (need > maxParts)
+    return fail(MKF_ERR_BUDGET, "no-alloc moments: %lld workgroup partials exceed the %lld that fit the %d-byte scratch",
+                need, maxParts, (int)MKF_NOALLOC_SCRATCH_BYTES);
+  double* part = nullptr;
+  rc = scratchOf(&part);
+  if (rc) return rc;
+  // Every workgroup writes its partial before the reduction reads it, so
+  // the scratch needs no clearing between calls.
+  const hipStream_t s = (hipStream_t)stream;
+  const hipError_t e = nParts ? launchFusedMoments(fv, *pM, (unsigned long long*)pBPFD, part, pMom, tl, s)
+                              : launchPBMMoments(fv, *pM, nullptr, org.rowWS, org.planeWS, part, pMom, s);
+  return e == hipSuccess ? MKF_OK : hipFail(e, "no-alloc fused moments launch");
+}
+
 int mkfFusedAutotune(const BMFieldInfo* pI, const BinMapF64* pM, MkfTiling* pChosen, void* stream) {
   if (!pM) return fail(MKF_ERR_ARG, "NULL map");
   FieldView fv;
