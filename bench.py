@@ -10,9 +10,21 @@ the C ABI. Prints ONE JSON line on stdout; progress goes to stderr.
   roofline         HBM roofline of the fused kernel: algorithmic bytes per
                    launch (voxels x element size) / mean launch time, from HIP
                    events recorded around each launch on the launch stream
+  roofline.measured_roof
+                   the practical read roof: a streaming-read kernel
+                   (mkfReadRoof) over the field's own bytes, best of a small
+                   sweep of load kinds and workgroups per CU
   cpu_baseline     the oracle's C restatement of the reference CPU path
                    (threshold + 64-bit chunk walk) on the same field bytes,
-                   timed on this host with `cores` threads
+                   timed on this host in two modes: 1 thread (the reference
+                   built with gcc) and the host's CPU share (its OpenACC
+                   multicore target), 3 passes each, with the CPU model
+  f64              the same field in float64 (8 GiB): its own autotuned
+                   tiling, timing, roofline and oracle parity
+
+The default run also reports the drop-in pair (binMapCUDA + mkfCUDAGetBPFD),
+phase moments, the host-field stream (PCIe-inclusive), the batched form on
+the uneven 64-field mix, and the serving form on two request streams.
 """
 
 import argparse
@@ -165,7 +177,7 @@ def bench_batch(args, dev, stream):
     }
     if not args.no_cpu:
         from oracle import oracle as orc
-        orc.set_threads(args.cpu_threads or min(16, os.cpu_count() or 1))
+        orc.set_threads(args.cpu_threads or cpu_share())
         t1 = time.perf_counter()
         ok = True
         for i, f in enumerate(fields):
@@ -181,53 +193,21 @@ def bench_batch(args, dev, stream):
     return res
 
 
-def bench_serve(args, dev):
-    """Request streams with Poisson arrivals at several rates through the
-    server (continuous batching, 64 GB budget), beside plain per-request
-    launches of the same requests in arrival order (copy in, one mkfFusedBPFD
-    per level, copy the histograms out, synchronise). Latency is completion
-    minus arrival; throughput is voxel-levels over first arrival to last
+def serve_sweep(lib, dev, fields, reqs, rates, seconds, seed, tag):
+    """One Poisson sweep through the server against plain per-request
+    launches of the same requests in arrival order. Served: submissions at
+    their arrival times, results polled as they come. Plain: per request,
+    copy in if the field is on the host, one mkfFusedBPFD per level, copy
+    the histograms out, synchronise. Latency is completion minus arrival;
+    throughput is voxel-levels from the first arrival to the last
     completion."""
     import numpy as np
     import torch
 
     import mkf_amd as M
-    from mkf_amd import synth
-    from mkf_amd.serve import Server, poisson_arrivals, request_stream
+    from mkf_amd.serve import Server, poisson_arrivals
 
-    lib = M.load()
-    NMAX = args.serve_requests
-    pool_spec, sim_spec, reqs = request_stream(NMAX, seed=args.serve_seed)
-    t0 = time.perf_counter()
-    pool = []
-    for i, (side, dt) in enumerate(pool_spec):
-        f = synth.grf_field_direct((side, side, side), 3.0, seed=7000 + i, device=dev, dtype=dt)
-        h = torch.empty(f.shape, dtype=dt, pin_memory=True)
-        h.copy_(f)
-        pool.append(h)
-        del f
-    g = torch.Generator(device=dev)
-    g.manual_seed(args.serve_seed)
-    # Each simulation keeps a ring of 3 snapshots f(k+1) = f(k) + 0.01 noise;
-    # its requests take them in turn.
-    sims = []
-    for i, (side, dt) in enumerate(sim_spec):
-        ring = [synth.grf_field_direct((side, side, side), 3.0, seed=8000 + i, device=dev, dtype=dt)]
-        for _ in range(2):
-            ring.append(ring[-1] + 0.01 * torch.randn(ring[-1].shape, generator=g, device=dev, dtype=dt))
-        sims.append(ring)
-    fields, turn = [], [0] * len(sims)
-    for r in reqs:
-        if r["sim"] is None:
-            fields.append(pool[r["pool"]])
-        else:
-            s = r["sim"]
-            fields.append(sims[s][turn[s] % 3])
-            turn[s] += 1
-    torch.cuda.synchronize()
-    torch.cuda.empty_cache()
-    log(f"[serve] {NMAX} requests over {len(pool)} pinned host fields and {len(sims)} simulations "
-        f"ready in {time.perf_counter() - t0:.1f}s")
+    NMAX = len(reqs)
     work = [fields[i].numel() * len(r["levels"]) for i, r in enumerate(reqs)]
     maps = [[M.bin_map(*lv) for lv in r["levels"]] for r in reqs]
     stream = torch.cuda.current_stream()
@@ -256,8 +236,14 @@ def bench_serve(args, dev):
         parts = {"queue_ms": [done[k]["admit"] - t_sub - arr[k] for k in range(n)],
                  "copy_ms": [done[k]["launch"] - done[k]["admit"] for k in range(n)],
                  "run_ms": [done[k]["done"] - done[k]["launch"] for k in range(n)]}
+        batches = np.array([done[k]["batch"] for k in range(n)])
+        launches = st1["launches"] - st0["launches"]
         return {"lat": lat, "span": span, "hist": [done[k]["hist"] for k in range(n)],
-                "batch": float(np.mean([done[k]["batch"] for k in range(n)])),
+                "batch": float(np.mean(batches)),
+                "batch_dist": {"p50": float(np.percentile(batches, 50)), "p90": float(np.percentile(batches, 90)),
+                               "max": int(batches.max()), "launches": int(launches),
+                               "requests_per_launch": round(n / max(launches, 1), 2)},
+                "gpu_busy_frac": round((st1["gpuBusy"] - st0["gpuBusy"]) / span, 3),
                 "bytes_in": st1["bytesIn"] - st0["bytesIn"],
                 "parts": {k: round(float(np.mean(v)) * 1e3, 3) for k, v in parts.items()}}
 
@@ -288,9 +274,9 @@ def bench_serve(args, dev):
         return {"lat": np.array(lat), "span": span, "hist": out, "bytes_in": bytes_in}
 
     rows = []
-    for rate in args.serve_rates:
-        n = int(min(NMAX, max(40, rate * args.serve_seconds)))
-        arr = poisson_arrivals(n, rate, seed=args.serve_seed + int(rate))
+    for rate in rates:
+        n = int(min(NMAX, max(40, rate * seconds)))
+        arr = poisson_arrivals(n, rate, seed=seed + int(rate))
         arr = arr - arr[0]
         sv = served(n, arr)
         pl = plain(n, arr)
@@ -306,7 +292,8 @@ def bench_serve(args, dev):
                "served": {"Gvox_levels_s": round(w / sv["span"] / 1e9, 2),
                           "p50_ms": round(float(np.percentile(sv["lat"], 50)) * 1e3, 3),
                           "p99_ms": round(float(np.percentile(sv["lat"], 99)) * 1e3, 3),
-                          "pcie_GBps": round(sv["bytes_in"] / sv["span"] / 1e9, 2), "mean_batch": round(sv["batch"], 2)},
+                          "pcie_GBps": round(sv["bytes_in"] / sv["span"] / 1e9, 2), "mean_batch": round(sv["batch"], 2),
+                          "batch_dist": sv["batch_dist"], "gpu_busy_frac": sv["gpu_busy_frac"]},
                "plain": {"Gvox_levels_s": round(w / pl["span"] / 1e9, 2),
                          "p50_ms": round(float(np.percentile(pl["lat"], 50)) * 1e3, 3),
                          "p99_ms": round(float(np.percentile(pl["lat"], 99)) * 1e3, 3),
@@ -316,20 +303,265 @@ def bench_serve(args, dev):
                "plain_by_origin": {"device": pct(pl["lat"], on_dev), "host": pct(pl["lat"], ~on_dev)},
                "served_mean_parts_ms": sv["parts"]}
         row["speedup"] = round(row["served"]["Gvox_levels_s"] / row["plain"]["Gvox_levels_s"], 2)
-        log(f"[serve] {rate:6.0f}/s x{n}: served {row['served']['Gvox_levels_s']:8.1f} Gvox-levels/s "
+        log(f"[serve:{tag}] {rate:6.0f}/s x{n}: served {row['served']['Gvox_levels_s']:8.1f} Gvox-levels/s "
             f"p50 {row['served']['p50_ms']:9.2f} p99 {row['served']['p99_ms']:9.2f} ms (batch {sv['batch']:.1f}, "
+            f"{sv['batch_dist']['launches']} launches, GPU busy {sv['gpu_busy_frac']:.0%}, "
             f"PCIe {row['served']['pcie_GBps']:.1f} GB/s) | plain {row['plain']['Gvox_levels_s']:8.1f} "
             f"p50 {row['plain']['p50_ms']:9.2f} p99 {row['plain']['p99_ms']:9.2f} ms | x{row['speedup']} same={same}")
-        log(f"[serve]          device-resident p50/p99 served {row['served_by_origin']['device']} plain "
+        log(f"[serve:{tag}]          device-resident p50/p99 served {row['served_by_origin']['device']} plain "
             f"{row['plain_by_origin']['device']}; served mean parts {sv['parts']}")
         rows.append(row)
     S.close()
+    return rows
+
+
+def snapshot_rings(sim_spec, dev, seed, base_seed):
+    """Per simulation, a ring of 3 device snapshots f(k+1) = f(k) + 0.01
+    noise of a GRF (sigma 3)."""
+    import torch
+    from mkf_amd import synth
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    sims = []
+    for i, (side, dt) in enumerate(sim_spec):
+        ring = [synth.grf_field_direct((side, side, side), 3.0, seed=base_seed + i, device=dev, dtype=dt)]
+        for _ in range(2):
+            ring.append(ring[-1] + 0.01 * torch.randn(ring[-1].shape, generator=g, device=dev, dtype=dt))
+        sims.append(ring)
+    return sims
+
+
+def bench_serve(args, dev):
+    """Two request streams through the server (continuous batching, 64 GB
+    budget), each at several Poisson rates beside plain per-request launches:
+    * the configured stream: cube side log-uniform 64-768, f32:f64 3:1, 1-16
+      levels, 30 % device-resident snapshots, host fields pinned (PCIe-bound);
+    * a device-resident stream: the same distribution, every request an
+      in-situ snapshot of one of 16 simulations (no PCIe), where batching
+      has the GPU to itself."""
+    import torch
+
+    from mkf_amd import synth
+    from mkf_amd.serve import request_stream
+
+    lib = __import__("mkf_amd").load()
+    NMAX = args.serve_requests
+    pool_spec, sim_spec, reqs = request_stream(NMAX, seed=args.serve_seed)
+    t0 = time.perf_counter()
+    pool = []
+    for i, (side, dt) in enumerate(pool_spec):
+        f = synth.grf_field_direct((side, side, side), 3.0, seed=7000 + i, device=dev, dtype=dt)
+        h = torch.empty(f.shape, dtype=dt, pin_memory=True)
+        h.copy_(f)
+        pool.append(h)
+        del f
+    sims = snapshot_rings(sim_spec, dev, args.serve_seed, 8000)
+    fields, turn = [], [0] * len(sims)
+    for r in reqs:
+        if r["sim"] is None:
+            fields.append(pool[r["pool"]])
+        else:
+            s = r["sim"]
+            fields.append(sims[s][turn[s] % 3])
+            turn[s] += 1
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    log(f"[serve] {NMAX} requests over {len(pool)} pinned host fields and {len(sims)} simulations "
+        f"ready in {time.perf_counter() - t0:.1f}s")
+    rows = serve_sweep(lib, dev, fields, reqs, args.serve_rates, args.serve_seconds, args.serve_seed, "mixed")
     del fields, sims, pool
+    torch.cuda.empty_cache()
+
+    # Device-resident stream: 16 simulations, every request a snapshot.
+    dseed = args.serve_seed + 100
+    _, dsim_spec, dreqs = request_stream(NMAX, seed=dseed, device_share=1.0, n_sims=16)
+    t0 = time.perf_counter()
+    dsims = snapshot_rings(dsim_spec, dev, dseed, 9000)
+    dfields, turn = [], [0] * len(dsims)
+    for r in dreqs:
+        s = r["sim"]
+        dfields.append(dsims[s][turn[s] % 3])
+        turn[s] += 1
+    torch.cuda.synchronize()
+    log(f"[serve] device-resident stream: {NMAX} requests over {len(dsims)} simulations "
+        f"ready in {time.perf_counter() - t0:.1f}s")
+    drows = serve_sweep(lib, dev, dfields, dreqs, args.serve_device_rates, args.serve_seconds, dseed, "device")
+    del dfields, dsims
     torch.cuda.empty_cache()
     return {"workload": f"Poisson arrivals; cube side log-uniform 64-768, f32:f64 3:1, 1-16 levels >= in [-1.5,1.5], "
                         f"30% device-resident snapshots of {len(sim_spec)} simulations (f += 0.01 noise), host fields "
                         f"pinned from a pool of {len(pool_spec)}; budget 64 GB; seed {args.serve_seed}",
-            "rates": rows}
+            "rates": rows,
+            "device_resident": {
+                "workload": f"the same distribution, every request a device-resident snapshot of one of "
+                            f"{len(dsim_spec)} simulations; seed {dseed}",
+                "rates": drows}}
+
+
+def cpu_share():
+    """Worker threads for host-side work: the CPUs this process may run on,
+    at most 16 (one GPU's share of its host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def pick_tiling(lib, fi, bm, spec):
+    """The tiling of the fused kernel: --tiling's, else the library's
+    autotuner's choice on this device and field."""
+    import mkf_amd as M
+    if spec:
+        return M.MkfTiling(*[int(v) for v in spec.split(",")])
+    tiling = M.MkfTiling()
+    M.check(lib.mkfFusedAutotune(fi.ref, C.byref(bm), C.byref(tiling), C.c_void_p(0)), "mkfFusedAutotune")
+    log(f"[bench] autotuned tiling: planePairs {tiling.planePairs}, waves {tiling.waves}, "
+        f"rowPairs {tiling.rowPairs}, voxelsPerLane {tiling.voxelsPerLane}")
+    return tiling
+
+
+def fused_leg(lib, fi, bm, tiling, hist, stream, K, W, label):
+    """Warm-up (at least W launches and 0.5 s), then K timed steps of the
+    histogram clear plus one mkfFusedBPFD launch, with HIP events around each
+    launch on the launch stream. Returns the per-launch kernel times, the wall
+    time per step and the last step's histogram."""
+    import numpy as np
+    import torch
+    import mkf_amd as M
+    sptr = C.c_void_p(stream.cuda_stream)
+
+    def launch():
+        rc = lib.mkfFusedBPFD(None, hist.data_ptr(), None, fi.ref, C.byref(bm), C.byref(tiling), sptr)
+        if rc != 0:
+            raise RuntimeError(f"mkfFusedBPFD failed: {M.last_error()}")
+
+    t_w = time.perf_counter()
+    n_w = 0
+    while n_w < W or time.perf_counter() - t_w < 0.5:
+        hist.zero_()
+        launch()
+        n_w += 1
+        if n_w % 16 == 0:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(K):
+        hist.zero_()
+        starts[i].record(stream)
+        launch()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    log(f"[bench] {label}: {K} steps: {wall * 1e3 / K:.3f} ms/step wall, kernel {sum(kern_ms) / K:.3f} ms "
+        f"(min {min(kern_ms):.3f})")
+    return {"kern_ms": kern_ms, "wall_ms": wall * 1e3 / K, "hist": hist.cpu().numpy().view(np.uint64).copy()}
+
+
+def traffic_of(S, dt, tiling):
+    """HBM bytes per launch from the rocprof FETCH_SIZE / WRITE_SIZE runs in
+    profiles/, per field shape, dtype and tiling (None if not measured)."""
+    tpath = os.path.join(ROOT, "profiles", "fused_traffic.json")
+    if not os.path.exists(tpath):
+        return None
+    with open(tpath) as fh:
+        ent = json.load(fh).get(f"{S}_{dt}")
+    if isinstance(ent, dict):
+        ent = ent.get(f"{tiling.planePairs},{tiling.waves},{tiling.rowPairs},{tiling.voxelsPerLane}")
+    return ent
+
+
+def read_roof(lib, ptr, nbytes, stream):
+    """Practical HBM read roof: mkfReadRoof over nbytes of device memory,
+    non-temporal and ordinary 16-byte loads at 4..16 workgroups per CU; the
+    best mean of 10 launches (after 3 untimed ones) per configuration."""
+    import torch
+    import mkf_amd as M
+    sptr = C.c_void_p(stream.cuda_stream)
+    out = torch.zeros(1, dtype=torch.int32, device="cuda")
+    best = None
+    rows = []
+    for nt in (1, 0):
+        for bpc in (4, 8, 16):
+            for _ in range(3):
+                M.check(lib.mkfReadRoof(C.c_void_p(ptr), nbytes, nt, bpc, C.c_void_p(out.data_ptr()), sptr), "roof")
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(10):
+                M.check(lib.mkfReadRoof(C.c_void_p(ptr), nbytes, nt, bpc, C.c_void_p(out.data_ptr()), sptr), "roof")
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 10
+            gbps = nbytes / (ms * 1e-3) / 1e9
+            rows.append({"nontemporal": bool(nt), "blocks_per_cu": bpc, "ms": round(ms, 4), "GBps": round(gbps, 1)})
+            if best is None or gbps > best["GBps"]:
+                best = rows[-1]
+    log(f"[bench] read roof: {best['GBps']} GB/s ({'non-temporal' if best['nontemporal'] else 'ordinary'} loads, "
+        f"{best['blocks_per_cu']} workgroups/CU) = {best['GBps'] / HBM_PEAK_GBS:.1%} of the 8 TB/s peak")
+    return {"GBps": best["GBps"], "frac_of_peak": round(best["GBps"] / HBM_PEAK_GBS, 4),
+            "kernel": "mkfReadRoof (streaming 16-byte loads, XOR checksum)", "bytes": nbytes,
+            "best": best, "sweep": rows}
+
+
+def cpu_baseline(f_np, bm, threads):
+    """The oracle's C restatement of the reference CPU path
+    (mkfAccGetBPFDSimple: threshold + 64-bit chunk walk) on the same field
+    bytes, in two modes: 1 thread, as the reference runs when built with gcc
+    (its OpenACC pragmas inert), on a 129-plane slab of the field; and the
+    host's CPU share (plane-pair ranges on pthreads, private histograms),
+    which models the OpenACC multicore target, on the whole field. Each mode
+    is timed over 3 passes; the best pass gives its rate. Returns the bench
+    line's cpu_baseline object and the whole field's histogram."""
+    from oracle import oracle as orc
+    Z, Y, X = f_np.shape
+    sample = f_np[: min(Z, 129)]
+
+    def passes(arr, nthreads):
+        orc.set_threads(nthreads)
+        ts, h = [], None
+        for _ in range(3):
+            t1 = time.perf_counter()
+            h, _ = orc.bpfd_field(arr, bm.t[0], bm.m)
+            ts.append(time.perf_counter() - t1)
+        return ts, h
+
+    t1s, _ = passes(sample, 1)
+    tNs, hist = passes(f_np, threads)
+    one = sample.size / min(t1s) / 1e9
+    alln = f_np.size / min(tNs) / 1e9
+    log(f"[bench] CPU oracle: 1 thread {one:.3f} Gvox/s ({sample.shape[0]} planes), {threads} threads "
+        f"{alln:.3f} Gvox/s (whole field); host: {os.cpu_count()} CPUs, {cpu_model()}")
+    return {
+        "value": round(alln, 4), "unit": "Gvox/s", "cores": threads, "kind": "port",
+        "sample": f"all-cores mode: the whole {X}x{Y}x{Z} field (same bytes as the GPU run), best of 3 passes "
+                  f"({', '.join(f'{t:.2f}' for t in tNs)} s); C restatement of mkfAccGetBPFDSimple (gcc -O3, "
+                  f"pthreads over plane pairs, private histograms)",
+        "modes": {
+            "1_thread": {"value": round(one, 4), "unit": "Gvox/s", "threads": 1,
+                         "sample": f"planes 0..{sample.shape[0] - 1} of the field ({sample.size} voxels), "
+                                   f"best of 3 passes ({', '.join(f'{t:.2f}' for t in t1s)} s)"},
+            "all_cores": {"value": round(alln, 4), "unit": "Gvox/s", "threads": threads,
+                          "sample": f"the whole field ({f_np.size} voxels), best of 3 passes"},
+        },
+        "host_cpus": os.cpu_count(), "cpus_available": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+        "cpu_model": cpu_model(),
+    }, hist
 
 
 def main():
@@ -343,7 +575,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--threshold", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle check")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = the CPUs available, at most 16")
+    ap.add_argument("--no-f64", action="store_true", help="skip the float64 leg of the default run")
     ap.add_argument("--tiling", type=str, default="",
                     help="planePairs,waves,rowPairs[,voxelsPerLane] (default: library choice)")
     ap.add_argument("--mode", choices=["all", "fused", "batch", "serve"], default="all",
@@ -353,13 +586,15 @@ def main():
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per rate (capped by requests)")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[50, 100, 200, 500, 1000, 2000])
+    ap.add_argument("--serve-device-rates", type=float, nargs="+", default=[500, 2000, 5000, 10000],
+                    help="rates of the device-resident stream")
     ap.add_argument("--batch-seed", type=int, default=4)
     ap.add_argument("--pmc-workload", action="store_true",
                     help="for counter runs: 3 fused launches, then 3 copies of a 1 GiB buffer (the calibration "
                          "kernel for FETCH_SIZE / WRITE_SIZE); with --mode batch, 3 batched launches; prints no JSON")
-    ap.add_argument("--pmc-target", choices=("fused", "bpfd"), default="fused",
+    ap.add_argument("--pmc-target", choices=("fused", "bpfd", "pbm"), default="fused",
                     help="with --pmc-workload: 'bpfd' makes the PBM once and runs 3 histograms of it instead "
-                         "(with --tiling's tiling, else the library's default)")
+                         "(with --tiling's tiling, else the library's default); 'pbm' runs 3 map passes")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -395,9 +630,13 @@ def main():
     S = args.size
     log(f"[bench] generating {S}^3 GRF sigma={args.sigma} seed={args.seed} ({args.dtype})")
     t0 = time.perf_counter()
-    field = synth.grf_field((S, S, S), args.sigma, args.seed, device=dev, dtype=torch.float64)
-    if args.dtype == "f32":
-        field = field.to(torch.float32)
+    field64 = synth.grf_field((S, S, S), args.sigma, args.seed, device=dev, dtype=torch.float64)
+    field = field64.to(torch.float32) if args.dtype == "f32" else field64
+    # The float64 leg runs on the same field before its rounding (default run only).
+    f64_leg = args.dtype == "f32" and not args.no_f64 and not args.pmc_workload
+    if not f64_leg and args.dtype == "f32":
+        del field64
+        field64 = None
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     log(f"[bench] field ready in {time.perf_counter() - t0:.1f}s")
@@ -406,16 +645,9 @@ def main():
     eb = 4 if args.dtype == "f32" else 8
     fi = M.FieldInfo(field.data_ptr(), (S, S, S), enc)
     bm = M.bin_map(">=", args.threshold)
-    if args.tiling:
-        tiling = M.MkfTiling(*[int(v) for v in args.tiling.split(",")])
-    else:
-        # The library's autotuner picks the tiling on this device and field.
-        tiling = M.MkfTiling()
-        M.check(lib.mkfFusedAutotune(fi.ref, C.byref(bm), C.byref(tiling), C.c_void_p(0)), "mkfFusedAutotune")
-        log(f"[bench] autotuned tiling: planePairs {tiling.planePairs}, waves {tiling.waves}, "
-            f"rowPairs {tiling.rowPairs}, voxelsPerLane {tiling.voxelsPerLane}")
     stream = torch.cuda.current_stream()
     sptr = C.c_void_p(stream.cuda_stream)
+    tiling = pick_tiling(lib, fi, bm, args.tiling)
     hist = torch.zeros(256, dtype=torch.int64, device=dev)
 
     def launch():
@@ -438,6 +670,16 @@ def main():
         torch.cuda.synchronize()
         log("[bench] counter workload done: 1 PBM, 3 histograms of the PBM")
         return
+    if args.pmc_workload and args.pmc_target == "pbm":
+        # 3 map passes of the field into a caller buffer.
+        org = M.BMOrg()
+        nwords = lib.setBMO(C.byref(org), M.dims((S, S, S)), b"\x00")
+        pbm = torch.empty(nwords, dtype=torch.int32, device=dev)
+        for _ in range(3):
+            M.check(lib.mkfPBMAsync(pbm.data_ptr(), C.byref(org), fi.ref, C.byref(bm), sptr), "mkfPBMAsync")
+        torch.cuda.synchronize()
+        log("[bench] counter workload done: 3 map passes")
+        return
     if args.pmc_workload:
         for _ in range(3):
             launch()
@@ -453,54 +695,18 @@ def main():
         log("[bench] counter workload done: 3 fused launches, 3 copies of 1 GiB")
         return
 
-    # Warm-up: at least W launches and at least 0.5 s of launches, so the
-    # GPU has left its idle clocks before the timed region.
-    t_w = time.perf_counter()
-    n_w = 0
-    while n_w < args.warmup or time.perf_counter() - t_w < 0.5:
-        hist.zero_()
-        launch()
-        n_w += 1
-        if n_w % 16 == 0:
-            torch.cuda.synchronize()
-    torch.cuda.synchronize()
-
+    leg = fused_leg(lib, fi, bm, tiling, hist, stream, args.steps, args.warmup, f"{S}^3 {args.dtype}")
+    gpu_hist = leg["hist"]
+    kern_ms = leg["kern_ms"]
     K = args.steps
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(K):
-        hist.zero_()
-        starts[i].record(stream)
-        launch()
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    gpu_hist = hist.cpu().numpy().view(np.uint64).copy()
-
     nvox = S * S * S
-    ms_per_step = wall * 1e3 / K
+    ms_per_step = leg["wall_ms"]
     value = nvox / (ms_per_step * 1e-3) / 1e9
     kern_avg = sum(kern_ms) / len(kern_ms)
     bytes_per_launch = nvox * eb
     achieved = bytes_per_launch / (kern_avg * 1e-3) / 1e9
-    log(f"[bench] {K} steps: {ms_per_step:.3f} ms/step wall, kernel {kern_avg:.3f} ms "
-        f"(min {min(kern_ms):.3f}), {achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.1%} of HBM peak")
 
     workload = f"fused PBM+BPFD, {S}x{S}x{S} {'float32' if eb == 4 else 'float64'} GRF (sigma {args.sigma:g}), >= {args.threshold:g}"
-    # HBM bytes per launch from the rocprof FETCH_SIZE / WRITE_SIZE runs in
-    # profiles/, per field shape, dtype and tiling.
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "fused_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            ent = json.load(fh).get(f"{S}_{args.dtype}")
-        if isinstance(ent, dict):
-            ent = ent.get(f"{tiling.planePairs},{tiling.waves},{tiling.rowPairs},{tiling.voxelsPerLane}")
-        traffic = ent
-
     result = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -519,11 +725,17 @@ def main():
                    "tiling": {"planePairs": tiling.planePairs, "waves": tiling.waves, "rowPairs": tiling.rowPairs,
                               "voxelsPerLane": tiling.voxelsPerLane, "autotuned": not args.tiling}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(S, args.dtype, tiling),
                      "kernel_ms_mean": round(kern_avg, 4), "kernel_ms_min": round(min(kern_ms), 4),
                      "bytes_per_launch": bytes_per_launch},
         "cpu_baseline": None,
     }
+
+    # The practical read roof of this device: the streaming read kernel over
+    # the field's own 4 GiB, best configuration of a small sweep.
+    roof = read_roof(lib, field.data_ptr(), nvox * eb, stream)
+    result["roofline"]["measured_roof"] = roof
+    result["roofline"]["frac_of_measured_roof"] = round(achieved / roof["GBps"], 4)
 
     # The reference's two-kernel drop-in path on the same field: binMapCUDA
     # (PBM into a caller buffer) then mkfCUDAGetBPFD (histogram of the PBM),
@@ -614,22 +826,41 @@ def main():
         f"{result['host_stream']['same_histogram_as_fused']}")
     del hostf
 
-    if not args.no_cpu:
-        from oracle import oracle as orc
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"[bench] copying field to host for the CPU baseline ({threads} threads)")
-        f_np = field.cpu().numpy()
-        orc.set_threads(threads)
-        t0 = time.perf_counter()
-        cpu_hist, _ = orc.bpfd_field(f_np, bm.t[0], bm.m)
-        cpu_s = time.perf_counter() - t0
-        parity = bool(np.array_equal(cpu_hist, gpu_hist))
-        log(f"[bench] CPU oracle: {cpu_s:.2f}s = {nvox / cpu_s / 1e9:.3f} Gvox/s; GPU == CPU: {parity}")
-        result["cpu_baseline"] = {
-            "value": round(nvox / cpu_s / 1e9, 4), "unit": "Gvox/s", "cores": threads, "kind": "port",
-            "sample": f"the whole {S}^3 field (same bytes as the GPU run), one pass, "
-                      f"{cpu_s:.2f}s; C restatement of mkfAccGetBPFDSimple (gcc -O3, pthreads over plane pairs)",
+    if f64_leg:
+        # The same field in float64 (8 GiB): its own tiling, timing and roofline.
+        fi64 = M.FieldInfo(field64.data_ptr(), (S, S, S), M.ENC_F64)
+        t64 = pick_tiling(lib, fi64, bm, "")
+        h64 = torch.zeros(256, dtype=torch.int64, device=dev)
+        leg64 = fused_leg(lib, fi64, bm, t64, h64, stream, K, args.warmup, f"{S}^3 f64")
+        k64 = sum(leg64["kern_ms"]) / len(leg64["kern_ms"])
+        a64 = nvox * 8 / (k64 * 1e-3) / 1e9
+        result["f64"] = {
+            "workload": f"fused PBM+BPFD, {S}x{S}x{S} float64 GRF (the same field before rounding), >= {args.threshold:g}",
+            "value": round(nvox / (leg64["wall_ms"] * 1e-3) / 1e9, 2), "unit": "Gvox/s",
+            "ms_per_step": round(leg64["wall_ms"], 4),
+            "tiling": {"planePairs": t64.planePairs, "waves": t64.waves, "rowPairs": t64.rowPairs,
+                       "voxelsPerLane": t64.voxelsPerLane, "autotuned": True},
+            "roofline": {"bound": "hbm", "achieved": round(a64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(a64 / HBM_PEAK_GBS, 4), "traffic": traffic_of(S, "f64", t64),
+                         "frac_of_measured_roof": round(a64 / roof["GBps"], 4),
+                         "kernel_ms_mean": round(k64, 4), "kernel_ms_min": round(min(leg64["kern_ms"]), 4),
+                         "bytes_per_launch": nvox * 8},
         }
+        if not args.no_cpu:
+            from oracle import oracle as orc
+            orc.set_threads(cpu_share())
+            ref64, _ = orc.bpfd_field(field64.cpu().numpy(), bm.t[0], bm.m)
+            ok64 = bool(np.array_equal(ref64, leg64["hist"]))
+            result["f64"]["parity"] = "bit-exact vs oracle" if ok64 else "MISMATCH vs oracle"
+            log(f"[bench] f64 leg: GPU == CPU: {ok64}")
+        del field64, h64
+        torch.cuda.empty_cache()
+
+    if not args.no_cpu:
+        f_np = field.cpu().numpy()
+        result["cpu_baseline"], cpu_hist = cpu_baseline(f_np, bm, args.cpu_threads or cpu_share())
+        parity = bool(np.array_equal(cpu_hist, gpu_hist))
+        log(f"[bench] GPU == CPU: {parity}")
         result["parity"] = "bit-exact vs oracle" if parity else "MISMATCH vs oracle"
         if not parity:
             print(json.dumps(result))

@@ -132,6 +132,14 @@ int mkfFusedLaunchInfo(const BMFieldInfo *pI, const BinMapF64 *pM, const MkfTili
 /* The same for mkfBPFDAsync on a PBM of organisation pO. */
 int mkfBPFDLaunchInfo(const BMOrg *pO, const MkfTiling *pT, MkfLaunchInfo *pInfo);
 
+/* Measurement aid, no counterpart in the reference: reads `bytes` (a
+ * multiple of 16, 16-byte aligned) of device memory p once with 16-byte
+ * loads, non-temporal when nonTemporal != 0, from blocksPerCU (1..64)
+ * workgroups of 256 lanes per CU, and XORs a checksum of every word into
+ * *pOut (device). Timed around the launch, it gives the practical HBM read
+ * roof the histogram kernels are compared with. */
+int mkfReadRoof(const void *p, size_t bytes, int nonTemporal, int blocksPerCU, unsigned *pOut, void *stream);
+
 /* Times the fused kernel on this field (the caller's data, a scratch
  * histogram) under a few tilings, keeps the fastest for fields of this shape
  * and encoding (process-wide), and stores it in *pChosen (may be NULL).

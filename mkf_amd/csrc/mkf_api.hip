@@ -10,6 +10,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -441,6 +442,59 @@ int mkfFusedBPFDMomentsNoAlloc(size_t* pBPFD, double* pMom, BMOrg* pO, const BMF
   const hipError_t e = nParts ? launchFusedMoments(fv, *pM, (unsigned long long*)pBPFD, part, pMom, tl, s)
                               : launchPBMMoments(fv, *pM, nullptr, org.rowWS, org.planeWS, part, pMom, s);
   return e == hipSuccess ? MKF_OK : hipFail(e, "no-alloc fused moments launch");
+}
+
+}  // extern "C"
+
+namespace {
+
+// Streaming read of n 16-byte words, for the practical HBM read roof: each
+// lane keeps four loads in flight (grid-stride, non-temporal or ordinary
+// loads) and folds them into an XOR, so no load can be dropped; one atomic
+// per wave publishes it.
+template <bool NT>
+__global__ __launch_bounds__(256) void kReadRoof(const uint4* __restrict__ p, size_t n, unsigned* __restrict__ out) {
+  typedef unsigned V4 __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(1))) V4* GV4;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned acc = 0;
+  auto ld = [&](size_t k) -> V4 {
+    if constexpr (NT) return __builtin_nontemporal_load((GV4)(p + k));
+    else return *(const V4*)(p + k);
+  };
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const V4 a = ld(i), b = ld(i + stride), c = ld(i + 2 * stride), d = ld(i + 3 * stride);
+    const V4 x = a ^ b ^ c ^ d;
+    acc ^= x.x ^ x.y ^ x.z ^ x.w;
+  }
+  for (; i < n; i += stride) {
+    const V4 a = ld(i);
+    acc ^= a.x ^ a.y ^ a.z ^ a.w;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc ^= __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) atomicXor(out, acc);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mkfReadRoof(const void* p, size_t bytes, int nonTemporal, int blocksPerCU, unsigned* pOut, void* stream) {
+  if (!p || !pOut || bytes < 16 || (bytes & 15) || ((uintptr_t)p & 15) || blocksPerCU < 1 || blocksPerCU > 64)
+    return fail(MKF_ERR_ARG, "read roof: need 16-byte aligned memory, a multiple of 16 bytes, 1..64 blocks per CU");
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+  const size_t n = bytes / 16;
+  const unsigned grid = (unsigned)std::min<size_t>((size_t)cus * blocksPerCU, (n + 255) / 256);
+  const hipStream_t s = (hipStream_t)stream;
+  if (nonTemporal)
+    hipLaunchKernelGGL(kReadRoof<true>, dim3(grid), dim3(256), 0, s, (const uint4*)p, n, pOut);
+  else
+    hipLaunchKernelGGL(kReadRoof<false>, dim3(grid), dim3(256), 0, s, (const uint4*)p, n, pOut);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MKF_OK : hipFail(e, "read roof launch");
 }
 
 int mkfFusedAutotune(const BMFieldInfo* pI, const BinMapF64* pM, MkfTiling* pChosen, void* stream) {
