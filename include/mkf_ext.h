@@ -134,11 +134,17 @@ int mkfBPFDLaunchInfo(const BMOrg *pO, const MkfTiling *pT, MkfLaunchInfo *pInfo
 
 /* Measurement aid, no counterpart in the reference: reads `bytes` (a
  * multiple of 16, 16-byte aligned) of device memory p once with 16-byte
- * loads, non-temporal when nonTemporal != 0, from blocksPerCU (1..64)
- * workgroups of 256 lanes per CU, and XORs a checksum of every word into
- * *pOut (device). Timed around the launch, it gives the practical HBM read
- * roof the histogram kernels are compared with. */
-int mkfReadRoof(const void *p, size_t bytes, int nonTemporal, int blocksPerCU, unsigned *pOut, void *stream);
+ * loads from blocksPerCU (1..64) workgroups of 4 waves per CU, and XORs
+ * every word into *pOut (device). Each wave streams blocks of loadsInFlight
+ * (4, 8 or 16) KiB of consecutive bytes, all loads of a block in flight at
+ * once. flags: MKF_ROOF_NONTEMPORAL (non-temporal loads), MKF_ROOF_CHUNKED
+ * (each wave reads one contiguous run of blocks; otherwise waves take
+ * blocks round robin). Timed around the launch, it gives the practical HBM
+ * read roof the histogram kernels are compared with. */
+#define MKF_ROOF_NONTEMPORAL 1
+#define MKF_ROOF_CHUNKED 2
+int mkfReadRoof(const void *p, size_t bytes, int flags, int loadsInFlight, int blocksPerCU, unsigned *pOut,
+                void *stream);
 
 /* Times the fused kernel on this field (the caller's data, a scratch
  * histogram) under a few tilings, keeps the fastest for fields of this shape

@@ -146,7 +146,7 @@ _SIGNATURES = {
     "mkfFusedLaunchInfo": (C.c_int, [C.POINTER(BMFieldInfo), C.POINTER(BinMapF64), C.POINTER(MkfTiling),
                                      C.POINTER(MkfLaunchInfo)]),
     "mkfBPFDLaunchInfo": (C.c_int, [C.POINTER(BMOrg), C.POINTER(MkfTiling), C.POINTER(MkfLaunchInfo)]),
-    "mkfReadRoof": (C.c_int, [_P, C.c_size_t, C.c_int, C.c_int, _P, _P]),
+    "mkfReadRoof": (C.c_int, [_P, C.c_size_t, C.c_int, C.c_int, C.c_int, _P, _P]),
     "mkfFusedBPFD": (C.c_int, [C.POINTER(KernInfo), _P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo),
                                C.POINTER(BinMapF64), C.POINTER(MkfTiling), _P]),
     "mkfPBMAsync": (C.c_int, [_P, C.POINTER(BMOrg), C.POINTER(BMFieldInfo), C.POINTER(BinMapF64), _P]),

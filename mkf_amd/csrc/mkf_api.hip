@@ -448,51 +448,83 @@ int mkfFusedBPFDMomentsNoAlloc(size_t* pBPFD, double* pMom, BMOrg* pO, const BMF
 
 namespace {
 
-// Streaming read of n 16-byte words, for the practical HBM read roof: each
-// lane keeps four loads in flight (grid-stride, non-temporal or ordinary
-// loads) and folds them into an XOR, so no load can be dropped; one atomic
-// per wave publishes it.
-template <bool NT>
+// Streaming read of n 16-byte words, for the practical HBM read roof. A
+// block is U loads of 1 KiB (64 lanes x 16 bytes), all in flight at once, so
+// a wave streams U KiB of consecutive bytes per step. Round robin (CH false):
+// wave w takes blocks w, w + waves, ..., so all waves read one moving
+// window. Chunked (CH true): wave w takes a contiguous run of blocks, so the
+// waves stream as many separate regions at once. The words are folded into
+// an XOR, so no load can be dropped; one atomic per wave publishes it.
+template <bool NT, bool CH, int U>
 __global__ __launch_bounds__(256) void kReadRoof(const uint4* __restrict__ p, size_t n, unsigned* __restrict__ out) {
   typedef unsigned V4 __attribute__((ext_vector_type(4)));
   typedef const __attribute__((address_space(1))) V4* GV4;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned acc = 0;
+  const int lane = threadIdx.x & 63;
+  const size_t waves = (size_t)gridDim.x * (blockDim.x >> 6);
+  const size_t wave = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const size_t blk = (size_t)64 * U;
+  const size_t full = n / blk;  // whole blocks
   auto ld = [&](size_t k) -> V4 {
     if constexpr (NT) return __builtin_nontemporal_load((GV4)(p + k));
     else return *(const V4*)(p + k);
   };
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    const V4 a = ld(i), b = ld(i + stride), c = ld(i + 2 * stride), d = ld(i + 3 * stride);
-    const V4 x = a ^ b ^ c ^ d;
-    acc ^= x.x ^ x.y ^ x.z ^ x.w;
+  size_t b0 = wave, b1 = full, step = waves;
+  if constexpr (CH) {
+    const size_t per = (full + waves - 1) / waves;
+    b0 = std::min(full, wave * per);
+    b1 = std::min(full, b0 + per);
+    step = 1;
   }
-  for (; i < n; i += stride) {
-    const V4 a = ld(i);
-    acc ^= a.x ^ a.y ^ a.z ^ a.w;
+  V4 acc = {0u, 0u, 0u, 0u};
+  for (size_t b = b0; b < b1; b += step) {
+    V4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = ld(b * blk + (size_t)u * 64 + lane);
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc ^= v[u];
   }
-  for (int o = 32; o > 0; o >>= 1) acc ^= __shfl_xor(acc, o);
-  if ((threadIdx.x & 63) == 0) atomicXor(out, acc);
+  // the words after the last whole block: word by word, by the last wave
+  if (wave == waves - 1)
+    for (size_t k = full * blk + lane; k < n; k += 64) acc ^= ld(k);
+  unsigned x = acc.x ^ acc.y ^ acc.z ^ acc.w;
+  for (int o = 32; o > 0; o >>= 1) x ^= __shfl_xor(x, o);
+  if (lane == 0) atomicXor(out, x);
+}
+
+template <bool NT, bool CH>
+void launchReadRoof(int U, unsigned grid, hipStream_t s, const uint4* p, size_t n, unsigned* out) {
+  if (U == 4)
+    hipLaunchKernelGGL((kReadRoof<NT, CH, 4>), dim3(grid), dim3(256), 0, s, p, n, out);
+  else if (U == 8)
+    hipLaunchKernelGGL((kReadRoof<NT, CH, 8>), dim3(grid), dim3(256), 0, s, p, n, out);
+  else
+    hipLaunchKernelGGL((kReadRoof<NT, CH, 16>), dim3(grid), dim3(256), 0, s, p, n, out);
 }
 
 }  // namespace
 
 extern "C" {
 
-int mkfReadRoof(const void* p, size_t bytes, int nonTemporal, int blocksPerCU, unsigned* pOut, void* stream) {
-  if (!p || !pOut || bytes < 16 || (bytes & 15) || ((uintptr_t)p & 15) || blocksPerCU < 1 || blocksPerCU > 64)
-    return fail(MKF_ERR_ARG, "read roof: need 16-byte aligned memory, a multiple of 16 bytes, 1..64 blocks per CU");
+int mkfReadRoof(const void* p, size_t bytes, int flags, int loadsInFlight, int blocksPerCU, unsigned* pOut,
+                void* stream) {
+  if (!p || !pOut || bytes < 16 || (bytes & 15) || ((uintptr_t)p & 15) || blocksPerCU < 1 || blocksPerCU > 64 ||
+      (loadsInFlight != 4 && loadsInFlight != 8 && loadsInFlight != 16) || (flags & ~3))
+    return fail(MKF_ERR_ARG,
+                "read roof: need 16-byte aligned memory, a multiple of 16 bytes, flags in 0..3, 4/8/16 loads in "
+                "flight and 1..64 blocks per CU");
   int dev = 0, cus = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
   const size_t n = bytes / 16;
-  const unsigned grid = (unsigned)std::min<size_t>((size_t)cus * blocksPerCU, (n + 255) / 256);
+  const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((size_t)cus * blocksPerCU, (n + 255) / 256));
   const hipStream_t s = (hipStream_t)stream;
-  if (nonTemporal)
-    hipLaunchKernelGGL(kReadRoof<true>, dim3(grid), dim3(256), 0, s, (const uint4*)p, n, pOut);
-  else
-    hipLaunchKernelGGL(kReadRoof<false>, dim3(grid), dim3(256), 0, s, (const uint4*)p, n, pOut);
+  const uint4* q = (const uint4*)p;
+  switch (flags) {
+    case 0: launchReadRoof<false, false>(loadsInFlight, grid, s, q, n, pOut); break;
+    case 1: launchReadRoof<true, false>(loadsInFlight, grid, s, q, n, pOut); break;
+    case 2: launchReadRoof<false, true>(loadsInFlight, grid, s, q, n, pOut); break;
+    default: launchReadRoof<true, true>(loadsInFlight, grid, s, q, n, pOut); break;
+  }
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? MKF_OK : hipFail(e, "read roof launch");
 }
