@@ -482,18 +482,120 @@ struct OnMom {
   bool haloLast;      // the voxel right of the wave is the field's last column, read from memory
 };
 
-// Per-wave tally. Cells that do not exist have code 0 (their pair bits are
-// cleared), so no per-cell validity test is needed. All-one cells are counted
-// from ballots in scalar registers; mixed cells go to the wave's LDS
-// sub-histogram; all-zero cells are what remains of the cell count.
-struct Tally {
-  uint32_t c255 = 0;
+// Sum over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t waveSum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
 
-  __device__ __forceinline__ void add(uint32_t code, uint32_t* hist) {
-    c255 += __builtin_popcountll(ballot(code == 255u));
-    if ((code - 1u) < 254u) __hip_atomic_fetch_add(hist + code, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+// Byte-wise arithmetic on a word of 4 cell codes (byte k = code of cell k).
+// Byte k of the result is (byte k of C) - 1 mod 256, with no borrow between
+// bytes: setting each byte's top bit keeps every byte's subtraction inside
+// it, and the XOR restores the top bit.
+__device__ __forceinline__ uint32_t bytesMinusOne(uint32_t C) {
+  constexpr uint32_t H = 0x80808080u;
+  return ((C | H) - 0x01010101u) ^ (~C & H);
+}
+// Bit 7 of byte k is set iff byte k of C is 0xFF (the low 7 bits + 1 carry
+// into bit 7, which stays inside the byte).
+__device__ __forceinline__ uint32_t allOneBytes(uint32_t C) {
+  return ((C & 0x7F7F7F7Fu) + 0x01010101u) & C & 0x80808080u;
+}
+
+// The lanes whose byte K of D is below 254, i.e. whose cell K is mixed when D
+// = bytesMinusOne(C) (codes 0 and 255 become 255 and 254): one compare with
+// an SDWA byte select.
+template <int K>
+__device__ __forceinline__ uint64_t mixedLanes(uint32_t D) {
+  uint64_t s;
+  const uint32_t lim = 254u;
+  if constexpr (K == 0) asm("v_cmp_gt_u32_sdwa %0, %1, %2 src0_sel:DWORD src1_sel:BYTE_0" : "=s"(s) : "v"(lim), "v"(D));
+  if constexpr (K == 1) asm("v_cmp_gt_u32_sdwa %0, %1, %2 src0_sel:DWORD src1_sel:BYTE_1" : "=s"(s) : "v"(lim), "v"(D));
+  if constexpr (K == 2) asm("v_cmp_gt_u32_sdwa %0, %1, %2 src0_sel:DWORD src1_sel:BYTE_2" : "=s"(s) : "v"(lim), "v"(D));
+  if constexpr (K == 3) asm("v_cmp_gt_u32_sdwa %0, %1, %2 src0_sel:DWORD src1_sel:BYTE_3" : "=s"(s) : "v"(lim), "v"(D));
+  return s;
+}
+// 4 x byte K of C (the byte offset of its histogram bin): one shift with an
+// SDWA byte select.
+template <int K>
+__device__ __forceinline__ uint32_t byteTimes4(uint32_t C) {
+  uint32_t a;
+  if constexpr (K == 0)
+    asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0" : "=v"(a) : "v"(C));
+  if constexpr (K == 1)
+    asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_1" : "=v"(a) : "v"(C));
+  if constexpr (K == 2)
+    asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_2" : "=v"(a) : "v"(C));
+  if constexpr (K == 3)
+    asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_3" : "=v"(a) : "v"(C));
+  return a;
+}
+
+// Tally of cell codes into the workgroup's LDS histogram. Cells that do not
+// exist have code 0 (their pair bits are cleared), so no per-cell validity
+// test is needed. All-one cells are counted per lane, four at a time, by a
+// byte-wise test and a popcount; only mixed cells (code 1..254) issue an LDS
+// atomic; all-zero cells are what remains of the cell count.
+struct Tally {
+  uint32_t ones = 0;  // this lane's all-one cells
+
+  template <int K>
+  __device__ __forceinline__ void addByte(uint32_t C, uint32_t D, uint32_t* hist) {
+    const uint64_t mix = mixedLanes<K>(D);
+    const uint32_t a = byteTimes4<K>(C);
+    if (laneBit(mix))
+      __hip_atomic_fetch_add((uint32_t*)((char*)hist + a), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  // The V cells of code word C (byte k: cell k; bytes V.. are zero).
+  template <int V>
+  __device__ __forceinline__ void add(uint32_t C, uint32_t* hist) {
+    ones += (uint32_t)__builtin_popcount(allOneBytes(C));
+    const uint32_t D = bytesMinusOne(C);
+    addByte<0>(C, D, hist);
+    if constexpr (V > 1) addByte<1>(C, D, hist);
+    if constexpr (V > 2) {
+      addByte<2>(C, D, hist);
+      addByte<3>(C, D, hist);
+    }
   }
 };
+
+// Clears the workgroup's LDS histogram and cell counter; every thread calls
+// it before its first tally.
+__device__ __forceinline__ void clearHistogram(uint32_t* hist, uint32_t* cnt) {
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+  if (threadIdx.x == 0) *cnt = 0;
+  __syncthreads();
+}
+
+// Adds the workgroup's histogram to the result with one 64-bit atomic per
+// nonzero bin. Bin 255 receives the waves' all-one cells, bin 0 the
+// workgroup's cells (each wave passes its own count) minus every other bin.
+// Every thread of the workgroup calls it once, after its tally; hist[256]
+// and cnt are the workgroup's LDS histogram and a counter, cleared at start.
+__device__ __forceinline__ void flushHistogram(uint32_t* hist, uint32_t* cnt, const Tally& tally, uint32_t cells,
+                                               unsigned long long* __restrict__ bpfd) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t ones = waveSum(tally.ones);
+  __syncthreads();
+  if (lane == 0) {
+    if (ones) atomicAdd(hist + 255, ones);
+    atomicAdd(cnt, cells);
+  }
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    uint32_t rest = 0;
+    for (int i = 1 + lane; i < 256; i += kWave) rest += hist[i];
+    rest = waveSum(rest);
+    if (lane == 0) hist[0] = *cnt - rest;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+    const uint32_t v = hist[i];
+    if (v) atomicAdd(bpfd + i, (unsigned long long)v);
+  }
+}
 
 // How a wave obtains the bit right of its last voxel.
 enum HaloFrom : int { kHaloNone = 0, kHaloLds = 1, kHaloMem = 2 };
@@ -598,8 +700,7 @@ __device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1
       for (int r = 0; r < TY; ++r) {
         if (r < nY) {
           const uint32_t C = Hp[r] | (Hc[r] << 4);
-#pragma unroll
-          for (int k = 0; k < V; ++k) tally.add((C >> (8 * k)) & 255u, hist);
+          tally.template add<V>(C, hist);
         }
       }
     }

@@ -262,36 +262,6 @@ __device__ __forceinline__ bool tileOf(const TileGrid& g, int& xt, int& yr, int&
   return true;
 }
 
-// Sum over the 64 lanes of a wave.
-__device__ __forceinline__ uint32_t waveSum(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-// Adds the per-wave sub-histograms of the workgroup to the result with one
-// 64-bit atomic per nonzero bin. A wave's bin 0 is its cell count minus its
-// all-one cells minus its mixed cells (the sum of its sub-histogram).
-__device__ __forceinline__ void flushHistogram(uint32_t (*hist)[256], const Tally& tally, uint32_t cells,
-                                               unsigned long long* __restrict__ bpfd) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int w = threadIdx.x >> 6;
-  const int nw = blockDim.x >> 6;
-  uint32_t mixed = 0;
-  for (int i = lane; i < 256; i += kWave) mixed += hist[w][i];
-  mixed = waveSum(mixed);
-  if (lane == 0) {
-    hist[w][0] = cells - tally.c255 - mixed;
-    hist[w][255] = tally.c255;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
-    unsigned long long s = 0;
-    for (int k = 0; k < nw; ++k) s += hist[k][i];
-    if (s) atomicAdd(bpfd + i, s);
-  }
-}
-
 // Geometry of one wave inside its tile: the wave holds V voxels per lane, so
 // it covers W = 64 V voxels starting at x0.
 struct WaveGeom {
@@ -345,7 +315,8 @@ template <typename T, int M, int V, int TY, bool G, bool MOM = false>
 __global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr, int X, int Y, int Z, TileGrid g,
                                                     unsigned long long* __restrict__ bpfd,
                                                     double* __restrict__ part = nullptr) {
-  __shared__ uint32_t hist[16][256];
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t cnt;
   __shared__ uint32_t xch[2][TY + 1][16];
   int xt, yr, zr;
   if (!tileOf(g, xt, yr, zr)) {
@@ -354,7 +325,7 @@ __global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr,
   }
   const int lane = threadIdx.x & (kWave - 1);
   const WaveGeom q = waveGeom(g, xt, yr, zr, TY, V, X, Y, Z);
-  for (int i = lane; i < 256; i += kWave) hist[q.w][i] = 0;
+  clearHistogram(hist, &cnt);
 
   const int xb = min(q.x0, X - 1);
   const bool vec = !G && (V == 1 || q.x0 + q.W <= X);
@@ -397,13 +368,13 @@ __global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr,
       mom.lastRow = q.y0 + q.nY == Y - 1;
       mom.lastPlane = q.z0 + q.nZ == Z - 1;
       mom.haloLast = q.halo == kHaloMem && q.x0 + q.W == X - 1;
-      walkStack<TY, V, FieldRows<T, M, V, G, PART>, OnMom>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist[q.w],
+      walkStack<TY, V, FieldRows<T, M, V, G, PART>, OnMom>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist,
                                                            tally, &mom);
-      flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
+      flushHistogram(hist, &cnt, tally, cellsPerRow * q.nY * q.nZ, bpfd);
       momBlockPartial(mom.acc, part);
     } else {
-      walkStack<TY, V>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist[q.w], tally);
-      flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
+      walkStack<TY, V>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist, tally);
+      flushHistogram(hist, &cnt, tally, cellsPerRow * q.nY * q.nZ, bpfd);
     }
   };
   if constexpr (!G && V > 1 && (MOM || V == 4)) {
@@ -420,13 +391,13 @@ template <int TY>
 __global__ __launch_bounds__(1024) void kBPFD(const uint32_t* __restrict__ pW, int X, int Y, int Z,
                                               uint32_t rowWS, uint32_t planeWS, TileGrid g,
                                               unsigned long long* __restrict__ bpfd) {
-  __shared__ uint32_t hist[16][256];
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t cnt;
   __shared__ uint32_t xch[2][TY + 1][16];
   int xt, yr, zr;
   if (!tileOf(g, xt, yr, zr)) return;
-  const int lane = threadIdx.x & (kWave - 1);
   const WaveGeom q = waveGeom(g, xt, yr, zr, TY, 1, X, Y, Z);
-  for (int i = lane; i < 256; i += kWave) hist[q.w][i] = 0;
+  clearHistogram(hist, &cnt);
 
   PbmRows src;
   const uint32_t lastWord = rowWS - 1u;
@@ -439,8 +410,8 @@ __global__ __launch_bounds__(1024) void kBPFD(const uint32_t* __restrict__ pW, i
   uint32_t cellsPerRow;
   src.keep = cellKeep<1>(q.x0, X, cellsPerRow);
   Tally tally;
-  walkStack<TY, 1>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist[q.w], tally);
-  flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
+  walkStack<TY, 1>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist, tally);
+  flushHistogram(hist, &cnt, tally, cellsPerRow * q.nY * q.nZ, bpfd);
 }
 
 // Histogram of an existing PBM with 4 voxels per lane (PbmRows4): each lane
@@ -449,12 +420,13 @@ template <int TY>
 __global__ __launch_bounds__(1024) void kBPFD4(const uint32_t* __restrict__ pW, int X, int Y, int Z,
                                                uint32_t rowWS, uint32_t planeWS, TileGrid g,
                                                unsigned long long* __restrict__ bpfd) {
-  __shared__ uint32_t hist[16][256];
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t cnt;
   int xt, yr, zr;
   if (!tileOf(g, xt, yr, zr)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const WaveGeom q = waveGeom(g, xt, yr, zr, TY, 4, X, Y, Z);
-  for (int i = lane; i < 256; i += kWave) hist[q.w][i] = 0;
+  clearHistogram(hist, &cnt);
 
   PbmRows4 src;
   src.row = pW;
@@ -467,9 +439,8 @@ __global__ __launch_bounds__(1024) void kBPFD4(const uint32_t* __restrict__ pW, 
   uint32_t cellsPerRow;
   src.keep = cellKeep<4>(q.x0, X, cellsPerRow);
   Tally tally;
-  walkStack<TY, 4>(src, (uint32_t(*)[TY + 1][16]) nullptr, q.w, kHaloNone, q.y0, q.nY, q.z0, q.nZ, hist[q.w],
-                   tally);
-  flushHistogram(hist, tally, cellsPerRow * q.nY * q.nZ, bpfd);
+  walkStack<TY, 4>(src, (uint32_t(*)[TY + 1][16]) nullptr, q.w, kHaloNone, q.y0, q.nY, q.z0, q.nZ, hist, tally);
+  flushHistogram(hist, &cnt, tally, cellsPerRow * q.nY * q.nZ, bpfd);
 }
 
 // ---------------------------------------------------------------------------
@@ -497,9 +468,10 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
                                                   unsigned long long* __restrict__ bpfd) {
   typedef float Vec __attribute__((ext_vector_type(4), aligned(4)));
   typedef const __attribute__((address_space(1))) Vec* GVec;
-  __shared__ uint32_t hist[1][256];
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t cnt;
   const int lane = threadIdx.x;
-  for (int i = lane; i < 256; i += kWave) hist[0][i] = 0;
+  clearHistogram(hist, &cnt);
   constexpr int VPL = 4 * NG;  // voxels per lane
   Tally tally;
   uint32_t cells = 0;
@@ -624,9 +596,7 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
           for (int q = 0; q < NG; ++q) {
             const uint32_t Hc = P[r][q] | (P[r + 1][q] << 2);
             if (z > z0 && r < nY) {
-              const uint32_t C = Hp[r][q] | (Hc << 4);
-#pragma unroll
-              for (int k = 0; k < 4; ++k) tally.add((C >> (8 * k)) & 255u, hist[0]);
+              tally.add<4>(Hp[r][q] | (Hc << 4), hist);
             }
             Hp[r][q] = Hc;
           }
@@ -638,7 +608,7 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
     else
       walk(std::false_type{});
   }
-  flushHistogram(hist, tally, cells, bpfd);
+  flushHistogram(hist, &cnt, tally, cells, bpfd);
 }
 
 // ---------------------------------------------------------------------------
@@ -655,9 +625,10 @@ template <int TY>
 __global__ __launch_bounds__(kWave) void kBPFD16(const uint32_t* __restrict__ pW, int X, int Y, int Z,
                                                  uint32_t rowWS, uint32_t planeWS, TileGrid g,
                                                  unsigned long long* __restrict__ bpfd) {
-  __shared__ uint32_t hist[1][256];
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t cnt;
   const int lane = threadIdx.x;
-  for (int i = lane; i < 256; i += kWave) hist[0][i] = 0;
+  clearHistogram(hist, &cnt);
   Tally tally;
   uint32_t cells = 0;
   for (int t = blockIdx.x; t < g.chunk * 8; t += gridDim.x) {
@@ -718,17 +689,13 @@ __global__ __launch_bounds__(kWave) void kBPFD16(const uint32_t* __restrict__ pW
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const uint32_t Hc = P[r][q] | (P[r + 1][q] << 2);
-          if (z > z0 && r < nY) {
-            const uint32_t C = Hp[r][q] | (Hc << 4);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) tally.add((C >> (8 * k)) & 255u, hist[0]);
-          }
+          if (z > z0 && r < nY) tally.add<4>(Hp[r][q] | (Hc << 4), hist);
           Hp[r][q] = Hc;
         }
       }
     }
   }
-  flushHistogram(hist, tally, cells, bpfd);
+  flushHistogram(hist, &cnt, tally, cells, bpfd);
 }
 
 // ---------------------------------------------------------------------------
