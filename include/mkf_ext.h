@@ -43,6 +43,29 @@ typedef struct {
 /* Text of the last error raised by any entry point in this thread. */
 const char *mkfLastError(void);
 
+/* Rows narrower than one PBM word (X < 32). The reference's row walk
+ * (src/bpfdCUDA.cuh:92-118, src/mkfACC.c:121-152) emits 2X - 1 patterns per
+ * row pair for them: its GPU walk reads the row's only word twice, which
+ * with zero tail bits counts every cell of the row pair twice plus the
+ * wrap-around cell (x = X-1 beside x = 0); its CPU walk reads the next row's
+ * word instead, and past the map on the last row. This process-wide mode
+ * (default MKF_NARROW_CELLS) sets what the drop-in histogram entry points
+ * (mkfCUDAGetBPFD[H], mkfCUDAGetBPFDautoCtx, mkfBPFDAsync, mkfAccGetBPFDSimple,
+ * mkfAccCUDAGetBPFD) do with such fields:
+ *   MKF_NARROW_CELLS      count the X-1 cells of each row pair (the histogram
+ *                         the fused, batched and serving forms always give);
+ *   MKF_NARROW_REFERENCE  the GPU walk's counts (mkfAccGetBPFDSimple, whose
+ *                         reference is the CPU walk, refuses);
+ *   MKF_NARROW_REJECT     refuse: NULL from mkfCUDAGetBPFD[H], 0 from the
+ *                         mkfAcc* calls, MKF_ERR_UNSUPPORTED from mkfBPFDAsync,
+ *                         the histogram untouched.
+ * With MKF_VERBOSE set, MKF_NARROW_CELLS logs each such call. Returns the
+ * previous mode, or MKF_ERR_ARG. No counterpart in the reference. */
+#define MKF_NARROW_CELLS 0
+#define MKF_NARROW_REFERENCE 1
+#define MKF_NARROW_REJECT 2
+int mkfSetNarrowRowMode(int mode);
+
 /* Test hook for error paths (no counterpart in the reference): the nth call
  * from now (1 = the next) of the named internal step fails as an allocation
  * or runtime error would, once; nth 0 disarms the point. Points:
@@ -311,6 +334,15 @@ void mkfServeDestroy(MkfServer *s);
  * with the lowest corner set}. Returns 4, or MKF_ERR_ARG for NULL, s < 0, or
  * an empty histogram with s > 0. Thread-safe. */
 int mkfCroftonMeasuresBPFD(double m[4], const size_t aBPFD[MKF_BINS], double s);
+
+/* The number of Kv terms (of 22 symmetry groups) that mkfSelectMeasuresBPFD
+ * computes, as the reference does (src/mkfUtil.c:64-69), in wrapping 32-bit
+ * arithmetic and that therefore differ from the exact Euler sum: the group's
+ * cell count does not fit 32 bits, or its product with the group weight
+ * wraps. 0: Kv is exact. -1 for NULL. With MKF_VERBOSE set,
+ * mkfSelectMeasuresBPFD reports a nonzero count on stderr. No counterpart
+ * in the reference. */
+int mkfKvWrapCount(const size_t aBPFD[MKF_BINS]);
 
 /* Read-only views of the measure tables used by mkfSelectMeasuresBPFD: the
  * symmetry group of each of the 256 patterns and the Euler weight of each of

@@ -109,6 +109,7 @@ MKF_SERVE_AFTER_STREAM = 2
 MKF_BATCH_MAX_LEVELS = 16
 MKF_MOM_N = 6
 MKF_NOALLOC_SCRATCH_BYTES = 4 << 20
+MKF_NARROW_CELLS, MKF_NARROW_REFERENCE, MKF_NARROW_REJECT = 0, 1, 2
 MKF_BATCH_ZERO_OUTPUT = 1
 
 _P = C.c_void_p
@@ -141,8 +142,10 @@ _SIGNATURES = {
     "mkfSelectMeasuresBPFD": (C.c_int, [C.POINTER(C.c_float), C.c_char_p, _SZP, C.c_float, C.c_int]),
     "volFrac": (C.c_float, [_SZP]),
     "mkfCroftonMeasuresBPFD": (C.c_int, [C.POINTER(C.c_double), _SZP, C.c_double]),
+    "mkfKvWrapCount": (C.c_int, [_SZP]),
     "mkfLastError": (C.c_char_p, []),
     "mkfDebugInjectFault": (C.c_int, [C.c_char_p, C.c_int]),
+    "mkfSetNarrowRowMode": (C.c_int, [C.c_int]),
     "mkfFusedLaunchInfo": (C.c_int, [C.POINTER(BMFieldInfo), C.POINTER(BinMapF64), C.POINTER(MkfTiling),
                                      C.POINTER(MkfLaunchInfo)]),
     "mkfBPFDLaunchInfo": (C.c_int, [C.POINTER(BMOrg), C.POINTER(MkfTiling), C.POINTER(MkfLaunchInfo)]),

@@ -111,29 +111,23 @@ int copyNonZeroDef(FieldDef d[], const FieldDef *pD, const int nD) {
   return rD;
 }
 
-/* Copies the non-NULL table entries enabled by mask, at most max of them.
- * Scans entries while mask has bits at or above the current one.
- * src/binMapUtil.c:107-121. */
+/* Copies the non-NULL table entries enabled by mask, in table order, at most
+ * max of them (src/binMapUtil.c:107-121 selects the same entries). An entry
+ * is read only when its mask bit is set: the reference's scan also reads the
+ * entry one past the highest set bit, which can lie outside the caller's
+ * table. */
 int copyValidPtrByMask(ConstFieldPtr r[], const int max, const ConstFieldPtr a[], const uint32_t mask) {
   int n = 0;
-  uint32_t bit;
-  int i = 0;
-  do {
-    bit = 1u << i;
-    if (a[i].p != NULL && (mask & bit)) r[n++] = a[i];
-    i++;
-  } while (mask > bit && n < max);
+  for (int i = 0; i < 32 && n < max && (mask >> i) != 0; i++)
+    if (((mask >> i) & 1u) && a[i].p != NULL) r[n++] = a[i];
   return n;
 }
 
-/* Counts the non-NULL table entries enabled by mask.  src/binMapUtil.c:123-133. */
+/* Counts the non-NULL table entries enabled by mask, reading only those
+ * entries.  src/binMapUtil.c:123-133. */
 int countValidPtrByMask(const ConstFieldPtr a[], uint32_t mask) {
-  int n = 0, i = 0;
-  do {
-    n += (a[i].p != NULL) && (mask & 1u);
-    mask >>= 1;
-    i++;
-  } while (mask > 0);
+  int n = 0;
+  for (int i = 0; i < 32 && (mask >> i) != 0; i++) n += ((mask >> i) & 1u) && a[i].p != NULL;
   return n;
 }
 

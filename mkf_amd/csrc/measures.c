@@ -17,6 +17,8 @@
  */
 #include <math.h>
 #include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "mkf.h"
@@ -61,6 +63,23 @@ static double eulerSumK(const size_t g[CELL8_SYMM_GROUPS]) {
   return (double)k * M_PI / 6;
 }
 
+/* Kv terms (symmetry groups) the reference's 32-bit arithmetic does not
+ * give exactly: the group count does not fit 32 bits, or its product with the
+ * weight wraps. */
+int mkfKvWrapCount(const size_t aBPFD[MKF_BINS]) {
+  if (!aBPFD) return -1;
+  size_t g[CELL8_SYMM_GROUPS] = {0};
+  for (int i = 0; i < MKF_BINS; i++) g[kPatternGroup[i]] += aBPFD[i];
+  int n = 0;
+  for (int i = 0; i < CELL8_SYMM_GROUPS; i++) {
+    const int32_t wrapped = (int32_t)((uint32_t)(int32_t)kGroupWeightK[i] * (uint32_t)g[i]);
+    const int exactFits = g[i] <= (size_t)INT32_MAX &&
+                          (int64_t)kGroupWeightK[i] * (int64_t)g[i] == (int64_t)wrapped;
+    if (kGroupWeightK[i] != 0 && !exactFits) n++;
+  }
+  return n;
+}
+
 /* src/mkfUtil.c:125-135: odd bins (lowest corner set) over all bins. */
 float volFrac(const size_t aBPFD[MKF_BINS]) {
   size_t s0 = 0, s1 = 0;
@@ -94,6 +113,10 @@ int mkfSelectMeasuresBPFD(float m[4], char symCh[], const size_t aBPFD[MKF_BINS]
     m[0] *= rkV;
   }
   memcpy(symCh, "Kv       Vv", 12);
+  if (getenv("MKF_VERBOSE")) {
+    const int w = mkfKvWrapCount(aBPFD);
+    if (w > 0) fprintf(stderr, "[mkf] mkfSelectMeasuresBPFD: %d Kv terms wrap in 32-bit arithmetic, as in the reference\n", w);
+  }
   return 2;
 }
 

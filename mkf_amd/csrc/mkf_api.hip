@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -56,6 +57,12 @@ std::mutex gFaultLock;
 std::map<std::string, int> gFaults;
 
 }  // namespace
+
+namespace {
+std::atomic<int> gNarrowMode{MKF_NARROW_CELLS};
+}  // namespace
+
+int mkf::narrowRowMode() { return gNarrowMode.load(std::memory_order_relaxed); }
 
 bool mkf::faultDue(const char* point) {
   std::lock_guard<std::mutex> g(gFaultLock);
@@ -200,6 +207,12 @@ extern "C" {
 
 const char* mkfLastError(void) { return gErr; }
 
+int mkfSetNarrowRowMode(int mode) {
+  if (mode != MKF_NARROW_CELLS && mode != MKF_NARROW_REFERENCE && mode != MKF_NARROW_REJECT)
+    return fail(MKF_ERR_ARG, "narrow-row mode %d: use MKF_NARROW_CELLS, _REFERENCE or _REJECT", mode);
+  return gNarrowMode.exchange(mode);
+}
+
 int mkfDebugInjectFault(const char* point, int nth) {
   if (!point || nth < 0) return fail(MKF_ERR_ARG, "fault injection: NULL point or negative count");
   static const char* const known[] = {"batch.upload", "batch.capture"};
@@ -282,6 +295,20 @@ void toLaunchInfo(const LaunchInfo& li, MkfLaunchInfo* out) {
 int mkfBPFDAsync(size_t* pBPFD, const BMOrg* pO, const BMPackWord* pW, const MkfTiling* pT, void* stream) {
   if (!pBPFD || !pO || !pW) return fail(MKF_ERR_ARG, "NULL argument");
   const int X = (int)pO->rowElem, Y = (int)pO->rowPairs + 1, Z = (int)pO->planePairs + 1;
+  if (X >= 2 && X < 32 && Y >= 2 && Z >= 2) {
+    const int mode = narrowRowMode();
+    if (mode == MKF_NARROW_REJECT)
+      return fail(MKF_ERR_UNSUPPORTED, "rows of %d voxels: the reference's walk counts 2X-1 patterns per row pair "
+                                       "(narrow-row mode: reject)", X);
+    if (mode == MKF_NARROW_REFERENCE) {
+      const hipError_t e = launchBPFDNarrowRef(pW, X, Y, Z, pO->rowWS, pO->planeWS, (unsigned long long*)pBPFD,
+                                               (hipStream_t)stream);
+      return e == hipSuccess ? MKF_OK : hipFail(e, "narrow-row reference walk");
+    }
+    if (getenv("MKF_VERBOSE"))
+      fprintf(stderr, "[mkf] rows of %d voxels: counting their X-1 cells (the reference's walk counts 2X-1 patterns; "
+                      "see mkfSetNarrowRowMode)\n", X);
+  }
   const Tiling tl = bpfdTiling(pT, X, Y, Z);
   int rc = checkTiling(tl, 4);
   if (rc) return rc;
@@ -667,6 +694,7 @@ size_t* mkfCUDAGetBPFD(KernInfo* pK, size_t* pBPFD, const BMOrg* pO, const BMPac
   if (pBPFD && pO && pW) {
     EventTimer t(0);
     const int rc = mkfBPFDAsync(pBPFD, pO, pW, NULL, 0);
+    if (rc == MKF_ERR_UNSUPPORTED) return NULL;  // narrow rows refused: histogram untouched
     const float ms = t.stopMs();
     if (pK) {
       pK->dtms[1] = ms;
@@ -746,6 +774,12 @@ int mkfCUDAGetBPFDautoCtx(Context* pC, const FieldDef def[3], const BinMapF64* p
 Bool32 mkfAccGetBPFDSimple(size_t rBPFD[MKF_BINS], BMPackWord* pW, const double* pF, const FieldDef def[3],
                            const BinMapF64* pM) {
   if (!rBPFD || !pW || !pF || !def || !pM) return 0;
+  if (def[0] < 32 && narrowRowMode() != MKF_NARROW_CELLS) {
+    // The reference's CPU walk reads the next row's word for such rows (and
+    // past the map on the last one): there is no defined result to reproduce.
+    fail(MKF_ERR_UNSUPPORTED, "mkfAccGetBPFDSimple: rows of %d voxels (narrow-row mode %d)", def[0], narrowRowMode());
+    return 0;
+  }
   memset(rBPFD, 0, sizeof(size_t) * MKF_BINS);
   const size_t nF = (size_t)def[0] * def[1] * def[2];
   BMOrg org;
@@ -784,6 +818,41 @@ Bool32 mkfAccGetBPFDSimple(size_t rBPFD[MKF_BINS], BMPackWord* pW, const double*
 Bool32 mkfAccCUDAGetBPFD(size_t rBPFD[MKF_BINS], const void* pF, const FieldDef def[3], const NumEnc enc,
                          const BinMapF64* pM) {
   if (!rBPFD || !pF || !def || !pM || (enc != ENC_F32 && enc != ENC_F64)) return 0;
+  FieldDef d[3];
+  const int nd = copyNonZeroDef(d, def, 3);
+  if (nd > 0 && d[0] < 32 && d[1] >= 2 && d[2] >= 2 && narrowRowMode() != MKF_NARROW_CELLS) {
+    // The reference maps the field and walks the map on the GPU: for rows of
+    // X < 32 its walk counts 2X-1 patterns per row pair. Reproduce (or refuse)
+    // that with the whole field on the device.
+    if (narrowRowMode() == MKF_NARROW_REJECT) {
+      fail(MKF_ERR_UNSUPPORTED, "mkfAccCUDAGetBPFD: rows of %d voxels (narrow-row mode: reject)", d[0]);
+      return 0;
+    }
+    memset(rBPFD, 0, sizeof(size_t) * MKF_BINS);
+    const size_t eb = enc == ENC_F32 ? 4 : 8, nF = (size_t)d[0] * d[1] * d[2];
+    BMOrg org;
+    const size_t words = setBMO(&org, d, 0);
+    void* dF = nullptr;
+    BMPackWord* dW = nullptr;
+    unsigned long long* dH = nullptr;
+    Bool32 ok = 0;
+    if (hipMalloc(&dF, nF * eb) == hipSuccess && hipMalloc((void**)&dW, words * 4) == hipSuccess &&
+        hipMalloc((void**)&dH, MKF_BINS * 8) == hipSuccess &&
+        hipMemcpy(dF, pF, nF * eb, hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemset(dH, 0, MKF_BINS * 8) == hipSuccess) {
+      const FieldView fv = planarField(dF, enc, d[0], d[1], d[2]);
+      ok = launchPBM(fv, *pM, dW, org.rowWS, org.planeWS, 0) == hipSuccess &&
+           launchBPFDNarrowRef(dW, d[0], d[1], d[2], org.rowWS, org.planeWS, dH, 0) == hipSuccess &&
+           hipMemcpy(rBPFD, dH, MKF_BINS * 8, hipMemcpyDeviceToHost) == hipSuccess;
+      if (!ok) hipFail(hipGetLastError(), "mkfAccCUDAGetBPFD");
+    } else {
+      fail(MKF_ERR_MEM, "mkfAccCUDAGetBPFD: device buffers for %zu voxels", nF);
+    }
+    if (dF) hipFree(dF);
+    if (dW) hipFree(dW);
+    if (dH) hipFree(dH);
+    return ok;
+  }
   memset(rBPFD, 0, sizeof(size_t) * MKF_BINS);
   return mkfHostStreamBPFD(rBPFD, pF, enc, def, pM, 0) == MKF_OK;
 }

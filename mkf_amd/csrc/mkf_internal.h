@@ -99,6 +99,15 @@ hipError_t launchFusedMoments(const FieldView& fv, const BinMapF64& m, unsigned 
 hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
                       unsigned long long* bpfd, const Tiling& tl, hipStream_t s, LaunchInfo* info = nullptr);
 
+// Adds the counts of the reference's GPU row walk for rows of X < 32 (2X - 1
+// patterns per row pair) of the PBM pW to bpfd.
+hipError_t launchBPFDNarrowRef(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
+                               unsigned long long* bpfd, hipStream_t s);
+
+// How the drop-in histogram entry points treat rows of X < 32
+// (mkfSetNarrowRowMode).
+int narrowRowMode();
+
 // Copies bytes (a multiple of 16) with a kernel on s; either side may be
 // the device address of mapped pinned host memory.
 hipError_t copyWords(void* dst, const void* src, size_t bytes, hipStream_t s);

@@ -1407,6 +1407,52 @@ hipError_t launchFusedMoments(const FieldView& fv, const BinMapF64& m, unsigned 
 #endif
 
 #if MKF_PART(6)
+// The reference's GPU row walk for rows narrower than one PBM word
+// (src/bpfdCUDA.cuh:92-118 with n = X < 32), for callers that ask for its
+// counts: four 64-bit registers hold the row pair's words in planes z and
+// z+1; the word gives X - 1 patterns (bits 0..1 of every register, then all
+// shift right by one), then the same word is ORed in again shifted up by one
+// and gives X more. One lane per (row pair, plane pair); LDS histogram per
+// workgroup, flushed with one 64-bit atomic per nonzero bin.
+__global__ __launch_bounds__(256) void kBPFDNarrowRef(const uint32_t* __restrict__ pW, int X, int Y, int Z,
+                                                      uint32_t rowWS, uint32_t planeWS,
+                                                      unsigned long long* __restrict__ bpfd) {
+  __shared__ uint32_t hist[256];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  const long long pairs = (long long)(Y - 1) * (Z - 1);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < pairs;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long y = i % (Y - 1), z = i / (Y - 1);
+    const uint32_t* p0 = pW + z * planeWS + y * rowWS;
+    const uint32_t w[4] = {p0[0], p0[rowWS], p0[planeWS], p0[planeWS + rowWS]};
+    uint64_t u[4] = {w[0], w[1], w[2], w[3]};
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int j = 0; j < X - 1 + pass; ++j) {
+        const uint32_t code =
+            (uint32_t)((u[0] & 3u) | ((u[1] & 3u) << 2) | ((u[2] & 3u) << 4) | ((u[3] & 3u) << 6));
+        atomicAdd(hist + code, 1u);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) u[q] >>= 1;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) u[q] |= (uint64_t)w[q] << 1;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 256; i += blockDim.x)
+    if (hist[i]) atomicAdd(bpfd + i, (unsigned long long)hist[i]);
+}
+
+hipError_t launchBPFDNarrowRef(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
+                               unsigned long long* bpfd, hipStream_t s) {
+  if (X < 2 || X >= 32 || Y < 2 || Z < 2) return X >= 32 ? hipErrorInvalidValue : hipSuccess;
+  const long long pairs = (long long)(Y - 1) * (Z - 1);
+  const unsigned blocks = (unsigned)std::min<long long>(4096, (pairs + 255) / 256);
+  hipLaunchKernelGGL(kBPFDNarrowRef, dim3(blocks), dim3(256), 0, s, pW, X, Y, Z, rowWS, planeWS, bpfd);
+  return hipGetLastError();
+}
+
 hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, uint32_t planeWS,
                       unsigned long long* bpfd, const Tiling& tl0, hipStream_t s, LaunchInfo* info) {
   if (info) *info = LaunchInfo{0, 0, 0, tl0};

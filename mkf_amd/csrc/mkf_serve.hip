@@ -22,6 +22,7 @@
 #include <thread>
 #include <vector>
 
+#include "mkf_arena.hpp"
 #include "mkf_internal.h"
 
 using namespace mkf;
@@ -34,22 +35,21 @@ constexpr int kSlots = 2;
 // Host copies queued on the copy stream at once: enough to keep the link
 // busy, few enough that waiting host requests hold no device memory.
 constexpr int kMaxCopies = 3;
-constexpr size_t kAlign = 256;
 
-size_t alignUp(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
+size_t alignUp(size_t v) { return ArenaMap::alignUp(v); }
 
-// First-fit allocator over one device allocation; freed blocks merge with
-// their neighbours.
+// The device reservation for copied-in fields; the first-fit bookkeeping is
+// ArenaMap's.
 class Arena {
  public:
   int init(size_t bytes) {
-    cap_ = alignUp(bytes);
-    if (cap_ == 0) return MKF_OK;
-    if (hipMalloc((void**)&base_, cap_) != hipSuccess) {
+    map_.reset(bytes);
+    if (map_.capacity() == 0) return MKF_OK;
+    if (hipMalloc((void**)&base_, map_.capacity()) != hipSuccess) {
       base_ = nullptr;
-      return setError(MKF_ERR_MEM, "serve: cannot reserve %zu device bytes", cap_);
+      map_.reset(0);
+      return setError(MKF_ERR_MEM, "serve: cannot reserve %zu device bytes", ArenaMap::alignUp(bytes));
     }
-    free_[0] = cap_;
     return MKF_OK;
   }
   void release() {
@@ -57,42 +57,16 @@ class Arena {
     base_ = nullptr;
   }
   void* alloc(size_t n) {
-    n = alignUp(n);
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second < n) continue;
-      const size_t off = it->first, rest = it->second - n;
-      free_.erase(it);
-      if (rest) free_[off + n] = rest;
-      used_ += n;
-      return base_ + off;
-    }
-    return nullptr;
+    const size_t off = map_.alloc(n);
+    return off == ArenaMap::kNone ? nullptr : base_ + off;
   }
-  void free(void* p, size_t n) {
-    n = alignUp(n);
-    size_t off = (size_t)((char*)p - base_);
-    used_ -= n;
-    auto next = free_.lower_bound(off);
-    if (next != free_.end() && off + n == next->first) {
-      n += next->second;
-      next = free_.erase(next);
-    }
-    if (next != free_.begin()) {
-      auto prev = std::prev(next);
-      if (prev->first + prev->second == off) {
-        prev->second += n;
-        return;
-      }
-    }
-    free_[off] = n;
-  }
-  size_t capacity() const { return cap_; }
-  size_t used() const { return used_; }
+  void free(void* p, size_t n) { map_.release((size_t)((char*)p - base_), n); }
+  size_t capacity() const { return map_.capacity(); }
+  size_t used() const { return map_.used(); }
 
  private:
   char* base_ = nullptr;
-  size_t cap_ = 0, used_ = 0;
-  std::map<size_t, size_t> free_;
+  ArenaMap map_;
 };
 
 // A host request whose field does not fit the budget is served as z-slabs

@@ -9,6 +9,7 @@
  *   orc_pbm_f64/_f32    src/binMapACC.c:43-93      (packed binary map)
  *   orc_bpfd_chunk      src/mkfACC.c:28-188        (64-bit chunk walk)
  *   orc_bpfd_field_*    src/mkfACC.c:206-239       (mkfAccGetBPFDSimple)
+ *   orc_bpfd_gpu_walk   src/bpfdCUDA.cuh:92-118    (the GPU row walk, any X)
  *   orc_measures        src/mkfUtil.c:64-135       (Kv, Vv)
  * and adds second opinions that do not share code with the restatement:
  *   orc_bpfd_direct     per-cell enumeration of the cell-code definition
@@ -233,6 +234,50 @@ void orc_bpfd_direct(uint64_t h[ORC_BINS], const uint32_t *pW, int X, int Y, int
         for (int i = 0; i < 8; i++) code |= orc_bit(pW, rowWS, planeWS, x + (i & 1), y + ((i >> 1) & 1), z + (i >> 2)) << i;
         h[code]++;
       }
+}
+
+/* ---- histogram: the reference GPU row walk ------------------------------ */
+
+/* addRowBPFD of the GPU path (src/bpfdCUDA.cuh:41-118): four 64-bit
+ * registers hold rows (y,z), (y+1,z), (y,z+1), (y+1,z+1) of a row pair; each
+ * pattern takes bits 0..1 of every register (in that order, 2 bits each) and
+ * then shifts all four right by one. The first word gives min(32, n) - 1
+ * patterns; each later whole word is ORed in shifted up by one and gives 32;
+ * a residual of n & 31 bits ORs in word n >> 5 the same way and gives that
+ * many. For n < 32 the residual step reads word 0 again. ADDS to h. */
+static void orc_gpu_emit(uint64_t h[ORC_BINS], uint64_t u[4], int cnt) {
+  for (int j = 0; j < cnt; j++) {
+    h[(u[0] & 3u) | ((u[1] & 3u) << 2) | ((u[2] & 3u) << 4) | ((u[3] & 3u) << 6)]++;
+    for (int q = 0; q < 4; q++) u[q] >>= 1;
+  }
+}
+
+static void orc_gpu_row(uint64_t h[ORC_BINS], const uint32_t *r00, const uint32_t *r01, const uint32_t *r10,
+                        const uint32_t *r11, int n) {
+  const uint32_t *r[4] = {r00, r01, r10, r11};
+  uint64_t u[4] = {r00[0], r01[0], r10[0], r11[0]};
+  const int m = n >> 5, k = n & 31;
+  orc_gpu_emit(h, u, (n < 32 ? n : 32) - 1);
+  for (int i = 1; i < m; i++) {
+    for (int q = 0; q < 4; q++) u[q] |= ((uint64_t)r[q][i]) << 1;
+    orc_gpu_emit(h, u, 32);
+  }
+  if (k > 0) {
+    for (int q = 0; q < 4; q++) u[q] |= ((uint64_t)r[q][m]) << 1;
+    orc_gpu_emit(h, u, k);
+  }
+}
+
+/* The reference GPU walk (src/bpfdCUDA.cuh:158-206 over src/bpfdCUDA.cuh:92-118)
+ * over every row pair and plane pair of the map: ADDS to h. For X >= 32 it
+ * equals the per-cell enumeration; for X < 32 it emits 2X - 1 patterns per
+ * row pair. */
+void orc_bpfd_gpu_walk(uint64_t h[ORC_BINS], const uint32_t *pW, int X, int Y, int Z, int rowWS, long planeWS) {
+  for (int z = 0; z + 1 < Z; z++)
+    for (int y = 0; y + 1 < Y; y++) {
+      const uint32_t *p0 = pW + (long)z * planeWS + (long)y * rowWS, *p1 = p0 + planeWS;
+      orc_gpu_row(h, p0, p0 + rowWS, p1, p1 + rowWS, X);
+    }
 }
 
 /* ---- whole CPU path (mkfAccGetBPFDSimple) ------------------------------- */

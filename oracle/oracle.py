@@ -74,6 +74,7 @@ def load():
         "orc_bpfd_chunk": (I, [_U64P, _U32P, I, I, I, I, L]),
         "orc_bpfd_chunk_range": (I, [_U64P, _U32P, I, I, I, I, L, I, I]),
         "orc_bpfd_direct": (None, [_U64P, _U32P, I, I, I, I, L]),
+        "orc_bpfd_gpu_walk": (None, [_U64P, _U32P, I, I, I, I, L]),
         "orc_bpfd_field_f64": (I, [_U64P, _U32P, _F64P, I, I, I, D, U]),
         "orc_bpfd_field_f32": (I, [_U64P, _U32P, _F32P, I, I, I, D, U]),
         "orc_num_threads": (I, []),
@@ -214,6 +215,36 @@ def bpfd_direct_c(words, X):
     h = np.zeros(256, dtype=np.uint64)
     load().orc_bpfd_direct(h, w, int(X), Y, Z, nW, nW * Y)
     return h
+
+
+def bpfd_gpu_walk(words, X):
+    """The reference GPU row walk (src/bpfdCUDA.cuh:92-118) over a (Z, Y,
+    rowWS) PBM: equal to the cell histogram for X >= 32; for X < 32 it emits
+    2X - 1 patterns per row pair."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    Z, Y, nW = w.shape
+    h = np.zeros(256, dtype=np.uint64)
+    if min(X, Y, Z) < 2:
+        return h
+    load().orc_bpfd_gpu_walk(h, w, int(X), Y, Z, nW, nW * Y)
+    return h
+
+
+def narrow_walk_numpy(words, X):
+    """Closed form of the GPU row walk for X < 32 on a map with zero tail
+    bits, from the register contents alone: after the first word's X - 1
+    patterns the registers hold bit X - 1 of each row; ORing the same word
+    in, shifted up by one, leaves bits (X-1, 0, 1, ..., X-1), whose X
+    patterns are the wrap-around cell (x = X-1 beside x = 0) and the cells
+    0..X-2 again. So the walk is twice the cell histogram plus the histogram
+    of the wrap-around cells."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    Z, Y, nW = w.shape
+    assert X < 32 and nW == 1
+    bits = ((w[:, :, 0, None] >> np.arange(X, dtype=np.uint32)) & 1).astype(np.uint8)  # (Z, Y, X)
+    cells = bpfd_numpy(bits)
+    wrap = np.stack([bits[:, :, X - 1], bits[:, :, 0]], axis=2)  # (Z, Y, 2): x-low = X-1, x-high = 0
+    return (2 * cells + bpfd_numpy(wrap)).astype(np.uint64)
 
 
 def bpfd_field(field, t, m):

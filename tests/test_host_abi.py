@@ -281,3 +281,93 @@ def test_fault_points_validate_their_arguments(lib):
     assert lib.mkfDebugInjectFault(b"batch.upload", -1) == M.MKF_ERR_ARG
     assert lib.mkfDebugInjectFault(b"batch.upload", 3) == M.MKF_OK
     assert lib.mkfDebugInjectFault(b"batch.upload", 0) == M.MKF_OK
+
+
+def test_narrow_row_mode_setter(lib):
+    """mkfSetNarrowRowMode returns the previous mode and refuses unknown ones
+    (no GPU work)."""
+    prev = lib.mkfSetNarrowRowMode(M.MKF_NARROW_REFERENCE)
+    try:
+        assert prev == M.MKF_NARROW_CELLS
+        assert lib.mkfSetNarrowRowMode(M.MKF_NARROW_REJECT) == M.MKF_NARROW_REFERENCE
+        assert lib.mkfSetNarrowRowMode(7) == M.MKF_ERR_ARG and "narrow-row mode" in M.last_error()
+        assert lib.mkfSetNarrowRowMode(M.MKF_NARROW_CELLS) == M.MKF_NARROW_REJECT
+    finally:
+        lib.mkfSetNarrowRowMode(M.MKF_NARROW_CELLS)
+
+
+def _kv_terms(lib, h):
+    """Group sums and weights from the library's tables (checked against
+    their derivation in test_tables_match_derivation)."""
+    group = np.ctypeslib.as_array(lib.mkfPatternGroupTable(), shape=(256,)).astype(np.int64)
+    weight = np.ctypeslib.as_array(lib.mkfGroupWeightKTable(), shape=(22,)).astype(np.int64)
+    g = np.zeros(22, dtype=np.uint64)
+    np.add.at(g, group, np.asarray(h, dtype=np.uint64))
+    return g, weight
+
+
+def _kv_op_order(lib, h, s):
+    """Kv restated with numpy scalars in the reference's order and
+    precisions (src/mkfUtil.c:64-69, 108-115): 32-bit wrapping products
+    summed in int64, times pi/6 in double, stored as float; then times the
+    float 1 / (s^3 t) whose denominator is formed in float and inverted in
+    double."""
+    g, w = _kv_terms(lib, h)
+    k = 0
+    for gi, wi in zip(g, w):
+        p = (int(wi) & 0xFFFFFFFF) * (int(gi) & 0xFFFFFFFF) & 0xFFFFFFFF
+        k += p - (1 << 32) if p >= 1 << 31 else p
+    m0 = np.float32(np.float64(k) * np.float64(math.pi) / np.float64(6))
+    s32 = np.float32(s)
+    denom = (s32 * s32 * s32) * np.float32(int(np.asarray(h, dtype=np.uint64).sum()))
+    return np.float32(m0 * np.float32(1.0 / np.float64(denom)))
+
+
+def test_kv_density_scaling_follows_float_op_order(lib, orc):
+    """s > 0: the library's Kv equals the float/double op-order restatement
+    on cases chosen so that evaluating 1/(s^3 t) in double throughout rounds
+    differently; a library that scaled in double would fail here."""
+    rng = np.random.default_rng(12)
+    found = 0
+    for _ in range(4000):
+        h = rng.integers(0, 10 ** 7, size=256, dtype=np.uint64)
+        s = float(rng.uniform(0.001, 0.5))
+        want = _kv_op_order(lib, h, s)
+        g, w = _kv_terms(lib, h)
+        naive = np.float32(float(int((g.astype(np.int64) * w).sum())) * math.pi / 6
+                           / (float(np.float32(s)) ** 3 * float(h.sum())))
+        if want == naive:
+            continue
+        found += 1
+        _, m, _ = product_measures(lib, h, s)
+        mo, _ = orc.measures(h, s)
+        assert m[0] == want and mo[0] == want and m[0] != naive
+        if found == 20:
+            break
+    assert found == 20
+
+
+def test_kv_wrap_count(lib, orc):
+    """mkfKvWrapCount: 0 where Kv is exact, else the number of wrapped terms,
+    and nonzero exactly where the oracle reports a wrap."""
+    def count(h):
+        a = np.ascontiguousarray(h, dtype=np.uint64)
+        return lib.mkfKvWrapCount(a.ctypes.data_as(C.POINTER(C.c_size_t)))
+    rng = np.random.default_rng(3)
+    h = rng.integers(0, 10 ** 6, size=256, dtype=np.uint64)
+    assert count(h) == 0 and not orc.measures(h, 0.0)[1]
+    group, weight = (np.ctypeslib.as_array(lib.mkfPatternGroupTable(), shape=(256,)),
+                     np.ctypeslib.as_array(lib.mkfGroupWeightKTable(), shape=(22,)))
+    # one pattern of each of two weighted groups with a count whose product wraps
+    pats = [p for p in range(256) if weight[group[p]] != 0]
+    g1 = pats[0]
+    g2 = next(p for p in pats if group[p] != group[g1])
+    w1 = abs(int(weight[group[g1]]))
+    h = np.zeros(256, dtype=np.uint64)
+    h[g1] = (2 ** 31) // w1 + 1          # |w| x n just above 2^31: wraps
+    assert count(h) == 1 and orc.measures(h, 0.0)[1]
+    h[g2] = 2 ** 33 + 7                  # does not fit 32 bits
+    assert count(h) == 2 and orc.measures(h, 0.0)[1]
+    h[0] = 2 ** 40                       # group 0 has weight 0: never counted
+    assert count(h) == 2
+    assert lib.mkfKvWrapCount(None) == -1

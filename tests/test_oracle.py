@@ -261,3 +261,29 @@ def test_volume_fraction_is_lower_corner_count(orc):
     lower = bits[:-1, :-1, :-1]
     assert m[3] == np.float32(lower.sum()) / np.float32(lower.size)
     assert m[1] == -1 and m[2] == -1
+
+
+@pytest.mark.parametrize("X", [2, 3, 5, 8, 17, 31])
+def test_gpu_walk_narrow_rows_closed_form(orc, X):
+    """The reference GPU row walk restated (orc_bpfd_gpu_walk) on rows of X <
+    32 equals its closed form from the register contents (twice the cells
+    plus the wrap-around cell per row pair), emits 2X - 1 patterns per row
+    pair, and differs from the cell histogram."""
+    rng = np.random.default_rng(X)
+    Z, Y = 7, 9
+    bits = rng.random((Z, Y, X)) < 0.45
+    words = orc.pbm_numpy(bits.astype(np.float64), 0.5, 6)  # bit = (v >= 0.5), zero tail bits
+    walk = orc.bpfd_gpu_walk(words, X)
+    np.testing.assert_array_equal(walk, orc.narrow_walk_numpy(words, X))
+    assert int(walk.sum()) == (2 * X - 1) * (Y - 1) * (Z - 1)
+    cells = orc.bpfd_numpy(bits)
+    assert not np.array_equal(walk, cells)
+
+
+@pytest.mark.parametrize("X", [32, 33, 63, 64, 65, 100, 129])
+def test_gpu_walk_equals_cells_from_one_word(orc, X):
+    """From X = 32 on, the GPU row walk counts exactly the row's cells."""
+    rng = np.random.default_rng(100 + X)
+    bits = rng.random((5, 6, X)) < 0.5
+    words = orc.pbm_numpy(bits.astype(np.float64), 0.5, 6)
+    np.testing.assert_array_equal(orc.bpfd_gpu_walk(words, X), orc.bpfd_numpy(bits))
