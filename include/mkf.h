@@ -158,7 +158,10 @@ void **autoFieldPtr(ConstFieldPtr ptr[BMCU_FIELD_MAX], const Context *pC);
  * *pO, blocks until done, stores the kernel time in pK->dtms[0] when pK is
  * given. Returns pW, or NULL on invalid input, allocation or launch failure.
  * Replaces binMapCUDA, src/binMapCUDA.cu:145-264. Tail bits x >= X of each
- * row's last word are written as zero. */
+ * row's last word are written as zero. A partial organisation (profID &
+ * 0x0F != 0; setBMO: min(2, Y) rows of min(2, Z) planes) maps exactly those
+ * rows and planes of the field at the table's pointer, so stepping the
+ * pointer walks the field in row-pair windows. */
 BMPackWord *binMapCUDA(KernInfo *pK, BMPackWord *pW, BMOrg *pO, const BMFieldInfo *pF, const BinMapF64 *pM);
 /* Frees the library-owned PBM buffer.  Replaces src/binMapCUDA.cu:267-270. */
 void binMapCUDACleanup(void);

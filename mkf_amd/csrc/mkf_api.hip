@@ -645,8 +645,14 @@ BMPackWord* binMapCUDA(KernInfo* pK, BMPackWord* pW, BMOrg* pO, const BMFieldInf
   const size_t words = setBMO(&org, def, (char)pI->profID);
   if (0 == words) return NULL;
   if (pI->profID & 0x0F) {
-    fail(MKF_ERR_UNSUPPORTED, "partial PBM buffer (profID 0x%02X)", pI->profID);
-    return NULL;
+    // Partial organisation (src/binMapUtil.c:60-87): a buffer of min(2, Y)
+    // rows of min(2, Z) planes. Map exactly those rows and planes of the
+    // field at the table's pointer (same strides), so the words written fit
+    // the buffer; stepping the pointer by rows and planes walks the field in
+    // row-pair windows. (The reference's kernels write the whole field into
+    // this buffer.)
+    fv.Y = (int)org.rowPairs + 1;
+    fv.Z = (int)org.planePairs + 1;
   }
   if (pO) *pO = org;
   if (NULL == pW) {
