@@ -600,7 +600,7 @@ def main():
                          "kernel for FETCH_SIZE / WRITE_SIZE); with --mode batch, 3 batched launches; prints no JSON")
     ap.add_argument("--pmc-target", choices=("fused", "bpfd", "pbm"), default="fused",
                     help="with --pmc-workload: 'bpfd' makes the PBM once and runs 3 histograms of it instead "
-                         "(with --tiling's tiling, else the library's default); 'pbm' runs 3 map passes")
+                         "(with --tiling's tiling, else the library's default); 'pbm' runs 3 map passes (then the copies)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -676,19 +676,19 @@ def main():
         torch.cuda.synchronize()
         log("[bench] counter workload done: 1 PBM, 3 histograms of the PBM")
         return
-    if args.pmc_workload and args.pmc_target == "pbm":
-        # 3 map passes of the field into a caller buffer.
-        org = M.BMOrg()
-        nwords = lib.setBMO(C.byref(org), M.dims((S, S, S)), b"\x00")
-        pbm = torch.empty(nwords, dtype=torch.int32, device=dev)
-        for _ in range(3):
-            M.check(lib.mkfPBMAsync(pbm.data_ptr(), C.byref(org), fi.ref, C.byref(bm), sptr), "mkfPBMAsync")
-        torch.cuda.synchronize()
-        log("[bench] counter workload done: 3 map passes")
-        return
     if args.pmc_workload:
-        for _ in range(3):
-            launch()
+        if args.pmc_target == "pbm":
+            # 3 map passes of the field into a caller buffer.
+            org = M.BMOrg()
+            nwords = lib.setBMO(C.byref(org), M.dims((S, S, S)), b"\x00")
+            pbm = torch.empty(nwords, dtype=torch.int32, device=dev)
+            for _ in range(3):
+                M.check(lib.mkfPBMAsync(pbm.data_ptr(), C.byref(org), fi.ref, C.byref(bm), sptr), "mkfPBMAsync")
+            torch.cuda.synchronize()
+            del pbm
+        else:
+            for _ in range(3):
+                launch()
         torch.cuda.synchronize()
         del field
         torch.cuda.empty_cache()
@@ -698,7 +698,8 @@ def main():
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
-        log("[bench] counter workload done: 3 fused launches, 3 copies of 1 GiB")
+        log(f"[bench] counter workload done: 3 {'map passes' if args.pmc_target == 'pbm' else 'fused launches'}, "
+            "3 copies of 1 GiB")
         return
 
     leg = fused_leg(lib, fi, bm, tiling, hist, stream, args.steps, args.warmup, f"{S}^3 {args.dtype}")

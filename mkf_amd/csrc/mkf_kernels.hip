@@ -15,13 +15,13 @@
 #include "mkf_device.hpp"
 #include "mkf_internal.h"
 
-// The file is compiled once per part (-DMKF_KPART=0..6), each part holding
+// The file is compiled once per part (-DMKF_KPART=0..7), each part holding
 // only its own launchers and so instantiating only its own kernels; the
 // parts compile in parallel. Without MKF_KPART one compile holds them all.
 //   0 tiling choice and the fused dispatcher   4 kFused16
 //   1 kPBM and the moment reduction            5 kFusedBPFD with moments
 //   2 kFusedBPFD for float fields              6 kBPFD4 / kBPFD16 / kBPFD
-//   3 kFusedBPFD for double fields
+//   3 kFusedBPFD for double fields             7 kPBM for double fields
 #ifdef MKF_KPART
 #define MKF_PART(p) (MKF_KPART == (p))
 #else
@@ -44,18 +44,25 @@ namespace mkf {
 // (Each lane holding 16 consecutive voxels, i.e. four lane-strided 16-byte
 // loads, was 7-10 % slower at 1024^3 and could not use non-temporal loads;
 // 256 voxels per wave with 32-byte stores was slower still.) A wave owns one
-// 1024-voxel segment and a run of consecutive rows (y fastest, then z),
-// kPbmAhead rows at a time; rows advance by incrementing (y, z), with no
-// division.
+// 1024-voxel segment and every C-th row of it (rows counted y fastest, then
+// z; C waves per segment), kPbmAhead rows at a time; rows advance by adding
+// (C mod Y, C / Y) to (y, z), with no division.
 #ifndef MKF_PBM_AHEAD
 #define MKF_PBM_AHEAD 2
 #endif
 #ifndef MKF_PBM_WAVE_TARGET
 #define MKF_PBM_WAVE_TARGET 16384
 #endif
+#ifndef MKF_PBM_INTERLEAVE
+#define MKF_PBM_INTERLEAVE 1
+#endif
+#ifndef MKF_PBM_STAGE
+#define MKF_PBM_STAGE 128
+#endif
 constexpr int kPbmWaves = 8;
 constexpr int kPbmAhead = MKF_PBM_AHEAD;
 constexpr int kPbmVoxels = 16;  // per lane
+constexpr int kPbmStage = MKF_PBM_STAGE;  // map rows staged per wave
 
 // Phase moments (MOM): each lane also sums, in double, the field values of
 // its voxels and their squares per phase (map bit 0 or 1) and counts them
@@ -64,11 +71,21 @@ constexpr int kPbmVoxels = 16;  // per lane
 // partials in a fixed order, so the result does not depend on scheduling.
 // With pW == nullptr no map is written.
 
+// Rows of a segment are dealt to its waves either as runs (wave c takes rows
+// c R .. c R + R - 1, rowStep 1) or interleaved (wave c takes rows c, c + C,
+// c + 2 C, ... for C waves, rowStep C), so that the waves resident at one time
+// read neighbouring rows and write neighbouring map lines. A wave steps its
+// row index r = y + Y z by rowStep = dy + Y dz.
+struct RowSched {
+  long long nRows, rowsPerWave, nChunks, chunkStep, rowStep;
+  int dy, dz;
+};
+
 template <typename T, int M, bool G, bool MOM>
 __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, T thr, int X, int Y, int nSegX,
-                                                           long long nRows, long long rowsPerWave,
-                                                           uint32_t* __restrict__ pW, uint32_t rowWS,
-                                                           uint32_t planeWS, double* __restrict__ part) {
+                                                           const RowSched rs, uint32_t* __restrict__ pW,
+                                                           uint32_t rowWS, uint32_t planeWS,
+                                                           double* __restrict__ part) {
   constexpr int V = kPbmVoxels;
   constexpr int E = 16 / sizeof(T);  // elements per 16-byte load
   constexpr int W = kWave * V;
@@ -76,9 +93,13 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   const int lane = threadIdx.x & (kWave - 1);
   const long long gw = (long long)blockIdx.x * kPbmWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int seg = (int)(gw % nSegX);
-  const long long r0 = (gw / nSegX) * rowsPerWave;
-  if (!MOM && r0 >= nRows) return;  // with moments every wave reaches the block reduction
-  const long long n = max(0ll, min(rowsPerWave, nRows - r0));
+  // Waves past the last chunk (the grid is whole workgroups) take no rows:
+  // interleaved, their first row would be another wave's.
+  const long long chunk = gw / nSegX;
+  const long long r0 = chunk * rs.chunkStep;
+  if (!MOM && chunk >= rs.nChunks) return;  // with moments every wave reaches the block reduction
+  const long long n =
+      chunk >= rs.nChunks ? 0ll : min(rs.rowsPerWave, (rs.nRows - r0 + rs.rowStep - 1) / rs.rowStep);
   int y = (int)(r0 % Y);
   long long z = r0 / Y;
   const int x0 = seg * W;
@@ -102,6 +123,22 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   const bool writer = pW != nullptr && jl < NQ && word < rowWS;
   const uint32_t sh = (uint32_t)(E * jl);
   MomAcc acc;
+  // Map staging (forms without moments): each writer lane keeps its word of
+  // up to kPbmStage rows in the wave's LDS slice and stores them back to
+  // back when the slice is full or the run ends, instead of one store per
+  // step. Only the lane that wrote a slot reads it back, so no barrier is
+  // needed. Gathering a wave's map writes in time makes them cheaper for HBM
+  // to interleave with the field's reads (the stores themselves cost
+  // nothing: folded into one L2-resident plane they leave the pass at the
+  // read roof).
+  constexpr bool STAGE = !MOM;
+  constexpr int kSt = STAGE ? kPbmStage : kPbmAhead;
+  static_assert(kSt % kPbmAhead == 0 && (!STAGE || kSt % 8 == 0), "staged rows must be whole steps");
+  __shared__ uint32_t stage[STAGE ? kPbmWaves * kSt * 32 : 1];
+  uint32_t* const myStage = stage + (STAGE ? (threadIdx.x >> 6) * kSt * 32 : 0);
+  const uint32_t widx = (word - (uint32_t)(x0 >> 5)) & 31u;  // the writer's word in the segment
+  int yb = y;  // first staged row
+  long long zb = z;
   // Partial segment (not the general form): load q of a row is read from
   // min(64 E q + E lane, X - x0 - E), so it ends inside the row (the host
   // sends rows shorter than E voxels to the general form), as an unsigned
@@ -160,8 +197,10 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
             for (int k = 0; k < V; ++k) v[u][k] = fi ? v[u][k] + t[k] : t[k];
           }
           wo[u] = (size_t)y * rowWS + (size_t)z * planeWS + word;
-          if (++y == Y) {
-            y = 0;
+          y += rs.dy;
+          z += rs.dz;
+          if (y >= Y) {
+            y -= Y;
             ++z;
           }
         }
@@ -205,7 +244,34 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
           uint32_t w32 = wq[0];
 #pragma unroll
           for (int q = 1; q < NQ; ++q) w32 = (jl == q) ? wq[q] : w32;
-          if (writer) pW[wo[u]] = w32;
+          if constexpr (STAGE) {
+            if (writer) myStage[(int)((i + u) % kSt) * 32 + widx] = w32;
+          } else if (writer) {
+            pW[wo[u]] = w32;
+          }
+        }
+      }
+      if constexpr (STAGE) {
+        if ((i + kPbmAhead) % kSt == 0 || i + kPbmAhead >= n) {
+          // Eight slots are read before their eight stores are issued.
+          const int nb = (int)(min(i + kPbmAhead, n) - (i / kSt) * kSt);
+          for (int k0 = 0; k0 < nb; k0 += 8) {
+            uint32_t t[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) t[j] = writer ? myStage[(k0 + j) * 32 + widx] : 0u;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              if (k0 + j < nb) {
+                if (writer) pW[(size_t)yb * rowWS + (size_t)zb * planeWS + word] = t[j];
+                yb += rs.dy;
+                zb += rs.dz;
+                if (yb >= Y) {
+                  yb -= Y;
+                  ++zb;
+                }
+              }
+            }
+          }
         }
       }
     }
@@ -1069,24 +1135,37 @@ Tiling chooseBpfdTiling(int X, int Y, int Z) {
 
 #endif
 
-#if MKF_PART(1)
-// About 16K waves in all (twice what 256 CUs hold at 32 waves each), each
-// streaming a run of rows of one 1024-voxel segment. (At 1024^3, 32K and 64K
-// waves, and 3 or 4 rows in flight instead of 2, were within 1 % for the map
-// and 2-6 % slower with moments.)
+#if MKF_PART(1) || MKF_PART(7)
+// About 16K waves in all, each streaming every C-th row of one 1024-voxel
+// segment, so the waves resident at one time read neighbouring rows and
+// write neighbouring map lines. (At 1024^3: runs of consecutive rows per
+// wave were 2-3 % slower; 4K, 8K and 32K waves, and 4 rows in flight instead
+// of 2, were within 1-4 %.)
 struct PbmGrid {
   int nSegX;
-  long long nRows, rowsPerWave, nBlocks;
+  RowSched rs;
+  long long nBlocks;
 };
 
 static PbmGrid pbmGrid(const FieldView& fv) {
   PbmGrid g;
   g.nSegX = (int)ceilDiv(fv.X, kWave * kPbmVoxels);
-  g.nRows = (long long)fv.Y * fv.Z;
+  RowSched& rs = g.rs;
+  rs.nRows = (long long)fv.Y * fv.Z;
   const long long wantChunks = std::max<long long>(1, (MKF_PBM_WAVE_TARGET + g.nSegX - 1) / g.nSegX);
-  g.rowsPerWave = ceilDiv(g.nRows, wantChunks);
-  const long long nWaves = g.nSegX * ceilDiv(g.nRows, g.rowsPerWave);
-  g.nBlocks = ceilDiv(nWaves, kPbmWaves);
+  rs.rowsPerWave = ceilDiv(rs.nRows, wantChunks);
+  const long long nChunks = ceilDiv(rs.nRows, rs.rowsPerWave);
+  rs.nChunks = nChunks;
+  if (MKF_PBM_INTERLEAVE) {
+    rs.chunkStep = 1;
+    rs.rowStep = nChunks;
+  } else {
+    rs.chunkStep = rs.rowsPerWave;
+    rs.rowStep = 1;
+  }
+  rs.dy = (int)(rs.rowStep % fv.Y);
+  rs.dz = (int)(rs.rowStep / fv.Y);
+  g.nBlocks = ceilDiv(g.nSegX * nChunks, kPbmWaves);
   return g;
 }
 
@@ -1097,10 +1176,10 @@ static hipError_t launchPBMT(const FieldView& fv, T thr, uint32_t* pW, uint32_t 
   // The general form also takes rows shorter than one 16-byte load.
   if (fv.nF > 1 || fv.sx != 1 || fv.X < (int)(16 / sizeof(T)))
     hipLaunchKernelGGL((kPBM<T, M, true, MOM>), dim3((unsigned)g.nBlocks), dim3(kPbmWaves * kWave), 0, s,
-                       fieldArgs<T>(fv), thr, fv.X, fv.Y, g.nSegX, g.nRows, g.rowsPerWave, pW, rowWS, planeWS, part);
+                       fieldArgs<T>(fv), thr, fv.X, fv.Y, g.nSegX, g.rs, pW, rowWS, planeWS, part);
   else
     hipLaunchKernelGGL((kPBM<T, M, false, MOM>), dim3((unsigned)g.nBlocks), dim3(kPbmWaves * kWave), 0, s,
-                       fieldArgs<T>(fv), thr, fv.X, fv.Y, g.nSegX, g.nRows, g.rowsPerWave, pW, rowWS, planeWS, part);
+                       fieldArgs<T>(fv), thr, fv.X, fv.Y, g.nSegX, g.rs, pW, rowWS, planeWS, part);
   return hipGetLastError();
 }
 
@@ -1119,13 +1198,27 @@ static hipError_t dispatchPBM(const FieldView& fv, T thr, uint32_t m, uint32_t* 
   }
 }
 
+// The double-field instantiations, compiled as a part of their own.
+hipError_t dispatchPBMf64(const FieldView& fv, double thr, uint32_t m, uint32_t* pW, uint32_t rowWS,
+                          uint32_t planeWS, double* part, bool mom, hipStream_t s);
+#endif
+
+#if MKF_PART(7)
+hipError_t dispatchPBMf64(const FieldView& fv, double thr, uint32_t m, uint32_t* pW, uint32_t rowWS,
+                          uint32_t planeWS, double* part, bool mom, hipStream_t s) {
+  return mom ? dispatchPBM<double, true>(fv, thr, m, pW, rowWS, planeWS, part, s)
+             : dispatchPBM<double, false>(fv, thr, m, pW, rowWS, planeWS, part, s);
+}
+#endif
+
+#if MKF_PART(1)
 hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint32_t rowWS, uint32_t planeWS,
                      hipStream_t s) {
   if (fv.X < 1 || fv.Y < 1 || fv.Z < 1) return hipErrorInvalidValue;
   // f32 fields compare against the threshold rounded to float
   // (src/binMapCUDA.cuh:138-139).
   if (fv.enc == ENC_F32) return dispatchPBM<float, false>(fv, (float)m.t[0], m.m, pW, rowWS, planeWS, nullptr, s);
-  if (fv.enc == ENC_F64) return dispatchPBM<double, false>(fv, m.t[0], m.m, pW, rowWS, planeWS, nullptr, s);
+  if (fv.enc == ENC_F64) return dispatchPBMf64(fv, m.t[0], m.m, pW, rowWS, planeWS, nullptr, false, s);
   return hipErrorInvalidValue;
 }
 
@@ -1135,12 +1228,12 @@ hipError_t launchPBMMoments(const FieldView& fv, const BinMapF64& m, uint32_t* p
                             uint32_t planeWS, double* part, double* mom, hipStream_t s) {
   if (fv.X < 1 || fv.Y < 1 || fv.Z < 1) return hipErrorInvalidValue;
   // Per-lane counts are 32-bit: at most 16 voxels per row.
-  if (pbmGrid(fv).rowsPerWave > (1ll << 27)) return hipErrorInvalidValue;
+  if (pbmGrid(fv).rs.rowsPerWave > (1ll << 27)) return hipErrorInvalidValue;
   hipError_t e;
   if (fv.enc == ENC_F32)
     e = dispatchPBM<float, true>(fv, (float)m.t[0], m.m, pW, rowWS, planeWS, part, s);
   else if (fv.enc == ENC_F64)
-    e = dispatchPBM<double, true>(fv, m.t[0], m.m, pW, rowWS, planeWS, part, s);
+    e = dispatchPBMf64(fv, m.t[0], m.m, pW, rowWS, planeWS, part, true, s);
   else
     return hipErrorInvalidValue;
   if (e != hipSuccess) return e;
