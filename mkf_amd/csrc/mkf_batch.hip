@@ -16,10 +16,11 @@
 // it from memory.
 //
 // Per level and row the wave forms the row's relation mask with one or two
-// ballots, clears invalid lanes in scalar registers, and packs the 2x2 pair
-// nibbles of its 4 row pairs into the 4 bytes of one word H. The cell codes
-// of plane pair (z-1, z) are the bytes of Hprev | H << 4; all-one cells are
-// counted from ballots, mixed cells go to the workgroup's LDS histogram, and
+// ballots and packs the 2x2 pair nibbles of its 4 row pairs into the 4 bytes
+// of one word H. The cell codes of plane pair (z-1, z) are the bytes of
+// Hprev | H << 4, with the bytes of cells that do not exist cleared; all-one
+// cells are counted per lane by a byte-wise test, mixed cells go to the
+// workgroup's LDS histogram (one SDWA compare and one atomic per byte), and
 // all-zero cells are what remains of the cell count.
 #include <stdio.h>
 #include <stdlib.h>
@@ -58,10 +59,13 @@ struct BatchItem {
 };
 
 // Shared state of a batch workgroup. hpk holds, per wave, level and lane,
-// the previous plane's H word (four row-pair nibbles, one per byte).
+// the previous plane's H word (four row-pair nibbles, one per byte); ones,
+// per level and lane, the all-one cells counted so far (every wave adds its
+// lane l's count to slot l).
 struct BatchShared {
   uint32_t hist[kMaxLevels][256];
   uint32_t hpk[kBatchWaves][kMaxLevels][kWave];
+  uint32_t ones[kMaxLevels][kWave];
   double xv[2][kBatchTY + 1][kBatchWaves];  // voxel at lane 0 of each row, per wave
   uint32_t msum[kMaxLevels];
   uint32_t cells;
@@ -72,7 +76,15 @@ struct BatchShared {
 // Bin 0 of a level is the workgroup's cells minus its all-one and mixed cells.
 __device__ void batchFlush(BatchShared& sh, const BatchField& fd) {
   const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x >> 6;
   const int nL = fd.nLevels;
+  __syncthreads();
+  // Wave w folds the all-one counts of levels w, w + 8, ... into bin 255.
+  for (int l = w; l < nL; l += kBatchWaves) {
+    const uint32_t n1 = waveSum(sh.ones[l][lane]);
+    sh.ones[l][lane] = 0;
+    if (lane == 0) sh.hist[l][255] += n1;
+  }
   __syncthreads();
   for (int l = 0; l < nL; ++l) {
     uint32_t part = 0;
@@ -125,15 +137,17 @@ __device__ __forceinline__ uint32_t pairBits(T f, T fr, T t) {
 
 // One level of one z-step: the pair nibbles of the wave's 5 rows at level
 // relation M, then (from the second plane on) the tally of its row pairs'
-// cells into histL. rgt[r] is the voxel right of each lane's voxel (lane 63:
-// the halo voxel). vr[r] holds the lanes whose cell in row pair r exists
-// (none for row pairs beyond the field), so nothing else is masked. A cell
-// is mixed when its code is neither 0 nor 255: two compares, the rest in
-// scalar registers.
+// cells: byte r of C is the code of the lane's cell in row pair r, cleared
+// by kb where that cell does not exist (so it counts as neither all-one nor
+// mixed). rgt[r] is the voxel right of each lane's voxel (lane 63: the halo
+// voxel). All-one cells are counted per lane by a byte-wise test (added to
+// the lane's slot of onesL), mixed cells go to histL with one SDWA compare
+// and one LDS atomic per byte, and all-zero cells are what remains of the
+// cell count.
 template <int M, typename T>
 __device__ __forceinline__ uint32_t levelStep(const T (&cur)[kBatchTY + 1], const T (&rgt)[kBatchTY + 1], T thr,
-                                              const uint64_t (&vr)[kBatchTY], bool count, uint32_t hp,
-                                              uint32_t* histL) {
+                                              uint32_t kb, bool count, uint32_t hp, uint32_t* histL,
+                                              uint32_t* onesL) {
   uint32_t P[kBatchTY + 1];
 #pragma unroll
   for (int r = 0; r <= kBatchTY; ++r) P[r] = pairBits<M>(cur[r], rgt[r], thr);
@@ -141,34 +155,32 @@ __device__ __forceinline__ uint32_t levelStep(const T (&cur)[kBatchTY + 1], cons
   const uint32_t hi = (lo >> 8) | (P[4] << 24);
   const uint32_t H = lo | (hi << 2);
   if (count) {
-    const uint32_t C = hp | (H << 4);
-    uint32_t ones = 0;
-#pragma unroll
-    for (int r = 0; r < kBatchTY; ++r) {
-      const uint32_t code = (C >> (8 * r)) & 255u;
-      const uint64_t all1 = ballot(code == 255u) & vr[r];
-      const uint64_t mixed = vr[r] & ~(all1 | ballot(code == 0u));
-      ones += (uint32_t)__builtin_popcountll(all1);
-      if (laneBit(mixed)) __hip_atomic_fetch_add(histL + code, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    if ((threadIdx.x & (kWave - 1)) == 0 && ones) atomicAdd(histL + 255, ones);
+    const uint32_t C = (hp | (H << 4)) & kb;
+    __hip_atomic_fetch_add(onesL, (uint32_t)__builtin_popcount(allOneBytes(C)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    const uint32_t D = bytesMinusOne(C);
+    Tally t;
+    t.addByte<0>(C, D, histL);
+    t.addByte<1>(C, D, histL);
+    t.addByte<2>(C, D, histL);
+    t.addByte<3>(C, D, histL);
   }
   return H;
 }
 
 template <typename T>
 __device__ __forceinline__ uint32_t levelDispatch(uint32_t m, const T (&cur)[kBatchTY + 1],
-                                                  const T (&rgt)[kBatchTY + 1], T thr, const uint64_t (&vr)[kBatchTY],
-                                                  bool count, uint32_t hp, uint32_t* histL) {
+                                                  const T (&rgt)[kBatchTY + 1], T thr, uint32_t kb, bool count,
+                                                  uint32_t hp, uint32_t* histL, uint32_t* onesL) {
   switch (m) {
-    case 1: return levelStep<1>(cur, rgt, thr, vr, count, hp, histL);
-    case 2: return levelStep<2>(cur, rgt, thr, vr, count, hp, histL);
-    case 3: return levelStep<3>(cur, rgt, thr, vr, count, hp, histL);
-    case 4: return levelStep<4>(cur, rgt, thr, vr, count, hp, histL);
-    case 5: return levelStep<5>(cur, rgt, thr, vr, count, hp, histL);
-    case 6: return levelStep<6>(cur, rgt, thr, vr, count, hp, histL);
-    case 7: return levelStep<7>(cur, rgt, thr, vr, count, hp, histL);
-    default: return levelStep<0>(cur, rgt, thr, vr, count, hp, histL);
+    case 1: return levelStep<1>(cur, rgt, thr, kb, count, hp, histL, onesL);
+    case 2: return levelStep<2>(cur, rgt, thr, kb, count, hp, histL, onesL);
+    case 3: return levelStep<3>(cur, rgt, thr, kb, count, hp, histL, onesL);
+    case 4: return levelStep<4>(cur, rgt, thr, kb, count, hp, histL, onesL);
+    case 5: return levelStep<5>(cur, rgt, thr, kb, count, hp, histL, onesL);
+    case 6: return levelStep<6>(cur, rgt, thr, kb, count, hp, histL, onesL);
+    case 7: return levelStep<7>(cur, rgt, thr, kb, count, hp, histL, onesL);
+    default: return levelStep<0>(cur, rgt, thr, kb, count, hp, histL, onesL);
   }
 }
 
@@ -199,9 +211,8 @@ __device__ void batchItem(BatchShared& sh, const BatchField& fd, const BatchItem
   const int hxo = min(x0 + kWave, X - 1) - xb;
   const uint64_t valid = ballot(x0 + lane < X - 1);
   if (lane == 0 && nY > 0) atomicAdd(&sh.cells, (uint32_t)__builtin_popcountll(valid) * (uint32_t)(nY * nZ));
-  uint64_t vr[kBatchTY];
-#pragma unroll
-  for (int r = 0; r < kBatchTY; ++r) vr[r] = r < nY ? valid : 0ull;
+  // Bytes of the lane's cells that exist: row pairs 0 .. nY-1, valid lanes.
+  const uint32_t kb = laneBit(valid) ? (nY >= 4 ? 0xFFFFFFFFu : (1u << (8 * nY)) - 1u) : 0u;
 
   // Lane l holds level l's threshold and relation; the level loop reads them
   // with readlane.
@@ -252,7 +263,7 @@ __device__ void batchItem(BatchShared& sh, const BatchField& fd, const BatchItem
       if (l + 1 < nL) hpNext = hpk[(l + 1) * kWave];
       const T thr = readLane(tl, l);
       const uint32_t m = readLane(ml, l);
-      hpk[l * kWave] = levelDispatch<T>(m, cur, rgt, thr, vr, count, hp, sh.hist[l]);
+      hpk[l * kWave] = levelDispatch<T>(m, cur, rgt, thr, kb, count, hp, sh.hist[l], &sh.ones[l][lane]);
     }
 #pragma unroll
     for (int r = 0; r <= kBatchTY; ++r) {
@@ -283,8 +294,10 @@ __global__ __launch_bounds__(kBatchWaves* kWave) void kBatch(const BatchField* _
                                                               const BatchItem* __restrict__ items, int nItems,
                                                               unsigned int* __restrict__ counter) {
   __shared__ BatchShared sh;
-  for (int l = 0; l < kMaxLevels; ++l)
+  for (int l = 0; l < kMaxLevels; ++l) {
     for (int i = threadIdx.x; i < 256; i += blockDim.x) sh.hist[l][i] = 0;
+    if (threadIdx.x < kWave) sh.ones[l][threadIdx.x] = 0;
+  }
   if (threadIdx.x < kMaxLevels) sh.msum[threadIdx.x] = 0;
   if (threadIdx.x == 0) sh.cells = 0;
   int curField = -1;
