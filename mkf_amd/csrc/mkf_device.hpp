@@ -162,6 +162,19 @@ __device__ __forceinline__ T globalLoad(const T* p, long long i) {
   typedef const __attribute__((address_space(1))) T* GlobalPtr;
   return ((GlobalPtr)p)[i];
 }
+// Load of a 32-bit word at a per-lane byte offset from a wave-uniform base,
+// as a raw buffer load: the base goes into a buffer descriptor in scalar
+// registers (readfirstlane makes its uniformity explicit) and the offset is
+// a 32-bit vector operand, so no 64-bit address is formed per load. Offsets
+// must be below 2^31; the descriptor's range check returns 0 beyond that.
+__device__ __forceinline__ uint32_t bufferLoadWord(const uint32_t* base, uint32_t byteOff) {
+  const uint64_t a = reinterpret_cast<uint64_t>(base);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc((void*)((((uint64_t)hi) << 32) | lo), 0, 0x7FFFFFFF, 0x00020000);
+  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)byteOff, 0, 0);
+}
 
 // Byte-packed pair words. A lane holds V consecutive voxels (V = 1, 2 or 4);
 // byte k of its pair word P holds (bit of voxel k, bit of voxel k+1) in bits
@@ -399,9 +412,9 @@ struct PbmRows {
 // byte k holds bits k and k+1 of v at its bits 0 and 1.
 struct PbmRows4 {
   static constexpr bool kSelfHalo = true;
-  const uint32_t* row;  // PBM (word offsets below are from the row start)
+  const uint32_t* row;  // PBM (offsets below are from the row start)
   long long sy, sz;     // word strides of y and z
-  uint32_t w0, w1;      // the lane's word and the next, clamped into the row
+  uint32_t b0, b1;      // byte offsets of the lane's word and the next, clamped into the row
   uint32_t sh;          // 4 * (lane & 7)
   uint32_t keep;        // pair-word bytes of cells that exist (per lane)
 
@@ -411,7 +424,8 @@ struct PbmRows4 {
 
   template <bool NT = false>
   __device__ __forceinline__ Raw load(long long off) const {
-    return Raw{globalLoad(row, off + w0), globalLoad(row, off + w1)};
+    const uint32_t* rp = row + off;  // wave-uniform
+    return Raw{bufferLoadWord(rp, b0), bufferLoadWord(rp, b1)};
   }
   __device__ __forceinline__ uint32_t pairs(const Raw& r, uint64_t& R0) const {
     R0 = 0;
@@ -642,13 +656,10 @@ __device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1
   long long yo[TY + 1];
 #pragma unroll
   for (int r = 0; r <= TY; ++r) yo[r] = (long long)(y0 + min(r, nY)) * src.sy;
-  typename Src::Raw cur[TY + 1], nxt[TY + 1];
-  {
-    const long long zo = (long long)z0 * src.sz;
-    loadRows<TY>(src, cur, yo, zo, std::make_integer_sequence<int, TY + 1>{});
-  }
-  uint32_t Hp[TY];
-  for (int z = z0; z <= z0 + nZ; ++z) {
+  // One z-step: loads plane z+1 into nxt, forms the pair words of plane z
+  // (in cur) into Hc, and tallies the cells between Hp (plane z-1) and Hc.
+  auto step = [&](int z, typename Src::Raw(&cur)[TY + 1], typename Src::Raw(&nxt)[TY + 1], const uint32_t(&Hp)[TY],
+                  uint32_t(&Hc)[TY]) {
     const long long zon = (long long)min(z + 1, z0 + nZ) * src.sz;
     loadRows<TY>(src, nxt, yo, zon, std::make_integer_sequence<int, TY + 1>{});
     const int par = z & 1;
@@ -692,7 +703,6 @@ __device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1
         }
       }
     }
-    uint32_t Hc[TY];
 #pragma unroll
     for (int r = 0; r < TY; ++r) Hc[r] = P[r] | (P[r + 1] << 2);
     if (z > z0) {
@@ -704,10 +714,28 @@ __device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1
         }
       }
     }
+  };
+  typename Src::Raw ra[TY + 1], rb[TY + 1];
+  uint32_t ha[TY], hb[TY];
+  loadRows<TY>(src, ra, yo, (long long)z0 * src.sz, std::make_integer_sequence<int, TY + 1>{});
+  if constexpr (Src::kSelfHalo) {
+    // Unrolled by two: the next plane's rows and the previous plane's words
+    // alternate between two register sets instead of being copied each step
+    // (the map histogram, which is issue-bound, saves those moves).
+    for (int z = z0; z <= z0 + nZ;) {
+      step(z, ra, rb, hb, ha);
+      if (++z > z0 + nZ) break;
+      step(z, rb, ra, ha, hb);
+      if (++z > z0 + nZ) break;
+    }
+  } else {
+    for (int z = z0; z <= z0 + nZ; ++z) {
+      step(z, ra, rb, hb, ha);
 #pragma unroll
-    for (int r = 0; r < TY; ++r) Hp[r] = Hc[r];
+      for (int r = 0; r < TY; ++r) hb[r] = ha[r];
 #pragma unroll
-    for (int r = 0; r <= TY; ++r) cur[r] = nxt[r];
+      for (int r = 0; r <= TY; ++r) ra[r] = rb[r];
+    }
   }
 }
 

@@ -499,8 +499,8 @@ __global__ __launch_bounds__(1024) void kBPFD4(const uint32_t* __restrict__ pW, 
   src.sy = rowWS;
   src.sz = planeWS;
   const uint32_t wl = (uint32_t)(q.x0 >> 5) + (uint32_t)(lane >> 3);
-  src.w0 = min(wl, rowWS - 1u);
-  src.w1 = min(wl + 1u, rowWS - 1u);
+  src.b0 = 4u * min(wl, rowWS - 1u);
+  src.b1 = 4u * min(wl + 1u, rowWS - 1u);
   src.sh = 4u * (uint32_t)(lane & 7);
   uint32_t cellsPerRow;
   src.keep = cellKeep<4>(q.x0, X, cellsPerRow);
@@ -526,7 +526,10 @@ __global__ __launch_bounds__(1024) void kBPFD4(const uint32_t* __restrict__ pW, 
 // The fused 16-voxel kernel runs at most 2 waves per SIMD (8 per CU): more
 // requests in flight lowered its throughput (1024^3: 3 waves per SIMD
 // 0.66-0.73 ms against 0.65 at 2).
-constexpr int kFused16PerCU = 8;
+#ifndef MKF_FUSED16_PER_CU
+#define MKF_FUSED16_PER_CU 8
+#endif
+constexpr int kFused16PerCU = MKF_FUSED16_PER_CU;
 
 template <int M, int TY, int NG>
 __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, float thr, int X, int Y, int Z,
