@@ -546,6 +546,32 @@ __device__ __forceinline__ uint32_t byteTimes4(uint32_t C) {
   return a;
 }
 
+// The lanes whose byte K of M is nonzero: one compare with an SDWA byte
+// select.
+template <int K>
+__device__ __forceinline__ uint64_t nonZeroByteLanes(uint32_t M) {
+  uint64_t s;
+  const uint32_t zero = 0u;
+  if constexpr (K == 0) asm("v_cmp_lt_u32_sdwa %0, %1, %2 src0_sel:DWORD src1_sel:BYTE_0" : "=s"(s) : "v"(zero), "v"(M));
+  if constexpr (K == 1) asm("v_cmp_lt_u32_sdwa %0, %1, %2 src0_sel:DWORD src1_sel:BYTE_1" : "=s"(s) : "v"(zero), "v"(M));
+  if constexpr (K == 2) asm("v_cmp_lt_u32_sdwa %0, %1, %2 src0_sel:DWORD src1_sel:BYTE_2" : "=s"(s) : "v"(zero), "v"(M));
+  if constexpr (K == 3) asm("v_cmp_lt_u32_sdwa %0, %1, %2 src0_sel:DWORD src1_sel:BYTE_3" : "=s"(s) : "v"(zero), "v"(M));
+  return s;
+}
+
+// Per-plane tests of a row-pair word Hc (byte k: the 4-bit half code of
+// cell k in one plane, bits 4..7 zero), made once per plane and used for
+// both plane pairs the plane belongs to:
+//   full  = Hc + 0x01010101: bit 4 of byte k set iff nibble k is 15
+//   some  = (Hc + 0x0F0F0F0F) & 0x10101010: bit 4 set iff nibble k is not 0
+// No byte carries into the next (15 + 15 < 256).
+struct HalfCodes {
+  uint32_t h, full, some;
+};
+__device__ __forceinline__ HalfCodes halfCodes(uint32_t Hc) {
+  return HalfCodes{Hc, Hc + 0x01010101u, (Hc + 0x0F0F0F0Fu) & 0x10101010u};
+}
+
 // Tally of cell codes into the workgroup's LDS histogram. Cells that do not
 // exist have code 0 (their pair bits are cleared), so no per-cell validity
 // test is needed. All-one cells are counted per lane, four at a time, by a
@@ -557,6 +583,29 @@ struct Tally {
   template <int K>
   __device__ __forceinline__ void addByte(uint32_t C, uint32_t D, uint32_t* hist) {
     const uint64_t mix = mixedLanes<K>(D);
+    const uint32_t a = byteTimes4<K>(C);
+    if (laneBit(mix))
+      __hip_atomic_fetch_add((uint32_t*)((char*)hist + a), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  // The V cells between planes a (below) and b (above), from their half
+  // codes: cell k is all-one iff both nibbles are 15, all-zero iff both are
+  // 0, and mixed otherwise; its code is a.h | b.h << 4.
+  template <int V>
+  __device__ __forceinline__ void addPlanes(const HalfCodes& a, const HalfCodes& b, uint32_t* hist) {
+    const uint32_t ones4 = a.full & b.full & 0x10101010u;
+    ones += (uint32_t)__builtin_popcount(ones4);
+    const uint32_t mixed = (a.some | b.some) & ~ones4;
+    const uint32_t C = a.h | (b.h << 4);
+    addMixed<0>(C, mixed, hist);
+    if constexpr (V > 1) addMixed<1>(C, mixed, hist);
+    if constexpr (V > 2) {
+      addMixed<2>(C, mixed, hist);
+      addMixed<3>(C, mixed, hist);
+    }
+  }
+  template <int K>
+  __device__ __forceinline__ void addMixed(uint32_t C, uint32_t mixed, uint32_t* hist) {
+    const uint64_t mix = nonZeroByteLanes<K>(mixed);
     const uint32_t a = byteTimes4<K>(C);
     if (laneBit(mix))
       __hip_atomic_fetch_add((uint32_t*)((char*)hist + a), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -658,8 +707,8 @@ __device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1
   for (int r = 0; r <= TY; ++r) yo[r] = (long long)(y0 + min(r, nY)) * src.sy;
   // One z-step: loads plane z+1 into nxt, forms the pair words of plane z
   // (in cur) into Hc, and tallies the cells between Hp (plane z-1) and Hc.
-  auto step = [&](int z, typename Src::Raw(&cur)[TY + 1], typename Src::Raw(&nxt)[TY + 1], const uint32_t(&Hp)[TY],
-                  uint32_t(&Hc)[TY]) {
+  auto step = [&](int z, typename Src::Raw(&cur)[TY + 1], typename Src::Raw(&nxt)[TY + 1],
+                  const HalfCodes(&Hp)[TY], HalfCodes(&Hc)[TY]) {
     const long long zon = (long long)min(z + 1, z0 + nZ) * src.sz;
     loadRows<TY>(src, nxt, yo, zon, std::make_integer_sequence<int, TY + 1>{});
     const int par = z & 1;
@@ -704,19 +753,16 @@ __device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1
       }
     }
 #pragma unroll
-    for (int r = 0; r < TY; ++r) Hc[r] = P[r] | (P[r + 1] << 2);
+    for (int r = 0; r < TY; ++r) Hc[r] = halfCodes(P[r] | (P[r + 1] << 2));
     if (z > z0) {
 #pragma unroll
       for (int r = 0; r < TY; ++r) {
-        if (r < nY) {
-          const uint32_t C = Hp[r] | (Hc[r] << 4);
-          tally.template add<V>(C, hist);
-        }
+        if (r < nY) tally.template addPlanes<V>(Hp[r], Hc[r], hist);
       }
     }
   };
   typename Src::Raw ra[TY + 1], rb[TY + 1];
-  uint32_t ha[TY], hb[TY];
+  HalfCodes ha[TY], hb[TY];
   loadRows<TY>(src, ra, yo, (long long)z0 * src.sz, std::make_integer_sequence<int, TY + 1>{});
   if constexpr (Src::kSelfHalo) {
     // Unrolled by two: the next plane's rows and the previous plane's words
