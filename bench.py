@@ -556,7 +556,7 @@ def cpu_baseline(f_np, bm, threads):
     return {
         "value": round(alln, 4), "unit": "Gvox/s", "cores": threads, "kind": "port",
         "sample": f"all-cores mode: the whole {X}x{Y}x{Z} field (same bytes as the GPU run), best of 3 passes "
-                  f"({', '.join(f'{t:.2f}' for t in tNs)} s); C restatement of mkfAccGetBPFDSimple (gcc -O3, "
+                  f"({', '.join(f'{t:.2f}' for t in tNs)} s); C restatement of mkfAccGetBPFDSimple (gcc -O3 -march=x86-64-v3, "
                   f"pthreads over plane pairs, private histograms)",
         "modes": {
             "1_thread": {"value": round(one, 4), "unit": "Gvox/s", "threads": 1,
@@ -734,6 +734,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(S, args.dtype, tiling),
                      "kernel_ms_mean": round(kern_avg, 4), "kernel_ms_min": round(min(kern_ms), 4),
+                     "kernel_ms_median": round(float(np.median(kern_ms)), 4),
                      "bytes_per_launch": bytes_per_launch},
         "cpu_baseline": None,
     }
@@ -852,6 +853,7 @@ def main():
                          "frac": round(a64 / HBM_PEAK_GBS, 4), "traffic": traffic_of(S, "f64", t64),
                          "measured_roof": roof64, "frac_of_measured_roof": round(a64 / roof64["GBps"], 4),
                          "kernel_ms_mean": round(k64, 4), "kernel_ms_min": round(min(leg64["kern_ms"]), 4),
+                         "kernel_ms_median": round(float(np.median(leg64["kern_ms"])), 4),
                          "bytes_per_launch": nvox * 8},
         }
         if not args.no_cpu:

@@ -84,7 +84,9 @@ def build_library(verbose=False, jobs=None):
 
 def build_oracle(verbose=False):
     tmp = ORACLE_LIB + ".tmp"
-    _run(["gcc", "-O3", "-march=x86-64-v2", "-std=gnu11", "-fPIC", "-shared", "-Wall", "-pthread",
+    # x86-64-v3 (AVX2, BMI2, FMA) rather than -march=native: the library is
+    # built here and run on the GPU machine's host, whose CPU may differ.
+    _run(["gcc", "-O3", "-march=x86-64-v3", "-std=gnu11", "-fPIC", "-shared", "-Wall", "-pthread",
           ORACLE_SRC, "-o", tmp, "-lm"])
     os.replace(tmp, ORACLE_LIB)
     if verbose:
