@@ -41,16 +41,26 @@ constexpr int kMaxLevels = MKF_BATCH_MAX_LEVELS;
 // 32-bit LDS counters, flushed once they pass 2^31, cannot wrap.
 constexpr int kMaxPlanePairs = 4096;
 
-// Device descriptor of one field of a batch.
+// Level families of the Gray-plane form (batchItemGray): the bit of a level
+// is !(f < t) (kFamGE: relations ">=", and "<" as its complement) or f > t
+// (kFamGT: ">", and "<=" as its complement). NaN takes the EQ bit: set for
+// ">=" and "<=", clear for ">" and "<", so the complements hold for it too.
+constexpr int kFamGE = 1, kFamGT = 2;
+
+// Device descriptor of one field of a batch (or of one part of its levels).
 struct BatchField {
   const void* p;
-  unsigned long long* out;  // nLevels x 256 u64, added to
+  unsigned long long* out;  // outLevels x 256 u64, added to
   long long sy, sz;
   int enc, X, Y, Z;
-  int nLevels;
+  int nLevels;           // levels tallied by this descriptor
+  int outLevels;         // histograms at out (every part of a field shares them)
   int segs, stacks, TZ;  // segments per x-tile, row stacks per workgroup, plane pairs per slab
   int nXT, nYB, nZR;     // x-tiles, y-blocks, slabs
-  double t[kMaxLevels];  // f32 fields: the threshold rounded to float
+  int gray, fam;         // Gray-plane form: 4 or 8 padded levels and the family; 0: per-level form
+  uint32_t comp;         // bit l: level l's histogram is stored complemented (bin c -> 255 - c)
+  uint8_t perm[kMaxLevels];  // output histogram of level l
+  double t[kMaxLevels];  // f32 fields: the threshold rounded to float (Gray form: sorted ascending)
   uint32_t m[kMaxLevels];
 };
 
@@ -95,9 +105,11 @@ __device__ void batchFlush(BatchShared& sh, const BatchField& fd) {
   }
   __syncthreads();
   for (int l = 0; l < nL; ++l) {
+    unsigned long long* o = fd.out + (size_t)fd.perm[l] * 256;
+    const int flip = ((fd.comp >> l) & 1u) ? 255 : 0;
     for (int i = threadIdx.x; i < 256; i += blockDim.x) {
       const uint32_t v = (i == 0) ? sh.cells - sh.msum[l] - sh.hist[l][255] : sh.hist[l][i];
-      if (v) atomicAdd(fd.out + (size_t)l * 256 + i, (unsigned long long)v);
+      if (v) atomicAdd(o + (i ^ flip), (unsigned long long)v);
     }
   }
   __syncthreads();
@@ -273,6 +285,164 @@ __device__ void batchItem(BatchShared& sh, const BatchField& fd, const BatchItem
   }
 }
 
+// ---------------------------------------------------------------------------
+// Gray-plane form. With one family's levels sorted by threshold, a voxel's
+// bits over levels 0..NL-1 form a thermometer: bit L is set iff L < r, r the
+// number of set bits (0..NL). The wave ballots the NL level masks A_L of a
+// row and keeps only the Gray code of r, in NP = log2(NL) + 1 plane masks:
+//   NL = 8: g0 = A0^A2^A4^A6, g1 = A1^A5, g2 = A3, g3 = A7
+//   NL = 4: g0 = A0^A2,       g1 = A1,    g2 = A3
+// (scalar XORs of the ballots: A_L & ~A_(L+1) are disjoint since the A_L
+// are nested). Pair bits, row packing and the previous plane's word are
+// formed once per plane instead of once per level; the code word of level L
+// (byte k: cell code of row pair k) is then a bitwise function of the plane
+// words, applied to all 4 row pairs and all 8 corners at once (grayLevel).
+// A segment covers 63 cells: lane 63 holds the voxel right of lane 62, so
+// every right neighbour is bit l + 1 of the same ballot and the waves of a
+// workgroup exchange nothing.
+// ---------------------------------------------------------------------------
+template <int NL>
+constexpr int kGrayPlanes = NL == 8 ? 4 : 3;
+
+template <int FAM, typename T>
+__device__ __forceinline__ uint64_t famMask(T f, T t) {
+  if constexpr (FAM == kFamGE)
+    return ~ballot(f < t);
+  else
+    return ballot(f > t);
+}
+
+// Code word of level L (bit i of byte k set iff corner i of cell k has
+// r > L) from the plane words c[j] (bit i of byte k: Gray bit j of corner
+// i's r). Every form maps all-zero planes (cells that do not exist) to 0.
+template <int NL, int L>
+__device__ __forceinline__ uint32_t grayLevel(const uint32_t (&c)[kGrayPlanes<NL>]) {
+  if constexpr (NL == 8) {
+    if constexpr (L == 0) return c[0] | c[1] | c[2];
+    if constexpr (L == 1) return c[1] | c[2];
+    if constexpr (L == 2) return c[2] | (c[1] & ~c[0]);
+    if constexpr (L == 3) return c[2];
+    if constexpr (L == 4) return c[2] & ~(c[1] & ~c[0]);
+    if constexpr (L == 5) return c[2] & ~c[1];
+    if constexpr (L == 6) return c[2] & ~(c[1] | c[0]);
+    return c[3];
+  } else {
+    if constexpr (L == 0) return c[0] | c[1];
+    if constexpr (L == 1) return c[1];
+    if constexpr (L == 2) return c[1] & ~c[0];
+    return c[2];
+  }
+}
+
+// Tally of one level's code word: all-one cells counted per lane, mixed
+// cells to the level's LDS histogram.
+__device__ __forceinline__ void grayTally(uint32_t W, uint32_t* hist, uint32_t& ones) {
+  ones += (uint32_t)__builtin_popcount(allOneBytes(W));
+  const uint32_t D = bytesMinusOne(W);
+  Tally t;
+  t.addByte<0>(W, D, hist);
+  t.addByte<1>(W, D, hist);
+  t.addByte<2>(W, D, hist);
+  t.addByte<3>(W, D, hist);
+}
+
+template <int NL, int L>
+__device__ __forceinline__ void grayLevels(const uint32_t (&c)[kGrayPlanes<NL>], int nL, BatchShared& sh,
+                                           uint32_t (&ones)[NL]) {
+  if constexpr (L < NL) {
+    if (L < nL) {
+      grayTally(grayLevel<NL, L>(c), sh.hist[L], ones[L]);
+      grayLevels<NL, L + 1>(c, nL, sh, ones);
+    }
+  }
+}
+
+template <typename T, int FAM, int NL>
+__device__ void batchItemGray(BatchShared& sh, const BatchField& fd, const BatchItem it) {
+  constexpr int NP = kGrayPlanes<NL>;
+  constexpr int kCells = kWave - 1;  // cells per segment
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int X = fd.X, Y = fd.Y;
+  const int nL = fd.nLevels;
+  const int segs = fd.segs;
+  const int seg = w % segs;
+  const int stack = w / segs;
+  const bool active = stack < fd.stacks;
+  const int x0 = (it.xt * segs + seg) * kCells;
+  const int y0 = (it.yb * fd.stacks + (active ? stack : 0)) * kBatchTY;
+  const int nY = active ? max(0, min(kBatchTY, Y - 1 - y0)) : 0;  // 0: the stack counts nothing
+  const int yc = min(y0, Y - 1);
+  const int z0 = it.zr * fd.TZ;
+  const int nZ = min(fd.TZ, fd.Z - 1 - z0);
+
+  const int xb = min(x0, X - 1);
+  const T* fw = (const T*)fd.p + xb;
+  const long long sy = fd.sy, sz = fd.sz;
+  const uint32_t lx = (uint32_t)(min(x0 + lane, X - 1) - xb);
+  const uint64_t valid = ballot(lane < kCells && x0 + lane < X - 1);
+  if (lane == 0 && nY > 0) atomicAdd(&sh.cells, (uint32_t)__builtin_popcountll(valid) * (uint32_t)(nY * nZ));
+  const uint32_t kb = laneBit(valid) ? (nY >= 4 ? 0xFFFFFFFFu : (1u << (8 * nY)) - 1u) : 0u;
+
+  T thr[NL];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) thr[l] = (T)fd.t[l];
+
+  long long yo[kBatchTY + 1];
+#pragma unroll
+  for (int r = 0; r <= kBatchTY; ++r) yo[r] = (long long)(yc + min(r, nY)) * sy;
+  T cur[kBatchTY + 1], nxt[kBatchTY + 1];
+#pragma unroll
+  for (int r = 0; r <= kBatchTY; ++r) cur[r] = globalLoad(fw, yo[r] + (long long)z0 * sz + lx);
+  uint32_t hp[NP] = {};
+  uint32_t ones[NL];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) ones[l] = 0;
+  for (int z = z0; z <= z0 + nZ; ++z) {
+    const long long zon = (long long)min(z + 1, z0 + nZ) * sz;
+#pragma unroll
+    for (int r = 0; r <= kBatchTY; ++r) nxt[r] = globalLoad(fw, yo[r] + zon + lx);
+    // Pair bits of every plane and row: bit 0 the lane's voxel, bit 1 its
+    // right neighbour (lane l + 1).
+    uint32_t P[NP][kBatchTY + 1];
+#pragma unroll
+    for (int r = 0; r <= kBatchTY; ++r) {
+      uint64_t A[NL];
+#pragma unroll
+      for (int l = 0; l < NL; ++l) A[l] = famMask<FAM>(cur[r], thr[l]);
+      uint64_t g[NP];
+      if constexpr (NL == 8) {
+        g[0] = A[0] ^ A[2] ^ A[4] ^ A[6];
+        g[1] = A[1] ^ A[5];
+        g[2] = A[3];
+        g[3] = A[7];
+      } else {
+        g[0] = A[0] ^ A[2];
+        g[1] = A[1];
+        g[2] = A[3];
+      }
+#pragma unroll
+      for (int j = 0; j < NP; ++j) P[j][r] = addLaneBit(laneBit(g[j] >> 1) ? 2u : 0u, g[j]);
+    }
+    uint32_t c[NP];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const uint32_t lo = P[j][0] | (P[j][1] << 8) | (P[j][2] << 16) | (P[j][3] << 24);
+      const uint32_t hi = (lo >> 8) | (P[j][4] << 24);
+      const uint32_t H = lo | (hi << 2);
+      c[j] = (hp[j] | (H << 4)) & kb;
+      hp[j] = H;
+    }
+    if (z > z0) grayLevels<NL, 0>(c, nL, sh, ones);
+#pragma unroll
+    for (int r = 0; r <= kBatchTY; ++r) cur[r] = nxt[r];
+  }
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    if (l < nL) __hip_atomic_fetch_add(&sh.ones[l][lane], ones[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+}
+
 __global__ void kBatchReset(unsigned int* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
 }
@@ -312,10 +482,28 @@ __global__ __launch_bounds__(kBatchWaves* kWave) void kBatch(const BatchField* _
     if (i >= nItems) break;
     curField = f;
     const BatchField& fd = fields[f];
-    if (fd.enc == ENC_F32) {
-      batchItem<float>(sh, fd, items[i]);
+    const BatchItem it = items[i];
+    const bool f32 = fd.enc == ENC_F32;
+    if (fd.gray == 8) {
+      if (fd.fam == kFamGE) {
+        if (f32) batchItemGray<float, kFamGE, 8>(sh, fd, it);
+        else batchItemGray<double, kFamGE, 8>(sh, fd, it);
+      } else {
+        if (f32) batchItemGray<float, kFamGT, 8>(sh, fd, it);
+        else batchItemGray<double, kFamGT, 8>(sh, fd, it);
+      }
+    } else if (fd.gray == 4) {
+      if (fd.fam == kFamGE) {
+        if (f32) batchItemGray<float, kFamGE, 4>(sh, fd, it);
+        else batchItemGray<double, kFamGE, 4>(sh, fd, it);
+      } else {
+        if (f32) batchItemGray<float, kFamGT, 4>(sh, fd, it);
+        else batchItemGray<double, kFamGT, 4>(sh, fd, it);
+      }
+    } else if (f32) {
+      batchItem<float>(sh, fd, it);
     } else {
-      batchItem<double>(sh, fd, items[i]);
+      batchItem<double>(sh, fd, it);
     }
   }
 }
@@ -349,14 +537,39 @@ MkfBatchShape normalise(const MkfBatchShape* s) {
   return r;
 }
 
-// Validates field `index` and fills its device descriptor (without shape).
-int describeField(const MkfBatchField& src, int index, BatchField& b) {
-  memset(&b, 0, sizeof(b));
+// Family of the Gray-plane form for a field's levels, 0 if they do not
+// qualify: every relation ">=" or "<" (kFamGE), or every one ">" or "<="
+// (kFamGT), "none" (0) and "all" (7) joining either; no threshold NaN.
+int grayFamily(const MkfBatchField& src) {
+  int fam = 0;
+  for (int l = 0; l < src.nLevels; ++l) {
+    const uint32_t m = src.pM[l].m & 7u;
+    const double t = src.pM[l].t[0];
+    if (t != t || m == 2 || m == 5) return 0;
+    const int f = (m == 6 || m == 1) ? kFamGE : (m == 4 || m == 3) ? kFamGT : 0;
+    if (f != 0) {
+      if (fam != 0 && fam != f) return 0;
+      fam = f;
+    }
+  }
+  return fam != 0 ? fam : kFamGE;
+}
+
+// Validates field `index` and appends its device descriptors (without
+// shape). A field with 3 or more levels of one Gray family gets the Gray-plane
+// form: its levels sorted by threshold, each as the family's base relation
+// with a complement flag ("<" = not ">=", "<=" = not ">"; "all" and "none"
+// as levels at an infinite threshold), in parts of at most 8 levels that
+// read the field once each. Other fields keep their levels in order (the
+// per-level form).
+int describeField(const MkfBatchField& src, int index, std::vector<BatchField>& out) {
   FieldDef d[3];
   const int nd = copyNonZeroDef(d, src.def, 3);
   if (!src.pF || !src.pBPFD || (src.enc != ENC_F32 && src.enc != ENC_F64) || src.nLevels < 1 ||
       src.nLevels > kMaxLevels || !src.pM || nd <= 0 || src.def[0] < 0 || src.def[1] < 0 || src.def[2] < 0)
     return setError(MKF_ERR_ARG, "batch: invalid description of field %d", index);
+  BatchField b;
+  memset(&b, 0, sizeof(b));
   b.p = src.pF;
   b.out = (unsigned long long*)src.pBPFD;
   b.enc = src.enc;
@@ -365,19 +578,72 @@ int describeField(const MkfBatchField& src, int index, BatchField& b) {
   b.Z = d[2];
   b.sy = d[0];
   b.sz = (long long)d[0] * d[1];
-  b.nLevels = src.nLevels;
+  b.outLevels = src.nLevels;
+  // f32 fields compare against the threshold rounded to float.
+  auto thr = [&](int l) { return (src.enc == ENC_F32) ? (double)(float)src.pM[l].t[0] : src.pM[l].t[0]; };
+  const int fam = src.nLevels >= 3 ? grayFamily(src) : 0;
+  if (fam == 0) {
+    b.nLevels = src.nLevels;
+    for (int l = 0; l < src.nLevels; ++l) {
+      b.t[l] = thr(l);
+      b.m[l] = src.pM[l].m & 7u;
+      b.perm[l] = (uint8_t)l;
+    }
+    out.push_back(b);
+    return MKF_OK;
+  }
+  struct Lv {
+    double t;
+    int idx;
+    bool comp;
+  };
+  std::vector<Lv> lv;
+  const double inf = __builtin_inf();
   for (int l = 0; l < src.nLevels; ++l) {
-    // f32 fields compare against the threshold rounded to float.
-    b.t[l] = (src.enc == ENC_F32) ? (double)(float)src.pM[l].t[0] : src.pM[l].t[0];
-    b.m[l] = src.pM[l].m & 7u;
+    const uint32_t m = src.pM[l].m & 7u;
+    if (fam == kFamGE) {
+      // bit = !(f < t); "all" is ">= -inf", "none" its complement
+      if (m == 6 || m == 1) lv.push_back({thr(l), l, m == 1});
+      else lv.push_back({-inf, l, m == 0});
+    } else {
+      // bit = f > t; "none" is "> +inf", "all" its complement
+      if (m == 4 || m == 3) lv.push_back({thr(l), l, m == 3});
+      else lv.push_back({inf, l, m == 7});
+    }
+  }
+  std::stable_sort(lv.begin(), lv.end(), [](const Lv& a, const Lv& c) { return a.t < c.t; });
+  const int n = src.nLevels;
+  const int parts = (n + 7) / 8;
+  for (int p = 0, first = 0; p < parts; ++p) {
+    const int cnt = n / parts + (p < n % parts ? 1 : 0);
+    BatchField q = b;
+    q.nLevels = cnt;
+    q.gray = cnt <= 4 ? 4 : 8;
+    q.fam = fam;
+    for (int l = 0; l < q.gray; ++l) {
+      if (l < cnt) {
+        const Lv& e = lv[first + l];
+        q.t[l] = e.t;
+        q.perm[l] = (uint8_t)e.idx;
+        if (e.comp) q.comp |= 1u << l;
+      } else {
+        q.t[l] = inf;  // above every level: keeps the bits a thermometer
+      }
+    }
+    out.push_back(q);
+    first += cnt;
   }
   return MKF_OK;
 }
 
+// Cells per 64-lane segment: the Gray-plane form gives lane 63 to the right
+// neighbour of lane 62.
+int segCells(const BatchField& b) { return b.gray ? kWave - 1 : kWave; }
+
 // Tile shape of one field: the largest power-of-two x-tile up to the shape's
 // limit that leaves the fewest empty segments at the ends of the rows.
 void shapeField(BatchField& b, const MkfBatchShape& s) {
-  const int nSeg = (int)std::max<long long>(1, cdiv(b.X - 1, kWave));
+  const int nSeg = (int)std::max<long long>(1, cdiv(b.X - 1, segCells(b)));
   int best = 1;
   long long bestWaste = 1LL << 40;
   for (int g = s.segments; g >= 1; g >>= 1) {
@@ -487,12 +753,13 @@ int batchLauncherRun(BatchLauncher* L, const MkfBatchField* fields, int n, const
   if (!L || !fields || n <= 0) return bfail(MKF_ERR_ARG, "no fields");
   const MkfBatchShape sh = normalise(shape);
   if (!shapeOk(sh)) return bfail(MKF_ERR_ARG, "shape: planePairs in 1..4096 and segments in {1, 2, 4, 8}");
-  L->fs.resize(n);
+  L->fs.clear();
   for (int f = 0; f < n; ++f) {
-    const int rc = describeField(fields[f], f, L->fs[f]);
+    const int rc = describeField(fields[f], f, L->fs);
     if (rc != MKF_OK) return rc;
-    shapeField(L->fs[f], sh);
   }
+  for (auto& b : L->fs) shapeField(b, sh);
+  n = (int)L->fs.size();
   buildItems(L->fs, L->items);
   if (L->items.size() > (size_t)0x7fffffff - 1) return bfail(MKF_ERR_UNSUPPORTED, "too many work items");
   const size_t oF = 0, oI = align256(sizeof(BatchField) * n);
@@ -521,7 +788,7 @@ int batchLauncherRun(BatchLauncher* L, const MkfBatchField* fields, int n, const
   int* counts = (int*)(L->hStage + oC);
   for (int f = 0; f < n; ++f) {
     outs[f] = L->fs[f].out;
-    counts[f] = 256 * L->fs[f].nLevels;
+    counts[f] = 256 * L->fs[f].outLevels;
   }
   if (copyWords(L->dBuf, L->hStageDev, total, s) != hipSuccess || hipEventRecord(L->staged, s) != hipSuccess)
     return bfail(MKF_ERR_HIP, "descriptor upload");
@@ -617,13 +884,16 @@ MkfBatch* mkfBatchCreate(const MkfBatchField* fields, int nFields, const MkfBatc
     bfail(MKF_ERR_ARG, "shape: planePairs in 1..4096 and segments in {1, 2, 4, 8}");
     return nullptr;
   }
-  std::vector<BatchField> host(nFields);
+  std::vector<BatchField> host;
+  for (int f = 0; f < nFields; ++f)
+    if (describeField(fields[f], f, host) != MKF_OK) return nullptr;
+  // Descriptors from here on (a field may have several).
+  nFields = (int)host.size();
   std::vector<unsigned long long*> outs;
   std::vector<int> counts;
-  for (int f = 0; f < nFields; ++f) {
-    if (describeField(fields[f], f, host[f]) != MKF_OK) return nullptr;
-    outs.push_back(host[f].out);
-    counts.push_back(256 * host[f].nLevels);
+  for (const auto& b : host) {
+    outs.push_back(b.out);
+    counts.push_back(256 * b.outLevels);
   }
   MkfBatch* B = new MkfBatch();
   B->flags = flags;
@@ -671,17 +941,23 @@ int mkfBatchAutotune(MkfBatch* B, const MkfBatchShape* candidates, int nCandidat
   if (B->broken) return bfail(MKF_ERR_ARG, "unusable after a failed re-plan; destroy it");
   hipStream_t st = (hipStream_t)stream;
   const MkfBatchShape prev = B->shape;
-  // Point every field at a scratch histogram while tuning.
+  // Point every field at a scratch histogram while tuning (the parts of a
+  // field, which are consecutive and share their output, share it too).
   size_t total = 0;
-  for (auto& b : B->host) total += (size_t)b.nLevels * 256;
+  for (auto& b : B->host) total += (size_t)b.outLevels * 256;
   unsigned long long* scratch = nullptr;
   if (hipMalloc((void**)&scratch, total * 8) != hipSuccess) return bfail(MKF_ERR_MEM, "tuning scratch");
   std::vector<unsigned long long*> saved;
   size_t off = 0;
-  for (auto& b : B->host) {
+  for (size_t f = 0; f < B->host.size(); ++f) {
+    BatchField& b = B->host[f];
     saved.push_back(b.out);
+    if (f > 0 && b.out == saved[f - 1]) {
+      b.out = B->host[f - 1].out;
+      continue;
+    }
     b.out = scratch + off;
-    off += (size_t)b.nLevels * 256;
+    off += (size_t)b.outLevels * 256;
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   (void)hipEventCreate(&e0);

@@ -233,3 +233,59 @@ def test_batch_autotune_failure_restores_or_refuses(lib, orc, cuda):
         lib.mkfDebugInjectFault(b"batch.upload", 0)
         lib.mkfDebugInjectFault(b"batch.capture", 0)
         lib.mkfBatchDestroy(b)
+
+
+INF = float("inf")
+GRAY_LEVELS = {
+    # ">=" / "<" family, unsorted, with duplicates, "all" ("!") and "none" ("")
+    "ge8": [(">=", -1.0), ("<", 0.5), (">=", 0.0), ("<", 0.0), (">=", 1.5), ("<", -1.5), ("!", 0.0), ("", 0.0)],
+    # ">" / "<=" family, 5 levels (8-level planes, 3 padded)
+    "gt5": [(">", 0.2), ("<=", -0.4), (">", -0.4), ("<=", 0.9), (">", 0.2)],
+    # 3 levels: the 4-level planes
+    "ge3": [(">=", 0.1), ("<", -0.1), (">=", 0.1)],
+    # 4 levels at infinite and zero thresholds
+    "gt4": [(">", INF), ("<=", -INF), (">", -0.0), ("<=", 0.0)],
+    # 9 levels: parts of 5 and 4
+    "ge9": [("<" if k % 3 == 0 else ">=", t) for k, t in enumerate(np.linspace(1.2, -1.2, 9))],
+    # 16 levels: two parts of 8, thresholds unsorted, both relations
+    "gt16": [("<=" if k % 2 else ">", float(t)) for k, t in
+             enumerate(np.random.default_rng(3).uniform(-1.8, 1.8, 16))],
+    # only "all" and "none": the default family at infinite thresholds
+    "allnone": [("!", 0.3), ("", 0.3), ("!", -2.0)],
+}
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_batch_gray_levels_match_oracle(lib, orc, cuda, dtype):
+    """Fields whose levels all belong to one family (">=" and "<", or ">" and
+    "<=") take the Gray-plane form: sorted levels, complemented histograms,
+    parts of at most 8 levels, 63-cell segments. Every histogram equals the
+    oracle's, on row lengths around the 63-cell segment edges and with NaN,
+    +-inf, +-0 and exact-threshold voxels."""
+    names = list(GRAY_LEVELS)
+    xs = [2, 3, 62, 63, 64, 65, 126, 127, 128, 190, 253]
+    specs = []
+    for k, x in enumerate(xs):
+        lv = GRAY_LEVELS[names[k % len(names)]]
+        specs.append(((3 + k % 4, 2 + (5 * k) % 11, x), dtype, lv))
+    for k, name in enumerate(names):
+        specs.append(((9, 13, 70 + 17 * k), dtype, GRAY_LEVELS[name]))
+    fields, maps, outs, arr = make_batch(cuda, specs, seed=60)
+    for i, (shape, _, lv) in enumerate(specs):
+        thr = [(r, float(np.float32(t))) for r, t in lv if np.isfinite(t)] or [(">=", 0.0)]
+        fields[i].copy_(_special_field(shape, dtype, 70 + i, thr).to(cuda))
+    ref = oracle_hists(orc, fields, maps)
+    for depth, segs in ((1, 8), (4, 2), (64, 1)):
+        for o in outs:
+            o.fill_(5)
+        b = lib.mkfBatchCreate(arr, len(specs), C.byref(bshape(depth, segs)), M.MKF_BATCH_ZERO_OUTPUT, None)
+        assert b, M.last_error()
+        try:
+            print(f"gray levels {dtype}: depth {depth} segments {segs}", flush=True)
+            M.check(lib.mkfBatchRun(b, None))
+            torch.cuda.synchronize()
+            for i, (o, bms) in enumerate(zip(outs, maps)):
+                np.testing.assert_array_equal(got(o, len(bms)), ref[i],
+                                              err_msg=f"depth {depth} segments {segs} field {i} {specs[i][0]}")
+        finally:
+            lib.mkfBatchDestroy(b)
