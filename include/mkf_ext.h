@@ -196,7 +196,9 @@ typedef struct {
 
 /* Work-item (slab) shape of a batch. A work item is a slab of planePairs
  * plane pairs by 8 row-stacks-worth of row pairs by one x-tile of
- * `segments` 64-voxel segments; the 8 waves of a workgroup cover
+ * `segments` segments of 64 cells (63 for fields in the Gray-plane form:
+ * 3 or more levels, all ">=" / "<" or all ">" / "<=", "all" and "none"
+ * joining either, no NaN threshold); the 8 waves of a workgroup cover
  * 8 / segments row stacks of 4 row pairs each. */
 typedef struct {
   int32_t planePairs; /* slab depth, 1..4096; 0 selects 32 */

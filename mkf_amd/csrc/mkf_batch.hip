@@ -35,9 +35,10 @@
 namespace mkf {
 
 constexpr int kBatchWaves = 8;
-constexpr int kBatchTY = 4;
+constexpr int kBatchTY = 4;  // row pairs per stack, per-level form
+constexpr int kGrayTY = 4;   // row pairs per stack, Gray-plane form
 constexpr int kMaxLevels = MKF_BATCH_MAX_LEVELS;
-// Slab depth limit: one item adds at most 8 x 64 x 4 x 4096 cells, so the
+// Slab depth limit: one item adds at most 8 x 64 x 8 x 4096 cells, so the
 // 32-bit LDS counters, flushed once they pass 2^31, cannot wrap.
 constexpr int kMaxPlanePairs = 4096;
 
@@ -346,20 +347,25 @@ __device__ __forceinline__ void grayTally(uint32_t W, uint32_t* hist, uint32_t& 
   t.addByte<3>(W, D, hist);
 }
 
-template <int NL, int L>
-__device__ __forceinline__ void grayLevels(const uint32_t (&c)[kGrayPlanes<NL>], int nL, BatchShared& sh,
+// Levels L.. of one z-step; c[k][j]: plane j of row-pair word k.
+template <int NL, int NW, int L>
+__device__ __forceinline__ void grayLevels(const uint32_t (&c)[NW][kGrayPlanes<NL>], int nL, BatchShared& sh,
                                            uint32_t (&ones)[NL]) {
   if constexpr (L < NL) {
     if (L < nL) {
-      grayTally(grayLevel<NL, L>(c), sh.hist[L], ones[L]);
-      grayLevels<NL, L + 1>(c, nL, sh, ones);
+#pragma unroll
+      for (int k = 0; k < NW; ++k) grayTally(grayLevel<NL, L>(c[k]), sh.hist[L], ones[L]);
+      grayLevels<NL, NW, L + 1>(c, nL, sh, ones);
     }
   }
 }
 
-template <typename T, int FAM, int NL>
+// One work item in the Gray-plane form: the wave's stack of TY + 1 rows (TY
+// row pairs, TY / 4 code words per plane) over the item's slab, all levels.
+template <typename T, int FAM, int NL, int TY>
 __device__ void batchItemGray(BatchShared& sh, const BatchField& fd, const BatchItem it) {
   constexpr int NP = kGrayPlanes<NL>;
+  constexpr int NW = TY / 4;         // row-pair words per plane
   constexpr int kCells = kWave - 1;  // cells per segment
   const int lane = threadIdx.x & (kWave - 1);
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -370,8 +376,8 @@ __device__ void batchItemGray(BatchShared& sh, const BatchField& fd, const Batch
   const int stack = w / segs;
   const bool active = stack < fd.stacks;
   const int x0 = (it.xt * segs + seg) * kCells;
-  const int y0 = (it.yb * fd.stacks + (active ? stack : 0)) * kBatchTY;
-  const int nY = active ? max(0, min(kBatchTY, Y - 1 - y0)) : 0;  // 0: the stack counts nothing
+  const int y0 = (it.yb * fd.stacks + (active ? stack : 0)) * TY;
+  const int nY = active ? max(0, min(TY, Y - 1 - y0)) : 0;  // 0: the stack counts nothing
   const int yc = min(y0, Y - 1);
   const int z0 = it.zr * fd.TZ;
   const int nZ = min(fd.TZ, fd.Z - 1 - z0);
@@ -382,31 +388,37 @@ __device__ void batchItemGray(BatchShared& sh, const BatchField& fd, const Batch
   const uint32_t lx = (uint32_t)(min(x0 + lane, X - 1) - xb);
   const uint64_t valid = ballot(lane < kCells && x0 + lane < X - 1);
   if (lane == 0 && nY > 0) atomicAdd(&sh.cells, (uint32_t)__builtin_popcountll(valid) * (uint32_t)(nY * nZ));
-  const uint32_t kb = laneBit(valid) ? (nY >= 4 ? 0xFFFFFFFFu : (1u << (8 * nY)) - 1u) : 0u;
+  // Bytes of cells that exist, per row-pair word.
+  uint32_t kb[NW];
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    const int n = min(max(nY - 4 * k, 0), 4);
+    kb[k] = laneBit(valid) ? (n >= 4 ? 0xFFFFFFFFu : (1u << (8 * n)) - 1u) : 0u;
+  }
 
   T thr[NL];
 #pragma unroll
   for (int l = 0; l < NL; ++l) thr[l] = (T)fd.t[l];
 
-  long long yo[kBatchTY + 1];
+  long long yo[TY + 1];
 #pragma unroll
-  for (int r = 0; r <= kBatchTY; ++r) yo[r] = (long long)(yc + min(r, nY)) * sy;
-  T cur[kBatchTY + 1], nxt[kBatchTY + 1];
+  for (int r = 0; r <= TY; ++r) yo[r] = (long long)(yc + min(r, nY)) * sy;
+  T cur[TY + 1], nxt[TY + 1];
 #pragma unroll
-  for (int r = 0; r <= kBatchTY; ++r) cur[r] = globalLoad(fw, yo[r] + (long long)z0 * sz + lx);
-  uint32_t hp[NP] = {};
+  for (int r = 0; r <= TY; ++r) cur[r] = globalLoad(fw, yo[r] + (long long)z0 * sz + lx);
+  uint32_t hp[NW][NP] = {};
   uint32_t ones[NL];
 #pragma unroll
   for (int l = 0; l < NL; ++l) ones[l] = 0;
   for (int z = z0; z <= z0 + nZ; ++z) {
     const long long zon = (long long)min(z + 1, z0 + nZ) * sz;
 #pragma unroll
-    for (int r = 0; r <= kBatchTY; ++r) nxt[r] = globalLoad(fw, yo[r] + zon + lx);
+    for (int r = 0; r <= TY; ++r) nxt[r] = globalLoad(fw, yo[r] + zon + lx);
     // Pair bits of every plane and row: bit 0 the lane's voxel, bit 1 its
     // right neighbour (lane l + 1).
-    uint32_t P[NP][kBatchTY + 1];
+    uint32_t P[NP][TY + 1];
 #pragma unroll
-    for (int r = 0; r <= kBatchTY; ++r) {
+    for (int r = 0; r <= TY; ++r) {
       uint64_t A[NL];
 #pragma unroll
       for (int l = 0; l < NL; ++l) A[l] = famMask<FAM>(cur[r], thr[l]);
@@ -424,18 +436,22 @@ __device__ void batchItemGray(BatchShared& sh, const BatchField& fd, const Batch
 #pragma unroll
       for (int j = 0; j < NP; ++j) P[j][r] = addLaneBit(laneBit(g[j] >> 1) ? 2u : 0u, g[j]);
     }
-    uint32_t c[NP];
+    uint32_t c[NW][NP];
 #pragma unroll
-    for (int j = 0; j < NP; ++j) {
-      const uint32_t lo = P[j][0] | (P[j][1] << 8) | (P[j][2] << 16) | (P[j][3] << 24);
-      const uint32_t hi = (lo >> 8) | (P[j][4] << 24);
-      const uint32_t H = lo | (hi << 2);
-      c[j] = (hp[j] | (H << 4)) & kb;
-      hp[j] = H;
+    for (int k = 0; k < NW; ++k) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        const uint32_t* q = &P[j][4 * k];
+        const uint32_t lo = q[0] | (q[1] << 8) | (q[2] << 16) | (q[3] << 24);
+        const uint32_t hi = (lo >> 8) | (q[4] << 24);
+        const uint32_t H = lo | (hi << 2);
+        c[k][j] = (hp[k][j] | (H << 4)) & kb[k];
+        hp[k][j] = H;
+      }
     }
-    if (z > z0) grayLevels<NL, 0>(c, nL, sh, ones);
+    if (z > z0) grayLevels<NL, NW, 0>(c, nL, sh, ones);
 #pragma unroll
-    for (int r = 0; r <= kBatchTY; ++r) cur[r] = nxt[r];
+    for (int r = 0; r <= TY; ++r) cur[r] = nxt[r];
   }
 #pragma unroll
   for (int l = 0; l < NL; ++l) {
@@ -486,19 +502,19 @@ __global__ __launch_bounds__(kBatchWaves* kWave) void kBatch(const BatchField* _
     const bool f32 = fd.enc == ENC_F32;
     if (fd.gray == 8) {
       if (fd.fam == kFamGE) {
-        if (f32) batchItemGray<float, kFamGE, 8>(sh, fd, it);
-        else batchItemGray<double, kFamGE, 8>(sh, fd, it);
+        if (f32) batchItemGray<float, kFamGE, 8, kGrayTY>(sh, fd, it);
+        else batchItemGray<double, kFamGE, 8, kGrayTY>(sh, fd, it);
       } else {
-        if (f32) batchItemGray<float, kFamGT, 8>(sh, fd, it);
-        else batchItemGray<double, kFamGT, 8>(sh, fd, it);
+        if (f32) batchItemGray<float, kFamGT, 8, kGrayTY>(sh, fd, it);
+        else batchItemGray<double, kFamGT, 8, kGrayTY>(sh, fd, it);
       }
     } else if (fd.gray == 4) {
       if (fd.fam == kFamGE) {
-        if (f32) batchItemGray<float, kFamGE, 4>(sh, fd, it);
-        else batchItemGray<double, kFamGE, 4>(sh, fd, it);
+        if (f32) batchItemGray<float, kFamGE, 4, kGrayTY>(sh, fd, it);
+        else batchItemGray<double, kFamGE, 4, kGrayTY>(sh, fd, it);
       } else {
-        if (f32) batchItemGray<float, kFamGT, 4>(sh, fd, it);
-        else batchItemGray<double, kFamGT, 4>(sh, fd, it);
+        if (f32) batchItemGray<float, kFamGT, 4, kGrayTY>(sh, fd, it);
+        else batchItemGray<double, kFamGT, 4, kGrayTY>(sh, fd, it);
       }
     } else if (f32) {
       batchItem<float>(sh, fd, it);
@@ -657,7 +673,7 @@ void shapeField(BatchField& b, const MkfBatchShape& s) {
   b.stacks = kBatchWaves / best;
   b.TZ = s.planePairs;
   b.nXT = (int)cdiv(nSeg, best);
-  b.nYB = (int)cdiv(cdiv(std::max(b.Y - 1, 1), kBatchTY), b.stacks);
+  b.nYB = (int)cdiv(cdiv(std::max(b.Y - 1, 1), b.gray ? kGrayTY : kBatchTY), b.stacks);
   b.nZR = (int)cdiv(std::max(b.Z - 1, 1), b.TZ);
 }
 
