@@ -22,6 +22,11 @@
 // cells are counted per lane by a byte-wise test, mixed cells go to the
 // workgroup's LDS histogram (one SDWA compare and one atomic per byte), and
 // all-zero cells are what remains of the cell count.
+//
+// Fields whose levels all share one relation family (">=" / "<", or ">" /
+// "<="; batchItemGray) take a faster form: the levels sorted, the pair bits
+// and packing done once per Gray plane of a voxel's level count instead of
+// once per level, and each level's code word recovered by a bitwise formula.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -775,7 +780,7 @@ int batchLauncherRun(BatchLauncher* L, const MkfBatchField* fields, int n, const
     if (rc != MKF_OK) return rc;
   }
   for (auto& b : L->fs) shapeField(b, sh);
-  n = (int)L->fs.size();
+  n = (int)L->fs.size();  // descriptors from here on (a field may have several)
   buildItems(L->fs, L->items);
   if (L->items.size() > (size_t)0x7fffffff - 1) return bfail(MKF_ERR_UNSUPPORTED, "too many work items");
   const size_t oF = 0, oI = align256(sizeof(BatchField) * n);
@@ -903,8 +908,7 @@ MkfBatch* mkfBatchCreate(const MkfBatchField* fields, int nFields, const MkfBatc
   std::vector<BatchField> host;
   for (int f = 0; f < nFields; ++f)
     if (describeField(fields[f], f, host) != MKF_OK) return nullptr;
-  // Descriptors from here on (a field may have several).
-  nFields = (int)host.size();
+  const int nDesc = (int)host.size();  // a field may have several descriptors
   std::vector<unsigned long long*> outs;
   std::vector<int> counts;
   for (const auto& b : host) {
@@ -916,18 +920,18 @@ MkfBatch* mkfBatchCreate(const MkfBatchField* fields, int nFields, const MkfBatc
   B->host.swap(host);
   hipStream_t st = (hipStream_t)stream;
   B->grid = persistentGrid();
-  if (hipMalloc((void**)&B->dFields, sizeof(BatchField) * nFields) != hipSuccess ||
+  if (hipMalloc((void**)&B->dFields, sizeof(BatchField) * nDesc) != hipSuccess ||
       hipMalloc((void**)&B->dCounter, sizeof(unsigned int)) != hipSuccess ||
-      hipMalloc((void**)&B->dOuts, sizeof(void*) * nFields) != hipSuccess ||
-      hipMalloc((void**)&B->dCounts, sizeof(int) * nFields) != hipSuccess ||
+      hipMalloc((void**)&B->dOuts, sizeof(void*) * nDesc) != hipSuccess ||
+      hipMalloc((void**)&B->dCounts, sizeof(int) * nDesc) != hipSuccess ||
       hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking) != hipSuccess) {
     bfail(MKF_ERR_MEM, "batch allocation");
     mkfBatchDestroy(B);
     return nullptr;
   }
   int rc = MKF_OK;
-  if (hipMemcpyAsync(B->dOuts, outs.data(), sizeof(void*) * nFields, hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(B->dCounts, counts.data(), sizeof(int) * nFields, hipMemcpyHostToDevice, st) != hipSuccess)
+  if (hipMemcpyAsync(B->dOuts, outs.data(), sizeof(void*) * nDesc, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(B->dCounts, counts.data(), sizeof(int) * nDesc, hipMemcpyHostToDevice, st) != hipSuccess)
     rc = bfail(MKF_ERR_HIP, "upload of the output table");
   if (rc == MKF_OK) rc = upload(B, s, st);
   if (rc == MKF_OK) rc = capture(B);
