@@ -105,7 +105,10 @@ def test_fused_moments_and_batch_on_the_large_field(lib, big, cuda):
     m = mom.cpu().numpy()
     assert m[4] + m[5] == X * Y * Z
     small = torch.rand((40, 50, 60), generator=torch.Generator(device=cuda).manual_seed(3), device=cuda)
-    b = Batch([big, small], [[(">=", 0.5), ("<", 0.5)], [(">=", 0.5)]])
+    # two levels: the per-level form; three of one family: the Gray-plane form
+    ge25 = fused(lib, big, ">=", 0.25)
+    b = Batch([big, big, small], [[(">=", 0.5), ("<", 0.5)], [("<", 0.5), (">=", 0.25), (">=", 0.5)],
+                                  [(">=", 0.5)]])
     try:
         b.run()
         torch.cuda.synchronize()
@@ -113,7 +116,11 @@ def test_fused_moments_and_batch_on_the_large_field(lib, big, cuda):
         np.testing.assert_array_equal(hb[0], ge)
         np.testing.assert_array_equal(hb[1], ge[::-1])
         assert int(hb[0].sum()) == cells
-        np.testing.assert_array_equal(b.hist(1).cpu().numpy().view(np.uint64)[0],
+        hg = b.hist(1).cpu().numpy().view(np.uint64)
+        np.testing.assert_array_equal(hg[0], ge[::-1])
+        np.testing.assert_array_equal(hg[1], ge25)
+        np.testing.assert_array_equal(hg[2], ge)
+        np.testing.assert_array_equal(b.hist(2).cpu().numpy().view(np.uint64)[0],
                                       fused(lib, small, ">=", 0.5, dims=(60, 50, 40)))
     finally:
         b.close()
