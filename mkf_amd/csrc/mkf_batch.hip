@@ -176,12 +176,12 @@ __device__ __forceinline__ uint32_t levelStep(const T (&cur)[kBatchTY + 1], cons
     const uint32_t C = (hp | (H << 4)) & kb;
     __hip_atomic_fetch_add(onesL, (uint32_t)__builtin_popcount(allOneBytes(C)), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
-    const uint32_t D = bytesMinusOne(C);
+    const uint32_t X = mixedBits(C);
     Tally t;
-    t.addByte<0>(C, D, histL);
-    t.addByte<1>(C, D, histL);
-    t.addByte<2>(C, D, histL);
-    t.addByte<3>(C, D, histL);
+    t.addMixed<0>(C, X, histL);
+    t.addMixed<1>(C, X, histL);
+    t.addMixed<2>(C, X, histL);
+    t.addMixed<3>(C, X, histL);
   }
   return H;
 }
@@ -344,12 +344,12 @@ __device__ __forceinline__ uint32_t grayLevel(const uint32_t (&c)[kGrayPlanes<NL
 // cells to the level's LDS histogram.
 __device__ __forceinline__ void grayTally(uint32_t W, uint32_t* hist, uint32_t& ones) {
   ones += (uint32_t)__builtin_popcount(allOneBytes(W));
-  const uint32_t D = bytesMinusOne(W);
+  const uint32_t X = mixedBits(W);
   Tally t;
-  t.addByte<0>(W, D, hist);
-  t.addByte<1>(W, D, hist);
-  t.addByte<2>(W, D, hist);
-  t.addByte<3>(W, D, hist);
+  t.addMixed<0>(W, X, hist);
+  t.addMixed<1>(W, X, hist);
+  t.addMixed<2>(W, X, hist);
+  t.addMixed<3>(W, X, hist);
 }
 
 // Levels L.. of one z-step; c[k][j]: plane j of row-pair word k.

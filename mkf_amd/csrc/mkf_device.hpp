@@ -517,6 +517,12 @@ __device__ __forceinline__ uint32_t allOneBytes(uint32_t C) {
   return ((C & 0x7F7F7F7Fu) + 0x01010101u) & C & 0x80808080u;
 }
 
+// Byte k of the result is nonzero iff byte k of C is mixed (neither 0 nor
+// 255): it holds the XOR of each pair of neighbouring bits of byte k, which
+// are all equal only for 0x00 and 0xFF (bit 7 is cleared: it would pair with
+// the next byte's bit 0).
+__device__ __forceinline__ uint32_t mixedBits(uint32_t C) { return (C ^ (C >> 1)) & 0x7F7F7F7Fu; }
+
 // The lanes whose byte K of D is below 254, i.e. whose cell K is mixed when D
 // = bytesMinusOne(C) (codes 0 and 255 become 255 and 254): one compare with
 // an SDWA byte select.
