@@ -174,9 +174,9 @@ __device__ __forceinline__ uint32_t levelStep(const T (&cur)[kBatchTY + 1], cons
   const uint32_t H = lo | (hi << 2);
   if (count) {
     const uint32_t C = (hp | (H << 4)) & kb;
-    __hip_atomic_fetch_add(onesL, (uint32_t)__builtin_popcount(allOneBytes(C)), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_WORKGROUP);
     const uint32_t X = mixedBits(C);
+    __hip_atomic_fetch_add(onesL, (uint32_t)__builtin_popcount(allOneBytesX(C, X)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
     Tally t;
     t.addMixed<0>(C, X, histL);
     t.addMixed<1>(C, X, histL);
@@ -343,8 +343,8 @@ __device__ __forceinline__ uint32_t grayLevel(const uint32_t (&c)[kGrayPlanes<NL
 // Tally of one level's code word: all-one cells counted per lane, mixed
 // cells to the level's LDS histogram.
 __device__ __forceinline__ void grayTally(uint32_t W, uint32_t* hist, uint32_t& ones) {
-  ones += (uint32_t)__builtin_popcount(allOneBytes(W));
   const uint32_t X = mixedBits(W);
+  ones += (uint32_t)__builtin_popcount(allOneBytesX(W, X));
   Tally t;
   t.addMixed<0>(W, X, hist);
   t.addMixed<1>(W, X, hist);

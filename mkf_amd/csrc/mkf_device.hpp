@@ -522,6 +522,13 @@ __device__ __forceinline__ uint32_t allOneBytes(uint32_t C) {
 // are all equal only for 0x00 and 0xFF (bit 7 is cleared: it would pair with
 // the next byte's bit 0).
 __device__ __forceinline__ uint32_t mixedBits(uint32_t C) { return (C ^ (C >> 1)) & 0x7F7F7F7Fu; }
+// Bit 7 of byte k is set iff byte k of C is 0xFF, given X = mixedBits(C):
+// X + 0x7F7F7F7F sets bit 7 of exactly the bytes with X nonzero (no byte
+// carries: X is at most 0x7F per byte), and a byte that is not mixed is 0xFF
+// iff its bit 7 is set.
+__device__ __forceinline__ uint32_t allOneBytesX(uint32_t C, uint32_t X) {
+  return C & ~(X + 0x7F7F7F7Fu) & 0x80808080u;
+}
 
 // The lanes whose byte K of D is below 254, i.e. whose cell K is mixed when D
 // = bytesMinusOne(C) (codes 0 and 255 become 255 and 254): one compare with
