@@ -252,6 +252,9 @@ GRAY_LEVELS = {
              enumerate(np.random.default_rng(3).uniform(-1.8, 1.8, 16))],
     # only "all" and "none": the default family at infinite thresholds
     "allnone": [("!", 0.3), ("", 0.3), ("!", -2.0)],
+    # not the Gray-plane form: a NaN threshold, and levels of both families
+    "nan": [(">=", float("nan")), ("<", 0.5), (">=", -0.5)],
+    "both": [(">=", 0.1), (">", 0.1), ("<=", -0.3), ("<", 0.6)],
 }
 
 
