@@ -566,12 +566,22 @@ int mkfFusedAutotune(const BMFieldInfo* pI, const BinMapF64* pM, MkfTiling* pCho
   // The built-in choice and the shapes that came out ahead of it on some
   // MI355X at 1024^3; several fields or an x stride run one voxel per lane.
   // Float fields also get the persistent 8- and 16-voxel kernels, whichever
-  // the built-in choice is not.
+  // the built-in choice is not, and each of them with its slab depth spread
+  // over three quarters of its resident slots: fewer, longer waves per CU,
+  // which streamed faster on some shapes (1024^3: 0.642 against 0.668 ms, one
+  // tile per wave either way) and slower on others (1000^3: 0.633 against
+  // 0.612).
   std::vector<Tiling> cand{chooseTiling(fv.X, fv.Y, fv.Z, eb)};
   if (eb == 4) {
     if (fv.X >= 4 && fv.nF == 1 && fv.sx == 1) {
       if (cand[0].V != 16) cand.push_back(choose16Tiling(fv.X, fv.Y, fv.Z, 16));
       if (cand[0].V != 8) cand.push_back(choose16Tiling(fv.X, fv.Y, fv.Z, 8));
+      for (int v : {16, 8}) {
+        const Tiling t = choose16Tiling(fv.X, fv.Y, fv.Z, v, 3);
+        bool dup = false;
+        for (const Tiling& c : cand) dup |= c.V == t.V && c.TP == t.TP && c.Ty == t.Ty && c.ZW == t.ZW;
+        if (!dup) cand.push_back(t);
+      }
     }
     cand.push_back(Tiling{256, 16, 4, 4});
     cand.push_back(Tiling{128, 16, 4, 1});

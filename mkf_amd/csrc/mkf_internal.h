@@ -63,7 +63,9 @@ int setError(int code, const char* fmt, ...);
 bool faultDue(const char* point);
 
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes);  // fused kernel
-Tiling choose16Tiling(int X, int Y, int Z, int V = 16);     // fused kernel, 8 or 16 voxels per lane (floats)
+// Fused kernel, 8 or 16 voxels per lane (floats): the slab depth that spreads
+// the tiles best over quarters / 4 of the kernel's resident slots.
+Tiling choose16Tiling(int X, int Y, int Z, int V = 16, int quarters = 4);
 Tiling chooseBpfdTiling(int X, int Y, int Z);              // histogram of a PBM
 Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes);  // fused kernel with moments
 bool tilingSupported(const Tiling& tl, int elemBytes);

@@ -976,12 +976,13 @@ static long long persistentWalk(int X, int Y, int Z, const Tiling& tl) {
 
 // The persistent fused kernel with V = 8 or 16 voxels per lane, 4 row pairs
 // and the slab depth of the cost model (any float field with X >= 4).
-Tiling choose16Tiling(int X, int Y, int Z, int V) {
+Tiling choose16Tiling(int X, int Y, int Z, int V, int quarters) {
   Tiling tl;
   tl.V = V == 8 ? 8 : 16;
   tl.Ty = 4;
   tl.ZW = 16;
-  tl.TP = persistentDepth(X, Y, Z, 4, kWave * tl.V, fused16Slots(tl.V, X));
+  const long long slots = std::max<long long>(1, (long long)fused16Slots(tl.V, X) * quarters / 4);
+  tl.TP = persistentDepth(X, Y, Z, 4, kWave * tl.V, slots);
   return tl;
 }
 

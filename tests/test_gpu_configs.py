@@ -64,7 +64,10 @@ def fused(lib, field, bm, tiling=None):
 
 TILINGS = {
     # the built-in choice (None) and each candidate mkfFusedAutotune times
-    "f32": [None, (128, 16, 4, 16), (128, 16, 4, 8), (64, 16, 2, 16), (256, 16, 4, 4), (128, 16, 4, 1)],
+    # (171 and 341 plane pairs: the persistent kernels spread over three
+    # quarters of their slots)
+    "f32": [None, (128, 16, 4, 16), (128, 16, 4, 8), (64, 16, 2, 16), (256, 16, 4, 4), (128, 16, 4, 1),
+            (171, 16, 4, 16), (341, 16, 4, 8)],
     "f64": [None, (512, 16, 4, 2), (512, 16, 8, 1), (256, 16, 4, 2)],
 }
 
