@@ -40,7 +40,7 @@ HIP_SOURCES = ["mkf_api.hip", "mkf_batch.hip", "mkf_serve.hip", "mkf_stream.hip"
 # mkf_kernels.hip is compiled once per part (-DMKF_KPART=i), each part holding
 # a disjoint set of launchers and their kernel instantiations.
 KERNEL_SOURCE = "mkf_kernels.hip"
-KERNEL_PARTS = 8
+KERNEL_PARTS = 9
 C_SOURCES = ["binmap_util.c", "measures.c"]
 
 

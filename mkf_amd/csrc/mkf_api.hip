@@ -392,7 +392,7 @@ int mkfFusedBPFDMoments(size_t* pBPFD, double* pMom, BMOrg* pO, const BMFieldInf
   const hipStream_t s = (hipStream_t)stream;
   // A field without cells adds nothing to the histogram; its moments come
   // from the map kernel (no map written).
-  const long long nParts = fusedMomentParts(fv, tl);
+  const long long nParts = fusedMomentParts(fv, tl, pM->m);
   const size_t bytes = (size_t)(nParts ? nParts : pbmMomentParts(fv)) * 6 * sizeof(double);
   double* part = nullptr;
   hipError_t e = hipMallocAsync((void**)&part, bytes, s);
@@ -454,7 +454,7 @@ int mkfFusedBPFDMomentsNoAlloc(size_t* pBPFD, double* pMom, BMOrg* pO, const BMF
                        : chooseMomTiling(fv.X, fv.Y, fv.Z, fv.enc & ENC_MASK_NUM);
   rc = checkTiling(tl, fv.enc & ENC_MASK_NUM);
   if (rc) return rc;
-  const long long nParts = fusedMomentParts(fv, tl);
+  const long long nParts = fusedMomentParts(fv, tl, pM->m);
   const long long need = nParts ? nParts : pbmMomentParts(fv);
   const long long maxParts = (long long)(kNoAllocDoubles / MKF_MOM_N);
   if (need > maxParts)

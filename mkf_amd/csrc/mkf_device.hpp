@@ -437,16 +437,16 @@ struct PbmRows4 {
 
 // Per-lane phase moments: sums in double of the values and their squares per
 // phase (map bit 0 or 1) and the counts. Selects, not differences, split the
-// phases, so a NaN stays in its own phase.
+// phases, so a NaN stays in its own phase. The selects act on the value
+// before its conversion to double (one 32-bit select each for a float).
 struct MomAcc {
   double s0 = 0, q0 = 0, s1 = 0, q1 = 0;
   uint32_t n1 = 0, nAll = 0;
 
   template <typename T>
   __device__ __forceinline__ void add(T v, bool in, bool own) {
-    const double f = (double)v;
-    const double fi = (own && in) ? f : 0.0;
-    const double fo = (own && !in) ? f : 0.0;
+    const double fi = (double)((own && in) ? v : T(0));
+    const double fo = (double)((own && !in) ? v : T(0));
     s1 += fi;
     q1 = __builtin_fma(fi, fi, q1);
     s0 += fo;
@@ -456,14 +456,61 @@ struct MomAcc {
   }
 };
 
+// v in the lanes of mask, +0 in the others: one v_cndmask on the float. (A
+// select written in C++ lets the compiler convert first and select both
+// halves of the double.)
+__device__ __forceinline__ float selectLanes(float v, uint64_t mask) {
+  float r;
+  asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(r) : "v"(v), "s"(mask));
+  return r;
+}
+
+// +0 in the lanes of mask, v in the others.
+__device__ __forceinline__ float selectOtherLanes(float v, uint64_t mask) {
+  float r;
+  asm("v_cndmask_b32_e64 %0, %1, 0, %2" : "=v"(r) : "v"(v), "s"(mask));
+  return r;
+}
+
+// Phase moments with the phases and the owned voxels given as wave masks:
+// the sums per lane in double, the counts per wave (wave-uniform, so in
+// scalar registers).
+struct MomWave {
+  double s0 = 0, q0 = 0, s1 = 0, q1 = 0;
+  uint64_t n1 = 0, nAll = 0;
+
+  __device__ __forceinline__ void sums(double fi, double fo) {
+    s1 += fi;
+    q1 = __builtin_fma(fi, fi, q1);
+    s0 += fo;
+    q0 = __builtin_fma(fo, fo, q0);
+  }
+  // One voxel per lane, every lane's voxel counted: in = the lanes whose
+  // voxel is in phase 1 (the caller counts them).
+  __device__ __forceinline__ void addSplit(float v, uint64_t in) {
+    sums((double)selectLanes(v, in), (double)selectOtherLanes(v, in));
+  }
+  // The same for the voxels of the lanes in own only.
+  __device__ __forceinline__ void addOwned(float v, uint64_t in, uint64_t own) {
+    sums((double)selectLanes(v, in & own), (double)selectLanes(v, ~in & own));
+  }
+  // As addOwned; returns the number of lanes counted in phase 1.
+  __device__ __forceinline__ uint32_t add(float v, uint64_t in, uint64_t own) {
+    addOwned(v, in, own);
+    return (uint32_t)__builtin_popcountll(in & own);
+  }
+};
+
 constexpr int kMomN = 6;  // {sum0, sumsq0, sum1, sumsq1, n0, n1}
 
 // Writes the workgroup's moments to part[blockIdx.x * kMomN ..] after a
 // fixed-order reduction (lanes by xor shuffles, then waves in order).
-// Every thread of the workgroup must call it.
-__device__ __forceinline__ void momBlockPartial(const MomAcc& a, double* __restrict__ part) {
+// Every thread of the workgroup must call it. n0 and n1 are further counts
+// of the calling lane (a wave's counts come in on one lane).
+__device__ __forceinline__ void momBlockPartial(const MomAcc& a, double* __restrict__ part, double n0 = 0.0,
+                                                double n1 = 0.0) {
   __shared__ double red[16][kMomN];
-  double t[kMomN] = {a.s0, a.q0, a.s1, a.q1, (double)(a.nAll - a.n1), (double)a.n1};
+  double t[kMomN] = {a.s0, a.q0, a.s1, a.q1, (double)(a.nAll - a.n1) + n0, (double)a.n1 + n1};
 #pragma unroll
   for (int j = 0; j < kMomN; ++j) {
 #pragma unroll

@@ -91,8 +91,8 @@ hipError_t launchFused(const FieldView& fv, const BinMapF64& m, unsigned long lo
 // As launchFused, and ADDS the phase moments of fv (as launchPBMMoments) to
 // mom, reading the field once. Needs X, Y, Z >= 2 (hipErrorInvalidValue
 // otherwise); the tiling's row pairs are set to 4. part is device scratch of
-// fusedMomentParts(fv, tl) x 6 doubles.
-long long fusedMomentParts(const FieldView& fv, const Tiling& tl);
+// fusedMomentParts(fv, tl, m.m) x 6 doubles (m: the relation mask).
+long long fusedMomentParts(const FieldView& fv, const Tiling& tl, uint32_t m);
 hipError_t launchFusedMoments(const FieldView& fv, const BinMapF64& m, unsigned long long* bpfd, double* part,
                               double* mom, const Tiling& tl, hipStream_t s);
 

@@ -15,13 +15,14 @@
 #include "mkf_device.hpp"
 #include "mkf_internal.h"
 
-// The file is compiled once per part (-DMKF_KPART=0..7), each part holding
+// The file is compiled once per part (-DMKF_KPART=0..8), each part holding
 // only its own launchers and so instantiating only its own kernels; the
 // parts compile in parallel. Without MKF_KPART one compile holds them all.
-//   0 tiling choice and the fused dispatcher   4 kFused16
-//   1 kPBM and the moment reduction            5 kFusedBPFD with moments
-//   2 kFusedBPFD for float fields              6 kBPFD4 / kBPFD16 / kBPFD
-//   3 kFusedBPFD for double fields             7 kPBM for double fields
+//   0 tiling choice and the fused dispatcher   5 kFusedBPFD with moments
+//   1 kPBM and the moment reduction            6 kBPFD4 / kBPFD16 / kBPFD
+//   2 kFusedBPFD for float fields              7 kPBM for double fields
+//   3 kFusedBPFD for double fields             8 kFused16 with moments
+//   4 kFused16
 #ifdef MKF_KPART
 #define MKF_PART(p) (MKF_KPART == (p))
 #else
@@ -531,10 +532,18 @@ __global__ __launch_bounds__(1024) void kBPFD4(const uint32_t* __restrict__ pW, 
 #endif
 constexpr int kFused16PerCU = MKF_FUSED16_PER_CU;
 
-template <int M, int TY, int NG>
+//
+// With MOM the kernel also sums the phase moments of the voxels each tile
+// owns: its segment's voxels below X, rows y0 .. y0 + nY - 1 (and row nY when
+// it is the field's last row), planes z0 .. z0 + nZ - 1 (and the last plane
+// likewise), and, when the segment ends one voxel short of the row's end,
+// that last voxel (read as the halo). Every workgroup writes its moments to
+// part[blockIdx.x] after its last tile.
+template <int M, int TY, int NG, bool MOM = false>
 __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, float thr, int X, int Y, int Z,
                                                   long long sy, long long sz, TileGrid g,
-                                                  unsigned long long* __restrict__ bpfd) {
+                                                  unsigned long long* __restrict__ bpfd,
+                                                  double* __restrict__ part = nullptr) {
   typedef float Vec __attribute__((ext_vector_type(4), aligned(4)));
   typedef const __attribute__((address_space(1))) Vec* GVec;
   __shared__ uint32_t hist[256];
@@ -544,6 +553,8 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
   constexpr int VPL = 4 * NG;  // voxels per lane
   Tally tally;
   uint32_t cells = 0;
+  MomWave mw;
+  uint32_t n1Lane = 0;  // the lane's voxels counted in phase 1 (wave counts are in mw)
   // Persistent: block b takes tiles b, b + gridDim.x, ... in the XCD-aware
   // order (L = (t & 7) * chunk + (t >> 3)), so all of a block's tiles lie in
   // its XCD's share of the field (the grid is a multiple of 8).
@@ -561,6 +572,8 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
     const int nZ = min(g.TZ, Z - 1 - z0);
     const bool vec = x0 + kWave * VPL <= X;
     const bool halo = x0 + kWave * VPL < X;  // the voxel right of the segment exists
+    const bool lastRow = y0 + nY == Y - 1, lastPlane = z0 + nZ == Z - 1;
+    const bool haloLast = x0 + kWave * VPL == X - 1;  // the halo is the row's last voxel
     // cells that exist: x < X - 1
     uint32_t keep[NG], nCells = 0;
 #pragma unroll
@@ -625,7 +638,22 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
       float hc[TY + 1];
       loadPlane((long long)z0 * sz, cur, hc);
       uint32_t Hp[TY][NG];
+      // Moments, partial segment: the row ends in group qs (the groups below
+      // it lie inside the row, those above it outside); pm[e] holds the lanes
+      // whose voxel e of group qs lies inside, ob the lane's voxels of group
+      // qs inside as 4 bits.
+      const int rowLen = min(X - x0, kWave * VPL);  // voxels of the segment inside the row
+      int qs = NG;
+      uint64_t pm[4] = {~0ull, ~0ull, ~0ull, ~0ull};
+      uint32_t ob = 15u;
+      if constexpr (MOM && !VEC) {
+        qs = rowLen >> 8;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) pm[e] = ballot(256 * qs + 4 * lane + e < rowLen);
+        ob = (1u << min(max(rowLen - 256 * qs - 4 * lane, 0), 4)) - 1u;
+      }
       for (int z = z0; z <= z0 + nZ; ++z) {
+        const bool planeOwn = z < z0 + nZ || lastPlane;
         uint32_t P[TY + 1][NG];
 #pragma unroll
         for (int r = 0; r <= TY; ++r) {
@@ -643,19 +671,56 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
               g4[2] = t2;
             }
           }
-          uint64_t m[NG][4];
-#pragma unroll
-          for (int q = 0; q < NG; ++q)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) m[q][e] = relMask<M>(cur[r][4 * q + e], thr);
           const uint64_t hb = (halo && (relMask<M>(hc[r], thr) & 1ull)) ? 1ull : 0ull;
+          if constexpr (MOM) {
+            // Group by group from the right, so that each group's masks die
+            // with it: its pair word, then the moments of its voxels. Rows
+            // past nY repeat row nY (the loads are clamped); row nY below TY
+            // is the field's last row, row TY only counts as the last row.
+            const bool rowOwn = planeOwn && (r < TY ? r <= nY : (nY == TY && lastRow));
+            uint64_t nextFirst = hb;
 #pragma unroll
-          for (int q = 0; q < NG; ++q) {
-            const uint64_t nextFirst = q < NG - 1 ? (m[q + 1][0] & 1ull) : hb;
-            uint32_t v5 = laneBit((m[q][0] >> 1) | (nextFirst << 63)) ? 1u : 0u;
+            for (int q = NG - 1; q >= 0; --q) {
+              uint64_t mq[4];
 #pragma unroll
-            for (int e = 3; e >= 0; --e) v5 = dblAddLaneBit(v5, m[q][e]);
-            P[r][q] = ((v5 * 0x204081u) & 0x03030303u) & keep[q];
+              for (int e = 0; e < 4; ++e) mq[e] = relMask<M>(cur[r][4 * q + e], thr);
+              uint32_t v5 = laneBit((mq[0] >> 1) | (nextFirst << 63)) ? 1u : 0u;
+#pragma unroll
+              for (int e = 3; e >= 0; --e) v5 = dblAddLaneBit(v5, mq[e]);
+              P[r][q] = ((v5 * 0x204081u) & 0x03030303u) & keep[q];
+              nextFirst = mq[0] & 1ull;
+              // v5 holds the lane's 4 phase bits in bits 0..3.
+              if (rowOwn && (VEC || q < qs)) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) mw.addSplit(cur[r][4 * q + e], mq[e]);
+                n1Lane += __builtin_popcount(v5 & 15u);
+              } else if (!VEC && rowOwn && q == qs) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) mw.addOwned(cur[r][4 * q + e], mq[e], pm[e]);
+                n1Lane += __builtin_popcount(v5 & ob);
+              }
+            }
+            if (rowOwn) {
+              mw.nAll += (uint32_t)rowLen;
+              if (haloLast) {
+                mw.n1 += mw.add(hc[r], hb, ballot(lane == 0));
+                mw.nAll += 1;
+              }
+            }
+          } else {
+            uint64_t m[NG][4];
+#pragma unroll
+            for (int q = 0; q < NG; ++q)
+#pragma unroll
+              for (int e = 0; e < 4; ++e) m[q][e] = relMask<M>(cur[r][4 * q + e], thr);
+#pragma unroll
+            for (int q = 0; q < NG; ++q) {
+              const uint64_t nextFirst = q < NG - 1 ? (m[q + 1][0] & 1ull) : hb;
+              uint32_t v5 = laneBit((m[q][0] >> 1) | (nextFirst << 63)) ? 1u : 0u;
+#pragma unroll
+              for (int e = 3; e >= 0; --e) v5 = dblAddLaneBit(v5, m[q][e]);
+              P[r][q] = ((v5 * 0x204081u) & 0x03030303u) & keep[q];
+            }
           }
         }
         if (z < z0 + nZ) loadPlane((long long)(z + 1) * sz, cur, hc);
@@ -678,6 +743,15 @@ __global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, f
       walk(std::false_type{});
   }
   flushHistogram(hist, &cnt, tally, cells, bpfd);
+  if constexpr (MOM) {
+    MomAcc acc;
+    acc.s0 = mw.s0;
+    acc.q0 = mw.q0;
+    acc.s1 = mw.s1;
+    acc.q1 = mw.q1;
+    momBlockPartial(acc, part, (lane == 0 ? (double)(mw.nAll - mw.n1) : 0.0) - (double)n1Lane,
+                    (lane == 0 ? (double)mw.n1 : 0.0) + (double)n1Lane);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -781,6 +855,11 @@ hipError_t launchFusedF64(const FieldView& fv, double thr, uint32_t m, unsigned 
 hipError_t launchFused16(const FieldView& fv, float thr, uint32_t m, unsigned long long* bpfd, const Tiling& tl,
                          hipStream_t s, LaunchInfo* info);  // part 4
 unsigned fused16Slots(int V, int X);      // part 4: resident slots of kFused16 with V voxels per lane
+unsigned fused16MomSlots(int V);          // part 8: the same for its moments form
+// part 8: kFused16 with moments (4 row pairs); with nParts set it only
+// reports its workgroup count (= moment partials) and launches nothing.
+hipError_t launchFused16Moments(const FieldView& fv, float thr, uint32_t m, unsigned long long* bpfd, double* part,
+                                const Tiling& tl, hipStream_t s, long long* nParts);
 
 // Kernel view of a field description; unused table slots repeat field 0.
 template <typename T>
@@ -1083,18 +1162,48 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
 //        pass: 0.715 + 0.670 ms
 //   f64: 2 voxels/lane, 128 plane pairs 1.595 ms (512: 1.609); two passes
 //        1.173 + 1.245 ms
-// Float fields take 32 plane pairs, or 16 where that gives fewer than 8
-// workgroups per CU (GRF fields, best of 3 rounds of 10 launches): 1024^3
-// 0.872 ms (64: 0.871); 1000 x 733 x 733 0.478 (64: 0.507); 300 x 669 x 669
-// 0.256 (8, the earlier choice: 0.418); 1030 x 722 x 722 0.796 (64: 0.873);
-// 1536 x 1000 x 700 1.232 (64: 1.227); at 16: 1000 x 366 x 366 0.153 (32:
-// 0.166); 700 x 500 x 500 0.216 (0.218); 2000 x 200 x 300 0.144 (0.142).
+// Float fields take the persistent kernel's moments form where each of its
+// waves walks at least 48 plane pairs: 16 voxels per lane for rows of whole
+// 1024-voxel segments, with the cost model's slab depth, else 8 voxels per
+// lane with 32 plane pairs. GRF fields, HIP events, warm clocks:
+//   1024^3         16: 0.814 ms (64 plane pairs; 128: 0.817)  8: 0.880  4: 0.861
+//   1000^3          8: 0.802 (16: 0.934)                       4: 0.846
+//   1536x1000x700   8: 0.791 (16 plane pairs: 0.803)           4: 1.240
+//   1030x1000x1000  8: 0.983                                   4: 1.662
+//   300x1000x3000   8: 1.020 (64: 1.109)                       4: 1.227
+//   2100x1000x500   8: 0.874 (16: 1.181)                       4: 1.562
+//   1030x722x722    8: 0.541                                   4: 0.890
+// Smaller fields walk too few planes per wave to fill the pipeline:
+//   1000x366x366    8: 0.184  4: 0.151;  700x500x500  8: 0.227  4: 0.213;
+//   2000x200x300    8: 0.159  4: 0.141;  256^3        8: 0.091  4: 0.070.
+// Elsewhere 4 voxels per lane with 32 plane pairs, or 16 where that gives
+// fewer than 8 workgroups per CU (best of 3 rounds of 10 launches):
+// 1000 x 733 x 733 0.478 (64: 0.507); 300 x 669 x 669 0.256 (8, the earlier
+// choice: 0.418); at 16: 1000 x 366 x 366 0.153 (32: 0.166); 700 x 500 x 500
+// 0.216 (0.218); 2000 x 200 x 300 0.144 (0.142).
+static long long momWalk(int X, int Y, int Z, const Tiling& tl) {
+  const long long tiles =
+      ceilDiv(X - 1, (long long)kWave * tl.V) * ceilDiv(Y - 1, tl.Ty) * ceilDiv(Z - 1, tl.TP);
+  return ceilDiv(tiles, (long long)fused16MomSlots(tl.V)) * tl.TP;
+}
+
 Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
   Tiling tl;
   tl.V = elemBytes >= 8 ? 2 : 4;
   tl.Ty = 4;
   tl.TP = elemBytes >= 8 ? 128 : 64;
   tl.ZW = 16;
+  if (elemBytes == 4 && X >= 4 && Y >= 2 && Z >= 2) {
+    Tiling tp = tl;
+    if (X >= kWave * 16 && X % (kWave * 16) <= 1) {
+      tp.V = 16;
+      tp.TP = persistentDepth(X, Y, Z, 4, kWave * 16, fused16MomSlots(16));
+      if (momWalk(X, Y, Z, tp) >= 48) return tp;
+    }
+    tp.V = 8;
+    tp.TP = 32;
+    if (momWalk(X, Y, Z, tp) >= 48) return tp;
+  }
   if (elemBytes == 4) {
     int cus = 0, dev = 0;
     (void)hipGetDevice(&dev);
@@ -1467,17 +1576,31 @@ static hipError_t dispatchFusedMomM(const FieldView& fv, T thr, uint32_t m, unsi
   }
 }
 
+// The persistent 8- and 16-voxel kernel takes one float field of unit x
+// stride and at least 4 voxels per row.
+static bool momentsBy16(const FieldView& fv, const Tiling& tl) {
+  return (tl.V == 8 || tl.V == 16) && fv.enc == ENC_F32 && fv.nF == 1 && fv.sx == 1 && fv.X >= 4;
+}
+
 static Tiling fusedMomTiling(const FieldView& fv, const Tiling& tl0) {
   Tiling tl = tl0;
   tl.Ty = 4;
-  if (tl.V >= 8) tl.V = 4;  // the 8- and 16-voxel kernel has no moments form
+  if (momentsBy16(fv, tl)) return tl;
+  if (tl.V >= 8) tl.V = 4;
   if (fv.nF > 1 || fv.sx != 1 || fv.X < tl.V) tl.V = 1;
   return tl;
 }
 
-long long fusedMomentParts(const FieldView& fv, const Tiling& tl) {
+long long fusedMomentParts(const FieldView& fv, const Tiling& tl0, uint32_t m) {
   if (fv.X < 2 || fv.Y < 2 || fv.Z < 2) return 0;
-  const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, fusedMomTiling(fv, tl));
+  const Tiling tl = fusedMomTiling(fv, tl0);
+  if (momentsBy16(fv, tl)) {
+    // The workgroup count follows the resident slots of the relation's own
+    // kernel.
+    long long n = 0;
+    return launchFused16Moments(fv, 0.f, m, nullptr, nullptr, tl, nullptr, &n) == hipSuccess ? n : 0;
+  }
+  const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, tl);
   return g.nTiles > 0 ? (long long)g.chunk * 8 : 0;
 }
 
@@ -1485,12 +1608,14 @@ hipError_t launchFusedMoments(const FieldView& fv, const BinMapF64& m, unsigned 
                               double* mom, const Tiling& tl0, hipStream_t s) {
   const int eb = fv.enc & ENC_MASK_NUM;
   if (!tilingSupported(tl0, eb)) return hipErrorInvalidValue;
-  const long long nParts = fusedMomentParts(fv, tl0);
+  const long long nParts = fusedMomentParts(fv, tl0, m.m);
   if (nParts == 0) return hipErrorInvalidValue;  // no cells: the caller takes the map path
   const Tiling tl = fusedMomTiling(fv, tl0);
   const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, tl);
   hipError_t e;
-  if (fv.enc == ENC_F32)
+  if (momentsBy16(fv, tl))
+    e = launchFused16Moments(fv, (float)m.t[0], m.m, bpfd, part, tl, s, nullptr);
+  else if (fv.enc == ENC_F32)
     e = tl.V == 1 ? dispatchFusedMomM<float, 1>(fv, (float)m.t[0], m.m, bpfd, part, g, s)
                   : dispatchFusedMomM<float, 4>(fv, (float)m.t[0], m.m, bpfd, part, g, s);
   else if (fv.enc == ENC_F64)
@@ -1500,6 +1625,56 @@ hipError_t launchFusedMoments(const FieldView& fv, const BinMapF64& m, unsigned 
     return hipErrorInvalidValue;
   if (e != hipSuccess) return e;
   return launchMomentsReduce(part, nParts, mom, s);
+}
+#endif
+
+#if MKF_PART(8)
+// The moments form runs up to 16 waves per CU (4 per SIMD), as many as its
+// registers allow: the 16-voxel kernel fits 2 per SIMD, the 8-voxel one 4.
+// Measured on MI355X (f32 GRF, 8-voxel kernel): 1536 x 1000 x 700 with 32
+// plane pairs 0.807 ms at 16 per CU against 0.885 at 8; 1024^3 0.852 against
+// 0.858.
+constexpr int kFused16MomPerCU = 16;
+
+unsigned fused16MomSlots(int V) {
+  return persistentSlots(V == 8 ? (const void*)kFused16<6, 4, 2, true> : (const void*)kFused16<6, 4, 4, true>,
+                         kFused16MomPerCU);
+}
+
+template <int M, int NG>
+static hipError_t launchFused16MomK(const FieldView& fv, float thr, unsigned long long* bpfd, double* part,
+                                    const Tiling& tl, hipStream_t s, long long* nParts) {
+  const TileGrid g = grid16(fv.X, fv.Y, fv.Z, tl);
+  const unsigned nBlocks = blocks16(g, tl, (const void*)kFused16<M, 4, NG, true>, kFused16MomPerCU);
+  if (nParts) {
+    *nParts = nBlocks;
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL((kFused16<M, 4, NG, true>), dim3(nBlocks), dim3(kWave), 0, s, (const float*)fv.p[0], thr, fv.X,
+                     fv.Y, fv.Z, fv.sy, fv.sz, g, bpfd, part);
+  return hipGetLastError();
+}
+
+template <int M>
+static hipError_t launchFused16MomM(const FieldView& fv, float thr, unsigned long long* bpfd, double* part,
+                                    const Tiling& tl, hipStream_t s, long long* nParts) {
+  return tl.V == 8 ? launchFused16MomK<M, 2>(fv, thr, bpfd, part, tl, s, nParts)
+                   : launchFused16MomK<M, 4>(fv, thr, bpfd, part, tl, s, nParts);
+}
+
+hipError_t launchFused16Moments(const FieldView& fv, float thr, uint32_t m, unsigned long long* bpfd, double* part,
+                                const Tiling& tl, hipStream_t s, long long* nParts) {
+  if (fv.X < 4 || fv.Y < 2 || fv.Z < 2 || tl.Ty != 4 || (tl.V != 8 && tl.V != 16)) return hipErrorInvalidValue;
+  switch (m & 7u) {
+    case 0: return launchFused16MomM<0>(fv, thr, bpfd, part, tl, s, nParts);
+    case 1: return launchFused16MomM<1>(fv, thr, bpfd, part, tl, s, nParts);
+    case 2: return launchFused16MomM<2>(fv, thr, bpfd, part, tl, s, nParts);
+    case 3: return launchFused16MomM<3>(fv, thr, bpfd, part, tl, s, nParts);
+    case 4: return launchFused16MomM<4>(fv, thr, bpfd, part, tl, s, nParts);
+    case 5: return launchFused16MomM<5>(fv, thr, bpfd, part, tl, s, nParts);
+    case 6: return launchFused16MomM<6>(fv, thr, bpfd, part, tl, s, nParts);
+    default: return launchFused16MomM<7>(fv, thr, bpfd, part, tl, s, nParts);
+  }
 }
 #endif
 

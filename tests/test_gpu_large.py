@@ -54,6 +54,24 @@ def test_cell_count_beyond_32_bits_and_complement(lib, big):
     assert np.all(np.abs(ge.astype(np.float64) / cells - 1 / 256) < 1e-4)
 
 
+def check_moments(m, big, inside):
+    """The library's moments against torch's float64 sums of the same field,
+    slab by slab (the field is uniform in [0, 1), so no sum cancels)."""
+    want = np.zeros(6)
+    for z0 in range(0, Z, 100):
+        sl = big[z0:z0 + 100].double()
+        sel1 = inside(sl)
+        for ph, sel in ((1, sel1), (0, ~sel1)):
+            v = torch.where(sel, sl, torch.zeros((), dtype=torch.float64, device=sl.device))
+            want[2 * ph] += float(v.sum())
+            want[2 * ph + 1] += float((v * v).sum())
+            want[4 + ph] += int(sel.sum())
+            del v
+        del sl, sel1
+    assert m[4] == want[4] and m[5] == want[5]
+    np.testing.assert_allclose(m[:4], want[:4], rtol=1e-10)
+
+
 def test_dropin_pair_and_moments_match_fused(lib, big, cuda):
     print("large field: drop-in pair and moments", flush=True)
     ref = fused(lib, big, ">", 0.25)
@@ -71,6 +89,7 @@ def test_dropin_pair_and_moments_match_fused(lib, big, cuda):
     np.testing.assert_array_equal(h.cpu().numpy().view(np.uint64), ref)
     m = mom.cpu().numpy()
     assert m[4] + m[5] == X * Y * Z
+    check_moments(m, big, lambda v: v > 0.25)
     assert abs(m[5] / (X * Y * Z) - 0.75) < 1e-4  # P(u > 0.25)
     del words
 
@@ -104,6 +123,7 @@ def test_fused_moments_and_batch_on_the_large_field(lib, big, cuda):
     np.testing.assert_array_equal(h.cpu().numpy().view(np.uint64), ge)
     m = mom.cpu().numpy()
     assert m[4] + m[5] == X * Y * Z
+    check_moments(m, big, lambda v: v >= 0.5)
     small = torch.rand((40, 50, 60), generator=torch.Generator(device=cuda).manual_seed(3), device=cuda)
     # two levels: the per-level form; three of one family: the Gray-plane form
     ge25 = fused(lib, big, ">=", 0.25)

@@ -183,9 +183,10 @@ FUSED_CASES = [
     ((4, 5, 129), torch.float64, (3, 16, 4, 1)),
     ((6, 9, 1025), torch.float64, None),
     ((17, 19, 333), torch.float64, (4, 16, 8, 2)),
-    # voxelsPerLane 16 (the plain kernel's float default) with moments: the
-    # moments kernel takes such a tiling with 4 voxels per lane and 4 row
-    # pairs; ragged and exact rows, one to three segments.
+    # voxelsPerLane 16 or 8: the persistent kernel's moments form, always
+    # with 4 row pairs; ragged and exact rows, one to three segments, the
+    # last column read only as the halo of the last segment (X = 1025 and
+    # 2049 at 16 voxels per lane, 513 at 8).
     ((3, 4, 1025), torch.float32, (2, 16, 4, 16)),
     ((3, 4, 1024), torch.float32, (2, 16, 4, 16)),
     ((5, 6, 1030), torch.float32, (3, 16, 4, 16)),
@@ -193,6 +194,14 @@ FUSED_CASES = [
     ((7, 9, 300), torch.float32, (3, 16, 4, 16)),
     ((6, 9, 1000), torch.float32, None),
     ((9, 6, 2100), torch.float32, (4, 16, 2, 16)),
+    ((3, 4, 513), torch.float32, (2, 16, 4, 8)),
+    ((5, 6, 700), torch.float32, (3, 16, 4, 8)),
+    ((4, 7, 4), torch.float32, (2, 16, 4, 8)),
+    ((6, 5, 5), torch.float32, (2, 16, 4, 16)),
+    # more tiles (13 row stacks x 299 plane pairs) than resident workgroups:
+    # each workgroup sums the moments of several tiles
+    ((300, 50, 300), torch.float32, (1, 16, 4, 16)),
+    ((300, 50, 300), torch.float32, (1, 16, 4, 8)),
 ]
 
 
