@@ -34,17 +34,21 @@ def run(lib, fn, field, bm, tiling=None):
 
 
 def test_no_alloc_matches_allocating_entry_point(lib, orc, cuda):
-    cases = [((96, 130, 700), torch.float32, (">=", 0.1)),   # the larger first
-             ((30, 41, 257), torch.float64, ("<", -0.2)),
-             ((1, 50, 70), torch.float32, (">", 0.0))]      # no cells: moments from the map pass
-    for k, (shape, dt, lv) in enumerate(cases):
-        print(f"no-alloc case {k}: {shape} {dt} {lv}", flush=True)
+    # the larger first; the persistent kernel's moments form at 16 and 8
+    # voxels per lane (its partials are per resident workgroup, not per tile)
+    cases = [((96, 130, 700), torch.float32, (">=", 0.1), None),
+             ((30, 41, 257), torch.float64, ("<", -0.2), None),
+             ((40, 60, 1025), torch.float32, (">=", 0.0), (4, 16, 4, 16)),
+             ((20, 30, 600), torch.float32, ("<=", 0.3), (2, 16, 4, 8)),
+             ((1, 50, 70), torch.float32, (">", 0.0), None)]  # no cells: moments from the map pass
+    for k, (shape, dt, lv, tiling) in enumerate(cases):
+        print(f"no-alloc case {k}: {shape} {dt} {lv} {tiling}", flush=True)
         f = synth.grf_field(shape, 2.0, seed=90 + k, device=cuda, dtype=dt)
         bm = M.bin_map(*lv)
-        rc, h_ref, m_ref, _ = run(lib, lib.mkfFusedBPFDMoments, f, bm)
+        rc, h_ref, m_ref, _ = run(lib, lib.mkfFusedBPFDMoments, f, bm, tiling)
         M.check(rc, "mkfFusedBPFDMoments")
         for rep in range(2):
-            rc, h, m, dfree = run(lib, lib.mkfFusedBPFDMomentsNoAlloc, f, bm)
+            rc, h, m, dfree = run(lib, lib.mkfFusedBPFDMomentsNoAlloc, f, bm, tiling)
             M.check(rc, "mkfFusedBPFDMomentsNoAlloc")
             np.testing.assert_array_equal(h, h_ref, err_msg=f"case {k} call {rep}")
             assert np.array_equal(m.view(np.uint64), m_ref.view(np.uint64)), f"moments differ: case {k} call {rep}"
