@@ -144,8 +144,17 @@ def phase_moments(field, t, m):
     threshold bits: per phase the exactly rounded sum of the values (as
     double) and of their squares (each square rounded to double), and the
     voxel count. Layout of CUDAImgMom::addCat (src/binMapCUDA.cuh:117-121:
-    category c at 2c, its square at 2c + 1), with the counts appended."""
+    category c at 2c, its square at 2c + 1), with the counts appended. A
+    sum holding a NaN, or both infinities, is NaN in every order of
+    addition (math.fsum raises for inf + -inf); one that holds infinities of
+    one sign only is that infinity."""
     import math
+
+    def exact_sum(a):
+        if not np.all(np.isfinite(a)):
+            if np.isnan(a).any() or (np.isposinf(a).any() and np.isneginf(a).any()):
+                return math.nan
+        return math.fsum(a)
 
     f = np.asarray(field)
     b = threshold_bits(f, t, m).ravel()
@@ -153,7 +162,7 @@ def phase_moments(field, t, m):
     out = []
     for ph in (False, True):
         sel = v[b == ph]
-        out += [math.fsum(sel), math.fsum(sel * sel)]
+        out += [exact_sum(sel), exact_sum(sel * sel)]
     n1 = int(np.count_nonzero(b))
     return np.array(out + [float(b.size - n1), float(n1)])
 

@@ -221,6 +221,33 @@ def test_fused_moments_match_oracle(lib, orc, cuda, shape, dtype, tiling):
         _check(mom, f_np, relop, thr)
 
 
+ALL_RELOPS = ["<", "=", "<=", ">", "!=", ">=", "", "!"]  # masks 1 .. 6, 0 and 7
+
+
+@pytest.mark.parametrize("shape,tiling", [((5, 7, 1025), (3, 16, 4, 16)),   # last column as the halo
+                                          ((6, 5, 600), (2, 16, 4, 8)),      # partial 512-voxel segment
+                                          ((4, 9, 1030), (2, 16, 4, 16))])   # partial 1024-voxel segment
+def test_fused_moments_special_values_all_relations(lib, orc, cuda, shape, tiling):
+    """The persistent kernel's moments form on ±0, denormals, ±inf and NaN
+    under all 8 relations: each value's phase follows the map bit (NaN
+    compares as EQ), so the phase holding an inf or a NaN has that sum."""
+    print(f"fused moments, special values {shape} {tiling}", flush=True)
+    rng = np.random.default_rng(sum(shape))
+    choices = np.array([0.0, -0.0, 1e-40, -1e-40, 0.25, -0.25, 0.5, 1.0, 3.0, np.inf, -np.inf], dtype=np.float32)
+    vals = rng.choice(choices, size=shape).astype(np.float32)
+    Z, Y, X = shape
+    vals[1, 2, X - 1] = np.nan   # last column
+    vals[Z - 1, Y - 1, 3] = np.nan
+    f = torch.from_numpy(vals).to(cuda)
+    fi = M.FieldInfo(f.data_ptr(), (X, Y, Z), M.ENC_F32)
+    for relop in ALL_RELOPS:
+        h, mom = _fused_moments(lib, fi, X, Y, Z, relop, 0.5, cuda, tiling)
+        bm = M.bin_map(relop, 0.5)
+        np.testing.assert_array_equal(h, orc.bpfd_chunk(orc.pbm_c(vals, bm.t[0], bm.m), X), err_msg=relop)
+        _check(mom, vals, relop, 0.5)
+        assert mom[4] + mom[5] == X * Y * Z
+
+
 def test_fused_moments_summed_strided_and_cell_free(lib, orc, cuda):
     Z, Y, X = 9, 11, 200
     fs = [synth.grf_field((Z, Y, X), sigma=1.5, seed=50 + i, device=cuda, dtype=torch.float32) for i in range(2)]

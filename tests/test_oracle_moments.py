@@ -63,3 +63,19 @@ def test_nan_goes_to_eq_phase():
     m = O.phase_moments(f, bm.t[0], bm.m)
     assert m[0] == -4.0 and m[1] == 4.0 and math.isnan(m[2]) and math.isnan(m[3])
     assert m[4] == 4 and m[5] == 12
+
+
+def test_infinities_in_a_phase():
+    """IEEE addition in any order: infinities of one sign give that infinity,
+    both signs give NaN; squares of either are +inf."""
+    bm = M.bin_map(">=", 0.0)
+    one = np.array([[[np.inf, 1.0, 2.0, -5.0]]])
+    m = O.phase_moments(one, bm.t[0], bm.m)
+    assert m[0] == -5.0 and m[1] == 25.0 and m[2] == math.inf and m[3] == math.inf
+    ne = M.bin_map("!=", 0.0)
+    both = np.array([[[np.inf, 1.0, -np.inf, 0.0]]])
+    m = O.phase_moments(both, ne.t[0], ne.m)
+    assert m[0] == 0.0 and m[1] == 0.0 and m[4] == 1
+    assert math.isnan(m[2]) and m[3] == math.inf and m[5] == 3   # inf, 1 and -inf in phase 1
+    with np.errstate(invalid="ignore"):
+        assert np.isnan(np.float64(np.inf) + np.float64(-np.inf))
