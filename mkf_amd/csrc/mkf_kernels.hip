@@ -539,8 +539,10 @@ constexpr int kFused16PerCU = MKF_FUSED16_PER_CU;
 // likewise), and, when the segment ends one voxel short of the row's end,
 // that last voxel (read as the halo). Every workgroup writes its moments to
 // part[blockIdx.x] after its last tile.
+// The moments form asks for 3 waves per SIMD (at most 168 VGPRs); the plain
+// form's code is unchanged by the attribute (1: no bound).
 template <int M, int TY, int NG, bool MOM = false>
-__global__ __launch_bounds__(kWave) void kFused16(const float* __restrict__ f, float thr, int X, int Y, int Z,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(MOM ? 3 : 1))) void kFused16(const float* __restrict__ f, float thr, int X, int Y, int Z,
                                                   long long sy, long long sz, TileGrid g,
                                                   unsigned long long* __restrict__ bpfd,
                                                   double* __restrict__ part = nullptr) {
@@ -1164,15 +1166,21 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
 //        1.173 + 1.245 ms
 // Float fields take the persistent kernel's moments form where each of its
 // waves walks at least 48 plane pairs: 16 voxels per lane for rows of whole
-// 1024-voxel segments, with the cost model's slab depth, else 8 voxels per
-// lane with 32 plane pairs. GRF fields, HIP events, warm clocks:
-//   1024^3         16: 0.814 ms (64 plane pairs; 128: 0.817)  8: 0.880  4: 0.861
-//   1000^3          8: 0.802 (16: 0.934)                       4: 0.846
-//   1536x1000x700   8: 0.791 (16 plane pairs: 0.803)           4: 1.240
-//   1030x1000x1000  8: 0.983                                   4: 1.662
-//   300x1000x3000   8: 1.020 (64: 1.109)                       4: 1.227
-//   2100x1000x500   8: 0.874 (16: 1.181)                       4: 1.562
-//   1030x722x722    8: 0.541                                   4: 0.890
+// 1024-voxel segments, else 8 voxels per lane with 32 plane pairs. The
+// 16-voxel form takes 64 plane pairs, halved while the grid holds fewer
+// tiles than resident workgroups; the depth matters more than the busiest-
+// CU model says (1024^3: 48 0.858 ms, 56 0.795, 64 0.714, 72 0.787, 86 (one
+// tile per slot) 0.752; 2048x1024x512: 32 0.709, 64 0.719, 86 0.757;
+// 1025x1000x1000: 64 0.724, 86 0.752; 1024x700x700: 32 0.374, 64 0.424).
+// GRF fields, HIP events, warm clocks (the 16-voxel column with 3 waves per
+// SIMD; the others from the first, 2-wave build of the kernel):
+//   1024^3         16: 0.714 ms (64 plane pairs)   8: 0.880  4: 0.861
+//   1000^3          8: 0.802 (16: 0.934)           4: 0.846
+//   1536x1000x700   8: 0.791 (16 plane pairs: 0.803) 4: 1.240
+//   1030x1000x1000  8: 0.983                       4: 1.662
+//   300x1000x3000   8: 1.020 (64: 1.109)           4: 1.227
+//   2100x1000x500   8: 0.874 (16: 1.181)           4: 1.562
+//   1030x722x722    8: 0.541                       4: 0.890
 // Smaller fields walk too few planes per wave to fill the pipeline:
 //   1000x366x366    8: 0.184  4: 0.151;  700x500x500  8: 0.227  4: 0.213;
 //   2000x200x300    8: 0.159  4: 0.141;  256^3        8: 0.091  4: 0.070.
@@ -1181,10 +1189,12 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
 // 1000 x 733 x 733 0.478 (64: 0.507); 300 x 669 x 669 0.256 (8, the earlier
 // choice: 0.418); at 16: 1000 x 366 x 366 0.153 (32: 0.166); 700 x 500 x 500
 // 0.216 (0.218); 2000 x 200 x 300 0.144 (0.142).
+static long long momTiles(int X, int Y, int Z, const Tiling& tl) {
+  return ceilDiv(X - 1, (long long)kWave * tl.V) * ceilDiv(Y - 1, tl.Ty) * ceilDiv(Z - 1, tl.TP);
+}
+
 static long long momWalk(int X, int Y, int Z, const Tiling& tl) {
-  const long long tiles =
-      ceilDiv(X - 1, (long long)kWave * tl.V) * ceilDiv(Y - 1, tl.Ty) * ceilDiv(Z - 1, tl.TP);
-  return ceilDiv(tiles, (long long)fused16MomSlots(tl.V)) * tl.TP;
+  return ceilDiv(momTiles(X, Y, Z, tl), (long long)fused16MomSlots(tl.V)) * tl.TP;
 }
 
 Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
@@ -1197,7 +1207,8 @@ Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
     Tiling tp = tl;
     if (X >= kWave * 16 && X % (kWave * 16) <= 1) {
       tp.V = 16;
-      tp.TP = persistentDepth(X, Y, Z, 4, kWave * 16, fused16MomSlots(16));
+      tp.TP = 64;
+      while (tp.TP > 16 && momTiles(X, Y, Z, tp) < (long long)fused16MomSlots(16)) tp.TP >>= 1;
       if (momWalk(X, Y, Z, tp) >= 48) return tp;
     }
     tp.V = 8;
@@ -1630,10 +1641,13 @@ hipError_t launchFusedMoments(const FieldView& fv, const BinMapF64& m, unsigned 
 
 #if MKF_PART(8)
 // The moments form runs up to 16 waves per CU (4 per SIMD), as many as its
-// registers allow: the 16-voxel kernel fits 2 per SIMD, the 8-voxel one 4.
-// Measured on MI355X (f32 GRF, 8-voxel kernel): 1536 x 1000 x 700 with 32
-// plane pairs 0.807 ms at 16 per CU against 0.885 at 8; 1024^3 0.852 against
-// 0.858.
+// registers allow: the 16-voxel kernel is held to 168 VGPRs (3 per SIMD;
+// unbounded it took 181 and fitted 2), the 8-voxel one takes 113 (4).
+// Measured on MI355X (f32 GRF): 8 voxels per lane, 1536 x 1000 x 700 with 32
+// plane pairs 0.807 ms at 16 per CU against 0.885 at 8; 16 voxels per lane,
+// 1024^3 with 64 plane pairs 0.717 ms at 3 per SIMD against 0.811 at 2
+// (2048 x 1024 x 512: 0.726 against 0.815). At 4 per SIMD (128 VGPRs) the
+// 16-voxel kernel spills 75 VGPRs, and the 8-voxel one at 5 spills 18.
 constexpr int kFused16MomPerCU = 16;
 
 unsigned fused16MomSlots(int V) {
