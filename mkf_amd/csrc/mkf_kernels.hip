@@ -377,8 +377,9 @@ __device__ __forceinline__ uint32_t cellKeep(int x0, int X, uint32_t& cellsPerRo
 
 // With MOM the kernel also writes the workgroup's phase moments of the
 // voxels it owns (see walkStack) to part[blockIdx.x]; blocks past the last
-// tile write zeros.
-template <typename T, int M, int V, int TY, bool G, bool MOM = false>
+// tile write zeros. PW compiles the walk for partial waves into the plain
+// 2-voxel double kernel too (for rows whose last segment is partial).
+template <typename T, int M, int V, int TY, bool G, bool MOM = false, bool PW = false>
 __global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr, int X, int Y, int Z, TileGrid g,
                                                     unsigned long long* __restrict__ bpfd,
                                                     double* __restrict__ part = nullptr) {
@@ -399,10 +400,10 @@ __global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr,
   uint32_t cellsPerRow;
   const uint32_t keep = cellKeep<V>(q.x0, X, cellsPerRow);
   // The walk, compiled for whole waves (and the general form) and, for the
-  // moments forms and 4 voxels per lane, once more for partial waves; each
-  // wave runs one, and both pass the same barriers. (The plain 2-voxel double
-  // kernel keeps one walk: the second copy cost it 4 % at 1024^3, and double
-  // rows with a partial segment take 1 voxel per lane instead.)
+  // moments forms, 4 voxels per lane and PW, once more for partial waves;
+  // each wave runs one, and both pass the same barriers. (The plain 2-voxel
+  // double kernel for rows of whole segments keeps one walk: the second copy
+  // cost it 4 % at 1024^3.)
   auto run = [&](auto partTag) {
     constexpr bool PART = decltype(partTag)::value;
     FieldRows<T, M, V, G, PART> src;
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr,
       flushHistogram(hist, &cnt, tally, cellsPerRow * q.nY * q.nZ, bpfd);
     }
   };
-  if constexpr (!G && V > 1 && (MOM || V == 4)) {
+  if constexpr (!G && V > 1 && (MOM || V == 4 || PW)) {
     if (vec)
       run(std::false_type{});
     else
@@ -1110,10 +1111,14 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
     }
   }
   if (elemBytes >= 8) {
-    // Rows of whole 128-voxel segments: 2 voxels per lane (one 16-byte load
-    // per row); otherwise 1 (1000^3 f64: 1.26 ms against 2.13 for the 2-voxel
-    // kernel's voxel-by-voxel partial waves).
-    tl.V = (X % (kWave * 2) <= 1) ? 2 : 1;
+    // 2 voxels per lane (one 16-byte load per row). Rows whose last
+    // 128-voxel segment is partial run the kernel form with a second walk for
+    // the partial waves; against 1 voxel per lane (GRF, HIP events, best of 3
+    // means of 10): 1000^3 1.152 ms (512 plane pairs) against 1.351;
+    // 1030 x 1000 x 1000 1.282 against 1.907; 700 x 1000 x 1000 0.863 (256)
+    // against 0.881; 300 x 1000 x 3000 1.246 (256) against 1.319. (Before
+    // that form, such waves loaded voxel by voxel: 1000^3 2.13 ms.)
+    tl.V = X >= 2 ? 2 : 1;
     tl.Ty = 4;
     tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * tl.V));
     tl.TP = balancedDepth(X, Y, Z, tl, 512);
@@ -1385,6 +1390,11 @@ static hipError_t launchFusedT(const FieldView& fv, T thr, unsigned long long* b
                          fv.Z, g, bpfd);
     else
       return hipErrorInvalidValue;
+  } else if (V == 2 && fv.X % (kWave * V) > 1) {
+    // Rows whose last segment is partial: the form with a second walk for
+    // the partial waves (PW).
+    hipLaunchKernelGGL((kFusedBPFD<T, M, V, TY, false, false, V == 2>), grid, block, 0, s, fieldArgs<T>(fv), thr,
+                       fv.X, fv.Y, fv.Z, g, bpfd);
   } else {
     hipLaunchKernelGGL((kFusedBPFD<T, M, V, TY, false>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X, fv.Y,
                        fv.Z, g, bpfd);

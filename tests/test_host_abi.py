@@ -261,7 +261,8 @@ def test_volFrac(lib):
     ((350, 20, 20), 4, (1, 4, 16)),
     ((200, 1000, 5000), 4, (1, 4, 16)),
     ((1024, 1024, 1024), 8, (2, 4, 16)),     # doubles: 2 voxels per lane
-    ((1000, 1000, 1000), 8, (1, 4, 16)),     # doubles with a partial 128-voxel segment: 1 voxel per lane
+    ((1000, 1000, 1000), 8, (2, 4, 16)),     # doubles with a partial 128-voxel segment: 2, with the partial walk
+    ((1030, 1000, 1000), 8, (2, 4, 16)),
 ])
 def test_default_tiling_choice(lib, dims, eb, want):
     """The built-in fused tiling (host logic only; the slab depth depends on
