@@ -1069,6 +1069,7 @@ Tiling choose16Tiling(int X, int Y, int Z, int V, int quarters) {
 }
 
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
+  X = std::max(X, 2);  // a row without cells takes the tiling of one cell
   Tiling tl;
   long long targetWaves;
   // Float fields (HIP events, x 1000 x 1000 unless given, best of 3 rounds
@@ -1113,15 +1114,24 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   if (elemBytes >= 8) {
     // 2 voxels per lane (one 16-byte load per row). Rows whose last
     // 128-voxel segment is partial run the kernel form with a second walk for
-    // the partial waves; against 1 voxel per lane (GRF, HIP events, best of 3
-    // means of 10): 1000^3 1.152 ms (512 plane pairs) against 1.351;
-    // 1030 x 1000 x 1000 1.282 against 1.907; 700 x 1000 x 1000 0.863 (256)
-    // against 0.881; 300 x 1000 x 3000 1.246 (256) against 1.319. (Before
-    // that form, such waves loaded voxel by voxel: 1000^3 2.13 ms.)
-    tl.V = X >= 2 ? 2 : 1;
+    // the partial waves where they span more than 3 segments and the slabs
+    // are at least 32 plane pairs deep; against 1 voxel per lane (GRF, HIP
+    // events): 1000^3 1.152-1.175 ms (256-512 plane pairs) against 1.347;
+    // 1030 x 1000 x 1000 1.282 against 1.907; 600 x 400 x 400 0.134 (32)
+    // against 0.147; 400^3 0.097 (32) against 0.100. Otherwise 1 voxel per
+    // lane: 250 x 300 x 350 0.091 ms (2 voxels, 8 plane pairs) against 0.061;
+    // 386 x 300 x 300 0.068 (16) against 0.056; 450 x 300 x 300 0.072 (16)
+    // against 0.063. (Before the second walk, partial waves loaded voxel by
+    // voxel: 1000^3 2.13 ms.)
+    tl.V = 2;
     tl.Ty = 4;
     tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * tl.V));
     tl.TP = balancedDepth(X, Y, Z, tl, 512);
+    if (X % (kWave * 2) > 1 && (X - 1 <= 3 * kWave * 2 || tl.TP < 32)) {
+      tl.V = 1;
+      tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave));
+      tl.TP = balancedDepth(X, Y, Z, tl, 512);
+    }
     return tl;
   } else if (X - 1 > kWave * 8) {
     tl.V = 4;
@@ -1207,6 +1217,7 @@ static long long momWalk(int X, int Y, int Z, const Tiling& tl) {
 }
 
 Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
+  X = std::max(X, 2);  // a row without cells takes the tiling of one cell
   Tiling tl;
   tl.V = elemBytes >= 8 ? 2 : 4;
   tl.Ty = 4;
@@ -1243,6 +1254,7 @@ Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
 
 // Histogram of an existing PBM: kBPFD4 with 4 row pairs.
 Tiling chooseBpfdTiling(int X, int Y, int Z) {
+  X = std::max(X, 2);  // a row without cells takes the tiling of one cell
   // 4 voxels per lane, 32 plane pairs per workgroup, or 16 where that gives
   // fewer than 4 workgroups per CU. Measured on MI355X on GRF maps (HIP
   // events, best of 3 rounds of 20 launches), against the 16-voxel kernel

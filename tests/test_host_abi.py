@@ -275,6 +275,16 @@ def test_default_tiling_choice(lib, dims, eb, want):
     assert 1 <= t.planePairs <= 65536
 
 
+@pytest.mark.parametrize("dims", [(1, 5, 5), (0, 5, 5), (1, 1, 1), (5, 0, 5), (5, 5, -7), (1 << 30, 2, 2)])
+@pytest.mark.parametrize("eb", [4, 8])
+def test_default_tiling_of_degenerate_dims(lib, dims, eb):
+    """Rows without cells, empty or negative dims and very long rows still
+    give a supported tiling (no division by a zero segment count)."""
+    t = lib.mkfDefaultTiling(M.dims(dims), eb)
+    assert t.voxelsPerLane in ((1, 4, 8, 16) if eb == 4 else (1, 2))
+    assert t.rowPairs in (2, 4, 8) and 1 <= t.waves <= 16 and 1 <= t.planePairs <= 65536
+
+
 def test_fault_points_validate_their_arguments(lib):
     """The error-path test hook accepts only known points (no GPU work)."""
     assert lib.mkfDebugInjectFault(b"no.such.point", 1) == M.MKF_ERR_ARG
