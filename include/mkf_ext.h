@@ -134,7 +134,8 @@ int mkfBPFDAsync(size_t *pBPFD, const BMOrg *pO, const BMPackWord *pW, const Mkf
 /* Tiling the library chooses for the fused kernel on a field of dims def
  * and element size (the histogram of a PBM has its own default). The choice
  * depends on the current device: its CU count and how many workgroups of the
- * persistent kernels it holds at once. */
+ * persistent kernels it holds at once. def NULL gives the all-zero tiling,
+ * which every entry point refuses, and sets mkfLastError. */
 MkfTiling mkfDefaultTiling(const FieldDef def[3], int elemBytes);
 
 /* Geometry of a histogram launch: its tiles, the workgroups launched, the

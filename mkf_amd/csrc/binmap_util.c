@@ -50,9 +50,13 @@ int encAlignment(const NumEnc e) {
 
 /* Relation string + float threshold -> {t, m}. Characters are consumed up to
  * the first one that is not a relation symbol; '!' is accepted only as the
- * first character and inverts the finished mask.  src/binMapUtil.c:37-58. */
+ * first character and inverts the finished mask.  src/binMapUtil.c:37-58.
+ * (The reference dereferences both pointers unchecked: here a NULL map
+ * returns NULL and a NULL string reads as "", the empty relation.) */
 BinMapF64 *setBinMapF64(BinMapF64 *pM, const char relopChars[], const float t) {
   int inv = 0;
+  if (!pM) return NULL;
+  if (!relopChars) relopChars = "";
   pM->t[0] = t;
   pM->m = BMC_NV;
   for (int i = 0;; ++i) {
@@ -78,6 +82,10 @@ BinMapF64 *setBinMapF64(BinMapF64 *pM, const char relopChars[], const float t) {
  * otherwise the buffer holds two rows by two planes (the reference's partial
  * "sliding" buffer).  src/binMapUtil.c:60-87. */
 size_t setBMO(BMOrg *pO, const FieldDef def[3], const char profID) {
+  if (!def) {  /* the reference reads def unchecked; here: an empty organisation */
+    if (pO) memset(pO, 0, sizeof(*pO));
+    return 0;
+  }
   unsigned maxRow = def[1] < 2 ? (unsigned)def[1] : 2u;
   unsigned maxPlane = def[2] < 2 ? (unsigned)def[2] : 2u;
   const BMStride rowStride = (BMStride)(((unsigned)def[0] + 31u) >> 5);

@@ -83,6 +83,7 @@ int mkfKvWrapCount(const size_t aBPFD[MKF_BINS]) {
 /* src/mkfUtil.c:125-135: odd bins (lowest corner set) over all bins. */
 float volFrac(const size_t aBPFD[MKF_BINS]) {
   size_t s0 = 0, s1 = 0;
+  if (!aBPFD) return -1;  /* no histogram (the reference reads it unchecked) */
   for (int i = 0; i < MKF_BINS; i += 2) {
     s0 += aBPFD[i];
     s1 += aBPFD[i + 1];
@@ -94,12 +95,14 @@ float volFrac(const size_t aBPFD[MKF_BINS]) {
 void mkfRefMeasureBPFD(float m[4], const size_t aBPFD[MKF_BINS], const float s) {
   (void)aBPFD;
   (void)s;
+  if (!m) return;
   m[0] = m[1] = m[2] = m[3] = -1;
 }
 
 /* src/mkfUtil.c:94-123 as compiled with NO_REF_MEASURES (profID unused). */
 int mkfSelectMeasuresBPFD(float m[4], char symCh[], const size_t aBPFD[MKF_BINS], const float s, const int profID) {
   (void)profID;
+  if (!m || !aBPFD) return 0;  /* no measures (the reference reads both unchecked) */
   size_t g[CELL8_SYMM_GROUPS] = {0};
   const size_t t = sumBins(aBPFD, MKF_BINS);
   for (int i = 0; i < MKF_BINS; i++) g[kPatternGroup[i]] += aBPFD[i];
@@ -112,7 +115,7 @@ int mkfSelectMeasuresBPFD(float m[4], char symCh[], const size_t aBPFD[MKF_BINS]
     const float rkV = (float)(1.0 / (double)denom);
     m[0] *= rkV;
   }
-  memcpy(symCh, "Kv       Vv", 12);
+  if (symCh) memcpy(symCh, "Kv       Vv", 12);
   if (getenv("MKF_VERBOSE")) {
     const int w = mkfKvWrapCount(aBPFD);
     if (w > 0) fprintf(stderr, "[mkf] mkfSelectMeasuresBPFD: %d Kv terms wrap in 32-bit arithmetic, as in the reference\n", w);

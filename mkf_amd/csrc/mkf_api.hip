@@ -228,6 +228,10 @@ int mkfDebugInjectFault(const char* point, int nth) {
 }
 
 MkfTiling mkfDefaultTiling(const FieldDef def[3], int elemBytes) {
+  if (!def) {
+    (void)fail(MKF_ERR_ARG, "NULL dims");
+    return MkfTiling{0, 0, 0, 0};
+  }
   const Tiling tl = chooseTiling(def[0], def[1], def[2], elemBytes);
   return MkfTiling{tl.TP, tl.ZW, tl.Ty, tl.V};
 }

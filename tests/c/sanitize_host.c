@@ -50,6 +50,23 @@ static void helpers(void) {
     CHECK(setBinMapF64(&bm, rel[i], 0.25f) == &bm);
     CHECK(bm.m <= 7 && bm.t[0] == 0.25);
   }
+  {  // NULL arguments are refused instead of dereferenced
+    BinMapF64 bm;
+    CHECK(setBinMapF64(NULL, ">=", 0.5f) == NULL);
+    CHECK(setBinMapF64(&bm, NULL, 0.5f) == &bm && bm.m == 0);
+    BMOrg o;
+    memset(&o, 0xFF, sizeof(o));
+    CHECK(setBMO(&o, NULL, 0) == 0 && o.rowWS == 0 && o.planeWS == 0);
+    float m[4];
+    char sym[16];
+    size_t h[256] = {0};
+    CHECK(mkfSelectMeasuresBPFD(NULL, sym, h, 1.0f, 2) == 0);
+    CHECK(mkfSelectMeasuresBPFD(m, sym, NULL, 1.0f, 2) == 0);
+    h[0] = 8;
+    CHECK(mkfSelectMeasuresBPFD(m, NULL, h, 1.0f, 2) == 2);
+    CHECK(volFrac(NULL) == -1.0f);
+    mkfRefMeasureBPFD(NULL, h, 1.0f);
+  }
   const FieldDef defs[][3] = {{1, 1, 1}, {33, 7, 5}, {64, 1, 9}, {1, 256, 256}, {1025, 3, 2}, {2, 2, 2}};
   for (size_t i = 0; i < sizeof(defs) / sizeof(defs[0]); i++) {
     BMOrg o;

@@ -285,6 +285,23 @@ def test_default_tiling_of_degenerate_dims(lib, dims, eb):
     assert t.rowPairs in (2, 4, 8) and 1 <= t.waves <= 16 and 1 <= t.planePairs <= 65536
 
 
+def test_null_arguments_do_not_crash(lib):
+    """Entry points that read their pointers unchecked in the reference
+    (setBinMapF64, setBMO, mkfSelectMeasuresBPFD, volFrac) and the tiling
+    query refuse NULL instead (no GPU work)."""
+    assert not lib.setBinMapF64(None, b">=", 0.5)
+    bm = M.BinMapF64()
+    lib.setBinMapF64(C.byref(bm), None, 0.5)
+    assert bm.m == 0 and bm.t[0] == 0.5
+    org = M.BMOrg(7, 7, 7, 7, 7)
+    assert lib.setBMO(C.byref(org), None, b"\x00") == 0 and org.rowWS == 0 and org.rowPairs == 0
+    assert lib.mkfSelectMeasuresBPFD(None, None, None, 1.0, 0) == 0
+    assert lib.volFrac(None) == -1.0
+    lib.mkfRefMeasureBPFD(None, None, 1.0)
+    t = lib.mkfDefaultTiling(None, 4)
+    assert (t.planePairs, t.waves, t.rowPairs, t.voxelsPerLane) == (0, 0, 0, 0) and "NULL dims" in M.last_error()
+
+
 def test_fault_points_validate_their_arguments(lib):
     """The error-path test hook accepts only known points (no GPU work)."""
     assert lib.mkfDebugInjectFault(b"no.such.point", 1) == M.MKF_ERR_ARG
