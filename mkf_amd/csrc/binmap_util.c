@@ -108,6 +108,7 @@ size_t setBMO(BMOrg *pO, const FieldDef def[3], const char profID) {
  * dim exceeds 1) and returns how many were kept.  src/binMapUtil.c:89-103. */
 int copyNonZeroDef(FieldDef d[], const FieldDef *pD, const int nD) {
   int rD = 0;
+  if (!d) return 0;  /* (NULL output, read unchecked by the reference) */
   if (pD) {
     for (int i = 0; i < nD; i++) {
       d[rD] = pD[i];
@@ -126,6 +127,7 @@ int copyNonZeroDef(FieldDef d[], const FieldDef *pD, const int nD) {
  * table. */
 int copyValidPtrByMask(ConstFieldPtr r[], const int max, const ConstFieldPtr a[], const uint32_t mask) {
   int n = 0;
+  if (!r || !a) return 0;  /* (NULL tables, read unchecked by the reference) */
   for (int i = 0; i < 32 && n < max && (mask >> i) != 0; i++)
     if (((mask >> i) & 1u) && a[i].p != NULL) r[n++] = a[i];
   return n;
@@ -135,6 +137,7 @@ int copyValidPtrByMask(ConstFieldPtr r[], const int max, const ConstFieldPtr a[]
  * entries.  src/binMapUtil.c:123-133. */
 int countValidPtrByMask(const ConstFieldPtr a[], uint32_t mask) {
   int n = 0;
+  if (!a) return 0;
   for (int i = 0; i < 32 && (mask >> i) != 0; i++) n += ((mask >> i) & 1u) && a[i].p != NULL;
   return n;
 }
@@ -172,6 +175,7 @@ int copyOrGenStride(FieldStride fs[], const int n, const int start, const FieldD
  * src/binMapUtil.c:158-164. */
 ConstFieldPtr *asFieldTab(const void **pp, const NumEnc id) {
   ConstFieldPtr *pFP = (ConstFieldPtr *)pp;
+  if (!pFP) return NULL;
   const int a = encAlignment(id);
   if (0 != (pFP->w & (size_t)(a - 1))) fprintf(stderr, "[mkf] alignment %d %p\n", a, pFP->p);
   return pFP;
@@ -205,6 +209,7 @@ const BMFieldInfo *setupFields(BMFieldInfo *pI, void **ppF, const int n, const i
 /* Field pointer table for a Context: consecutive fields of one allocation
  * when there are several, else the address of pDF.  src/binMapUtil.c:188-205. */
 void **autoFieldPtr(ConstFieldPtr ptr[BMCU_FIELD_MAX], const Context *pC) {
+  if (!ptr || !pC) return NULL;  /* (read unchecked by the reference) */
   if (pC->nField > 1) {
     const FieldStride stride = (FieldStride)pC->nElem * pC->bytesElem;
     size_t w = (size_t)pC->pDF;

@@ -766,6 +766,7 @@ void mkfCUDACleanup(int lvl) {
 // host field -> device, PBM (optionally copied back), histogram into pDZ
 // (copied back to pHZ when given). Returns 1, or 0 when the PBM failed.
 int mkfCUDAGetBPFDautoCtx(Context* pC, const FieldDef def[3], const BinMapF64* pM, const int profile) {
+  if (!pC || !def || !pM) return fail(MKF_ERR_ARG, "NULL context, dims or map"), 0;
   if (pC->pHF) {
     if (NULL == pC->pDF && hipMalloc(&(pC->pDF), pC->bytesF) != hipSuccess) pC->pDF = NULL;
     if (pC->pDF) hipMemcpy(pC->pDF, pC->pHF, pC->bytesF, hipMemcpyHostToDevice);

@@ -66,6 +66,11 @@ static void helpers(void) {
     CHECK(mkfSelectMeasuresBPFD(m, NULL, h, 1.0f, 2) == 2);
     CHECK(volFrac(NULL) == -1.0f);
     mkfRefMeasureBPFD(NULL, h, 1.0f);
+    CHECK(copyNonZeroDef(NULL, NULL, 3) == 0);
+    CHECK(copyValidPtrByMask(NULL, 4, NULL, 0xFu) == 0);
+    CHECK(countValidPtrByMask(NULL, 0xFu) == 0);
+    CHECK(asFieldTab(NULL, ENC_F32) == NULL);
+    CHECK(autoFieldPtr(NULL, NULL) == NULL);
   }
   const FieldDef defs[][3] = {{1, 1, 1}, {33, 7, 5}, {64, 1, 9}, {1, 256, 256}, {1025, 3, 2}, {2, 2, 2}};
   for (size_t i = 0; i < sizeof(defs) / sizeof(defs[0]); i++) {

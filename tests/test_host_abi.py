@@ -287,8 +287,8 @@ def test_default_tiling_of_degenerate_dims(lib, dims, eb):
 
 def test_null_arguments_do_not_crash(lib):
     """Entry points that read their pointers unchecked in the reference
-    (setBinMapF64, setBMO, mkfSelectMeasuresBPFD, volFrac) and the tiling
-    query refuse NULL instead (no GPU work)."""
+    (setBinMapF64, setBMO, the measures, the field-table helpers and the
+    context wrapper) and the tiling query refuse NULL instead (no GPU work)."""
     assert not lib.setBinMapF64(None, b">=", 0.5)
     bm = M.BinMapF64()
     lib.setBinMapF64(C.byref(bm), None, 0.5)
@@ -300,6 +300,12 @@ def test_null_arguments_do_not_crash(lib):
     lib.mkfRefMeasureBPFD(None, None, 1.0)
     t = lib.mkfDefaultTiling(None, 4)
     assert (t.planePairs, t.waves, t.rowPairs, t.voxelsPerLane) == (0, 0, 0, 0) and "NULL dims" in M.last_error()
+    assert lib.copyNonZeroDef(None, None, 3) == 0
+    assert lib.copyValidPtrByMask(None, 4, None, 0xF) == 0
+    assert lib.countValidPtrByMask(None, 0xF) == 0
+    assert not lib.asFieldTab(None, M.ENC_F32)
+    assert not lib.autoFieldPtr(None, None)
+    assert lib.mkfCUDAGetBPFDautoCtx(None, None, None, 0) == 0 and "NULL context" in M.last_error()
 
 
 def test_fault_points_validate_their_arguments(lib):
