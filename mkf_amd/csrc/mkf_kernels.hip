@@ -1074,11 +1074,14 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   long long targetWaves;
   // Float fields (HIP events, x 1000 x 1000 unless given, best of 3 rounds
   // of 10-20 launches on MI355X):
-  // * rows that fill their 1024-voxel segments at least 80 %: the persistent
-  //   16-voxel kernel. X = 1024 (x 1024 x 1024): 0.669 ms against 0.714 for 4
-  //   voxels per lane; 1000: 0.686 against 0.714; 900: 0.591 against 0.633;
-  //   1900 x 1000 x 560: 0.722 against 0.888. At 800 (78 %) 4 voxels per lane
-  //   are as fast (0.547 against 0.555).
+  // * rows that fill their 1024-voxel segments at least 80 %, or their one
+  //   segment at least 68 %: the persistent 16-voxel kernel. X = 1024
+  //   (x 1024 x 1024): 0.669 ms against 0.714 for 4 voxels per lane; 1000:
+  //   0.686 against 0.714; 900: 0.591 against 0.633; 1900 x 1000 x 560: 0.722
+  //   against 0.888. One segment (GRF, HIP events, best of 3 means of 10):
+  //   800: 0.500 against 0.553; 769: 0.495 against 0.575; 700: 0.488 against
+  //   0.505; 650: 0.487 against 0.472; 600: 0.486 against 0.444 (the
+  //   16-voxel kernel's time barely depends on the fill).
   // * rows that fill their 512-voxel segments at least 90 %: the 8-voxel
   //   kernel. 1536 x 1000 x 700: 0.695 against 0.762 (16) and 0.759 (4).
   // * rows of 257 to 513 voxels: the 8-voxel kernel. 280 x 1000 x 3600:
@@ -1102,7 +1105,7 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   //   (4) and 1.028 (8); 256 x 1000 x 4000: 0.721 against 0.841;
   //   300 x 669 x 669: 0.133 against 0.149 (4).
   if (elemBytes == 4) {
-    if (segmentsFilled(X, 0.8)) {
+    if (segmentsFilled(X, 0.8) || (X - 1 <= kWave * 16 && segmentsFilled(X, 0.68))) {
       const Tiling t16 = choose16Tiling(X, Y, Z, 16);
       if (persistentWalk(X, Y, Z, t16) >= 48) return t16;
     }

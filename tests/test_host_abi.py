@@ -252,7 +252,9 @@ def test_volFrac(lib):
     ((900, 20, 20), 4, (4, 4, 16)),          # a small field: waves would walk too few planes
     ((1536, 20, 20), 4, (4, 4, 16)),
     ((300, 20, 20), 4, (1, 4, 16)),          # rows of 257..513 voxels on a small field: 1 voxel per lane
-    ((769, 1000, 1000), 4, (4, 4, 16)),      # 75 %: 4 voxels per lane, 3 segments in one workgroup
+    ((769, 1000, 1000), 4, (16, 4, None)),   # one segment filled 75 %: 16 voxels per lane
+    ((650, 1000, 1000), 4, (4, 4, 16)),      # one segment filled 63 %: 4 voxels per lane
+    ((1600, 1000, 1000), 4, (4, 4, 16)),     # two segments filled 78 %: 4 voxels per lane
     ((1030, 1000, 1000), 4, (4, 4, 16)),     # 50 % of 1024, 67 % of 512
     ((1200, 100, 100), 4, (4, 4, 16)),       # 59 % of 1024, 78 % of 512
     ((257, 20, 20), 4, (1, 4, 16)),          # rows of at most 257 voxels: 1 voxel per lane
