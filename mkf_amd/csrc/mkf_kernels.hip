@@ -1094,7 +1094,9 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   //   33 plane pairs), at 2^29 the persistent kernels (1000 x 733 x 733: 0.363
   //   against 0.402; 1536 x 591 x 591: 0.357 against 0.419; 67 and 132).
   // * other rows longer than 513 voxels: 4 voxels per lane, slab depth from
-  //   balancedDepth. 1030: 0.736 against 1.008 (1 voxel), 1.105 (16) and 0.792 (8);
+  //   balancedDepth (with the per-word tally of the field walk: 1030 0.686 ms,
+  //   2100 x 1000 x 500 0.650). 1030: 0.736 against 1.008 (1 voxel), 1.105
+  //   (16) and 0.792 (8);
   //   1240: 0.854 against 1.097 (16) and 0.871 (8); 640: 0.454 against 0.549
   //   (16); 576: 0.420 against 0.510 (1); 2100 x 1000 x 500: 0.694 against
   //   0.818 (16); 700 x 500 x 500: 0.124 against 0.167 (16); 2000 x 200 x 300:
