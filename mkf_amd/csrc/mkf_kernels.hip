@@ -1185,8 +1185,10 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
 //   f64: 2 voxels/lane, 128 plane pairs 1.595 ms (512: 1.609); two passes
 //        1.173 + 1.245 ms
 // Float fields take the persistent kernel's moments form where each of its
-// waves walks at least 48 plane pairs: 16 voxels per lane for rows that fill
-// their 1024-voxel segments to 95 %, else 8 voxels per lane with 32 plane
+// waves walks at least 48 plane pairs: 16 voxels per lane for rows of whole
+// 1024-voxel segments (a partial one costs the 16-voxel form more: 1000 x 733
+// x 733 0.501 ms against 0.448 at 8, 1000 x 600 x 600 0.348 against 0.298,
+// 1000^3 0.795 against 0.804), else 8 voxels per lane with 32 plane
 // pairs for rows that fill their 512-voxel segments at least half (narrower
 // rows idle too many lanes: X = 64, 100, 200 took 4.55, 2.60, 1.44 ms at 8
 // against 3.74, 1.99, 1.24 at 4; X = 300 1.02 against 1.23). The 16-voxel
@@ -1198,7 +1200,7 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
 // GRF fields, HIP events, warm clocks (the 16-voxel column with 3 waves per
 // SIMD; the others from the first, 2-wave build of the kernel):
 //   1024^3         16: 0.714 ms (64 plane pairs)   8: 0.880  4: 0.861
-//   1000^3         16: 0.788 (64)  8: 0.802        4: 0.846
+//   1000^3          8: 0.802 (16: 0.788)           4: 0.846
 //   2049x600x600   16: 0.596 (32)  8: 0.617        4: 0.861
 //   1536x1000x700   8: 0.791 (16 plane pairs: 0.803) 4: 1.240
 //   1030x1000x1000  8: 0.983                       4: 1.662
@@ -1230,7 +1232,7 @@ Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
   tl.ZW = 16;
   if (elemBytes == 4 && X >= 4 && Y >= 2 && Z >= 2) {
     Tiling tp = tl;
-    if (segmentsFilled(X, 0.95)) {
+    if (X >= kWave * 16 && X % (kWave * 16) <= 1) {
       tp.V = 16;
       tp.TP = 64;
       while (tp.TP > 16 && momTiles(X, Y, Z, tp) < (long long)fused16MomSlots(16)) tp.TP >>= 1;
