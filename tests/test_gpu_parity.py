@@ -106,6 +106,24 @@ def test_fused_and_pbm_histogram_tilings(lib, orc, cuda, tiling, vpl, dtype, X):
     np.testing.assert_array_equal(hb.cpu().numpy().view(np.uint64), h_ref)
 
 
+# 2 voxels per lane on double rows whose last 128-voxel segment is partial:
+# the kernel form with a second walk for the partial waves. Workgroups as wide
+# as the row or split into x-tiles (the bit right of a tile from memory), 2, 4
+# and 8 row pairs, one to many plane pairs per slab.
+F64_PW_TILINGS = [(1, 16, 4, 2), (3, 16, 2, 2), (8, 2, 4, 2), (64, 3, 8, 2), (5, 1, 2, 2), (7, 4, 4, 2)]
+
+
+@pytest.mark.parametrize("tiling", F64_PW_TILINGS)
+@pytest.mark.parametrize("X", [129, 200, 386, 1030, 1089])  # 129: whole segments (control)
+def test_f64_two_voxels_on_ragged_rows(lib, orc, cuda, tiling, X):
+    print(f"f64 2 voxels per lane {tiling} X={X}", flush=True)
+    f = synth.grf_field((19, 23, X), sigma=2.0, seed=X, device=cuda, dtype=torch.float64)
+    for relop, thr in ((">=", 0.0), ("<", -0.4)):
+        _, h_ref = oracle_all(orc, host_np(f), relop, thr)
+        h, _ = dev_fused(lib, f, relop, thr, tiling=tiling)
+        np.testing.assert_array_equal(h, h_ref, err_msg=f"{relop} {thr}")
+
+
 # 16 voxels per lane (float fields): rows of one to three 1024-voxel
 # segments, exact and ragged (the last segment gathered, the voxel right of a
 # segment read when it exists), 2 and 4 row pairs, slab depths that do and do
