@@ -587,7 +587,12 @@ int mkfFusedAutotune(const BMFieldInfo* pI, const BinMapF64* pM, MkfTiling* pCho
         if (!dup) cand.push_back(t);
       }
     }
-    cand.push_back(Tiling{256, 16, 4, 4});
+    // 4 voxels per lane at the built-in depth model (mid-size fields where
+    // the persistent kernels are chosen can run faster there: 1024 x 512 x
+    // 512 0.167 ms at 32 plane pairs against 0.183) and at 256 plane pairs.
+    const Tiling t4 = choose4Tiling(fv.X, fv.Y, fv.Z);
+    if (!(t4.V == cand[0].V && t4.TP == cand[0].TP && t4.ZW == cand[0].ZW)) cand.push_back(t4);
+    if (!(t4.TP == 256 && t4.ZW == 16)) cand.push_back(Tiling{256, 16, 4, 4});
     cand.push_back(Tiling{128, 16, 4, 1});
   } else {
     cand.push_back(Tiling{512, 16, 8, 1});

@@ -66,6 +66,7 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes);  // fused kernel
 // Fused kernel, 8 or 16 voxels per lane (floats): the slab depth that spreads
 // the tiles best over quarters / 4 of the kernel's resident slots.
 Tiling choose16Tiling(int X, int Y, int Z, int V = 16, int quarters = 4);
+Tiling choose4Tiling(int X, int Y, int Z);  // float fields at 4 voxels per lane
 Tiling chooseBpfdTiling(int X, int Y, int Z);              // histogram of a PBM
 Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes);  // fused kernel with moments
 bool tilingSupported(const Tiling& tl, int elemBytes);

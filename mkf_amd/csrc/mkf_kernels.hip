@@ -1068,6 +1068,18 @@ Tiling choose16Tiling(int X, int Y, int Z, int V, int quarters) {
   return tl;
 }
 
+// Float fields at 4 voxels per lane: workgroups as wide as pickWaves gives,
+// slab depth from the busiest-CU model over 256, 128, ... plane pairs.
+Tiling choose4Tiling(int X, int Y, int Z) {
+  X = std::max(X, 2);
+  Tiling tl;
+  tl.V = 4;
+  tl.Ty = 4;
+  tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * 4));
+  tl.TP = balancedDepth(X, Y, Z, tl, 256);
+  return tl;
+}
+
 Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
   X = std::max(X, 2);  // a row without cells takes the tiling of one cell
   Tiling tl;
@@ -1139,11 +1151,7 @@ Tiling chooseTiling(int X, int Y, int Z, int elemBytes) {
     }
     return tl;
   } else if (X - 1 > kWave * 8) {
-    tl.V = 4;
-    tl.Ty = 4;
-    tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * 4));
-    tl.TP = balancedDepth(X, Y, Z, tl, 256);
-    return tl;
+    return choose4Tiling(X, Y, Z);
   } else if (X - 1 > kWave * 4 && segmentsFilled(X, 0.72, kWave * 4) && grid32Filled(X, Y, Z)) {
     // Rows of 258..513 voxels that fill their two 256-voxel segments at
     // least 72 %, on grids of at least 2 workgroups per CU at 32 plane
