@@ -82,13 +82,16 @@ struct RowSched {
   int dy, dz;
 };
 
-template <typename T, int M, bool G, bool MOM>
+// VPL voxels per lane (16, 8 or 4): segments of 1024, 512 or 256 voxels, so
+// that narrow and ragged rows keep most lanes busy.
+template <typename T, int M, bool G, bool MOM, int VPL = kPbmVoxels>
 __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, T thr, int X, int Y, int nSegX,
                                                            const RowSched rs, uint32_t* __restrict__ pW,
                                                            uint32_t rowWS, uint32_t planeWS,
                                                            double* __restrict__ part) {
-  constexpr int V = kPbmVoxels;
+  constexpr int V = VPL;
   constexpr int E = 16 / sizeof(T);  // elements per 16-byte load
+  static_assert(V % E == 0 && V >= E && V <= kPbmVoxels, "whole 16-byte loads, at most 16 voxels per lane");
   constexpr int W = kWave * V;
   typedef T Vec __attribute__((ext_vector_type(E), aligned(sizeof(T))));
   const int lane = threadIdx.x & (kWave - 1);
@@ -1307,9 +1310,28 @@ struct PbmGrid {
   long long nBlocks;
 };
 
+// Voxels per lane of the map kernel: rows of at most 256 or 512 voxels take
+// the narrowest segment that still holds the whole row (4 or 8 voxels per
+// lane: fewer loads past the row's end); longer rows keep 1024-voxel
+// segments, whose waves store whole 128-byte map lines. Against 1024-voxel
+// segments for every row (f32 GRF, HIP events, identical maps): 64 x 4000 x
+// 4000 4.15 ms against 6.04; 200 x 1000 x 5000 1.46 against 1.99; 300 x 1000
+// x 3000 1.06 against 1.25; 509 x 511 x 517 0.122 against 0.138 (f64 alike).
+// Splitting a longer row over narrower segments splits its map stores too,
+// and they cost more than the idle lanes: 256-voxel segments took 1030 x
+// 1000 x 1000 to 1.45 ms against 0.93. The general form (several fields, an
+// x stride, rows shorter than one load) keeps 16.
+static int pbmVoxels(const FieldView& fv) {
+  const int eb = fv.enc & ENC_MASK_NUM;
+  if (fv.nF > 1 || fv.sx != 1 || fv.X < 16 / std::max(eb, 1)) return kPbmVoxels;
+  if (fv.X <= kWave * 4) return 4;
+  if (fv.X <= kWave * 8) return 8;
+  return 16;
+}
+
 static PbmGrid pbmGrid(const FieldView& fv) {
   PbmGrid g;
-  g.nSegX = (int)ceilDiv(fv.X, kWave * kPbmVoxels);
+  g.nSegX = (int)ceilDiv(fv.X, kWave * pbmVoxels(fv));
   RowSched& rs = g.rs;
   rs.nRows = (long long)fv.Y * fv.Z;
   const long long wantChunks = std::max<long long>(1, (MKF_PBM_WAVE_TARGET + g.nSegX - 1) / g.nSegX);
@@ -1333,13 +1355,27 @@ template <typename T, int M, bool MOM>
 static hipError_t launchPBMT(const FieldView& fv, T thr, uint32_t* pW, uint32_t rowWS, uint32_t planeWS,
                              double* part, hipStream_t s) {
   const PbmGrid g = pbmGrid(fv);
+  const dim3 grid((unsigned)g.nBlocks), block(kPbmWaves * kWave);
   // The general form also takes rows shorter than one 16-byte load.
-  if (fv.nF > 1 || fv.sx != 1 || fv.X < (int)(16 / sizeof(T)))
-    hipLaunchKernelGGL((kPBM<T, M, true, MOM>), dim3((unsigned)g.nBlocks), dim3(kPbmWaves * kWave), 0, s,
-                       fieldArgs<T>(fv), thr, fv.X, fv.Y, g.nSegX, g.rs, pW, rowWS, planeWS, part);
-  else
-    hipLaunchKernelGGL((kPBM<T, M, false, MOM>), dim3((unsigned)g.nBlocks), dim3(kPbmWaves * kWave), 0, s,
-                       fieldArgs<T>(fv), thr, fv.X, fv.Y, g.nSegX, g.rs, pW, rowWS, planeWS, part);
+  if (fv.nF > 1 || fv.sx != 1 || fv.X < (int)(16 / sizeof(T))) {
+    hipLaunchKernelGGL((kPBM<T, M, true, MOM>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X, fv.Y, g.nSegX, g.rs,
+                       pW, rowWS, planeWS, part);
+  } else {
+    switch (pbmVoxels(fv)) {
+      case 16:
+        hipLaunchKernelGGL((kPBM<T, M, false, MOM, 16>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X, fv.Y,
+                           g.nSegX, g.rs, pW, rowWS, planeWS, part);
+        break;
+      case 8:
+        hipLaunchKernelGGL((kPBM<T, M, false, MOM, 8>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X, fv.Y,
+                           g.nSegX, g.rs, pW, rowWS, planeWS, part);
+        break;
+      default:
+        hipLaunchKernelGGL((kPBM<T, M, false, MOM, 4>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X, fv.Y,
+                           g.nSegX, g.rs, pW, rowWS, planeWS, part);
+        break;
+    }
+  }
   return hipGetLastError();
 }
 
