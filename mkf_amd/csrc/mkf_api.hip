@@ -263,7 +263,7 @@ int mkfPBMMomentsAsync(BMPackWord* pW, double* pMom, BMOrg* pO, const BMFieldInf
   // Workgroup partials come from the stream-ordered pool, so calls on
   // different streams never share scratch.
   double* part = nullptr;
-  const size_t bytes = (size_t)pbmMomentParts(fv) * 6 * sizeof(double);
+  const size_t bytes = (size_t)pbmMomentParts(fv, pW, org.rowWS, org.planeWS) * 6 * sizeof(double);
   hipError_t e = hipMallocAsync((void**)&part, bytes, s);
   if (e != hipSuccess) return hipFail(e, "moments scratch");
   e = launchPBMMoments(fv, *pM, pW, org.rowWS, org.planeWS, part, pMom, s);
@@ -397,7 +397,8 @@ int mkfFusedBPFDMoments(size_t* pBPFD, double* pMom, BMOrg* pO, const BMFieldInf
   // A field without cells adds nothing to the histogram; its moments come
   // from the map kernel (no map written).
   const long long nParts = fusedMomentParts(fv, tl, pM->m);
-  const size_t bytes = (size_t)(nParts ? nParts : pbmMomentParts(fv)) * 6 * sizeof(double);
+  const long long need = nParts ? nParts : pbmMomentParts(fv, nullptr, org.rowWS, org.planeWS);
+  const size_t bytes = (size_t)need * 6 * sizeof(double);
   double* part = nullptr;
   hipError_t e = hipMallocAsync((void**)&part, bytes, s);
   if (e != hipSuccess) return hipFail(e, "moments scratch");
@@ -459,7 +460,7 @@ int mkfFusedBPFDMomentsNoAlloc(size_t* pBPFD, double* pMom, BMOrg* pO, const BMF
   rc = checkTiling(tl, fv.enc & ENC_MASK_NUM);
   if (rc) return rc;
   const long long nParts = fusedMomentParts(fv, tl, pM->m);
-  const long long need = nParts ? nParts : pbmMomentParts(fv);
+  const long long need = nParts ? nParts : pbmMomentParts(fv, nullptr, org.rowWS, org.planeWS);
   const long long maxParts = (long long)(kNoAllocDoubles / MKF_MOM_N);
   if (need > maxParts)
     return fail(MKF_ERR_BUDGET, "no-alloc moments: %lld workgroup partials exceed the %lld that fit the %d-byte scratch",

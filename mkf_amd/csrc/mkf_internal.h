@@ -77,9 +77,9 @@ hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint
 
 // As launchPBM (pW may be nullptr: no map), and ADDS the phase moments
 // {sum0, sumsq0, sum1, sumsq1, n0, n1} of the thresholded values to mom
-// (device, 6 doubles). part is device scratch of pbmMomentParts(fv) x 6
-// doubles.
-long long pbmMomentParts(const FieldView& fv);
+// (device, 6 doubles). part is device scratch of
+// pbmMomentParts(fv, pW, rowWS, planeWS) x 6 doubles.
+long long pbmMomentParts(const FieldView& fv, const uint32_t* pW, uint32_t rowWS, uint32_t planeWS);
 hipError_t launchPBMMoments(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint32_t rowWS,
                             uint32_t planeWS, double* part, double* mom, hipStream_t s);
 

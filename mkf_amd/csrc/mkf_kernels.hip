@@ -1314,9 +1314,10 @@ struct PbmGrid {
 // the narrowest segment that still holds the whole row (4 or 8 voxels per
 // lane: fewer loads past the row's end); longer rows keep 1024-voxel
 // segments, whose waves store whole 128-byte map lines. Against 1024-voxel
-// segments for every row (f32 GRF, HIP events, identical maps): 64 x 4000 x
-// 4000 4.15 ms against 6.04; 200 x 1000 x 5000 1.46 against 1.99; 300 x 1000
-// x 3000 1.06 against 1.25; 509 x 511 x 517 0.122 against 0.138 (f64 alike).
+// segments for every row (f32 GRF, HIP events, identical maps): 200 x 1000
+// x 5000 1.46 ms against 1.99; 300 x 1000 x 3000 1.06 against 1.25; 509 x
+// 511 x 517 0.122 against 0.138 (f64 alike). Rows of whole map words take
+// the plane-row view below instead.
 // Splitting a longer row over narrower segments splits its map stores too,
 // and they cost more than the idle lanes: 256-voxel segments took 1030 x
 // 1000 x 1000 to 1.45 ms against 0.93. The general form (several fields, an
@@ -1327,6 +1328,34 @@ static int pbmVoxels(const FieldView& fv) {
   if (fv.X <= kWave * 4) return 4;
   if (fv.X <= kWave * 8) return 8;
   return 16;
+}
+
+// The map pass on a contiguous planar field whose map rows are exactly its
+// rows (X a multiple of 32, rowWS = X / 32, planeWS = Y rowWS) may take each
+// plane as one row of X Y voxels: voxel (x, y, z) keeps map bit
+// 32 planeWS z + X y + x, and rows narrower than a segment, or ragged ones,
+// then fill whole segments (f32 GRF, identical maps and moments: 64 x 4000 x
+// 4000 0.77 ms against 4.16; 96 x 2000 x 5000 0.72 against 2.68; 1056 x 1000
+// x 1000 0.79 against 0.92). The reference launches such fields collapsed to
+// 1-D (src/gridUtil.cuh:12-38, src/binMapCUDA.cuh:282-294). Whole-word rows
+// are asked for also when no map is written (moments only), so the moments
+// come out the same to the bit either way. Rows of whole 1024-voxel segments
+// keep their own shape.
+static FieldView pbmView(const FieldView& fv, const uint32_t* pW, uint32_t& rowWS, uint32_t planeWS) {
+  const long long XY = (long long)fv.X * fv.Y;
+  const bool planar = fv.nF == 1 && fv.sx == 1 && fv.sy == fv.X && fv.sz == XY;
+  const bool sameBits =
+      fv.X % 32 == 0 && (!pW || ((long long)rowWS * 32 == fv.X && (long long)planeWS * 32 == XY));
+  if (!planar || !sameBits || fv.Y < 2 || fv.Z < 1 || fv.X % (kWave * kPbmVoxels) == 0 || XY > (1ll << 30))
+    return fv;
+  FieldView v = fv;
+  v.X = (int)XY;
+  v.Y = fv.Z;
+  v.Z = 1;
+  v.sy = XY;
+  v.sz = XY * fv.Z;
+  rowWS = (uint32_t)(XY / 32);
+  return v;
 }
 
 static PbmGrid pbmGrid(const FieldView& fv) {
@@ -1408,9 +1437,10 @@ hipError_t dispatchPBMf64(const FieldView& fv, double thr, uint32_t m, uint32_t*
 #endif
 
 #if MKF_PART(1)
-hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint32_t rowWS, uint32_t planeWS,
+hipError_t launchPBM(const FieldView& fv0, const BinMapF64& m, uint32_t* pW, uint32_t rowWS, uint32_t planeWS,
                      hipStream_t s) {
-  if (fv.X < 1 || fv.Y < 1 || fv.Z < 1) return hipErrorInvalidValue;
+  if (fv0.X < 1 || fv0.Y < 1 || fv0.Z < 1) return hipErrorInvalidValue;
+  const FieldView fv = pbmView(fv0, pW, rowWS, planeWS);
   // f32 fields compare against the threshold rounded to float
   // (src/binMapCUDA.cuh:138-139).
   if (fv.enc == ENC_F32) return dispatchPBM<float, false>(fv, (float)m.t[0], m.m, pW, rowWS, planeWS, nullptr, s);
@@ -1418,11 +1448,14 @@ hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint
   return hipErrorInvalidValue;
 }
 
-long long pbmMomentParts(const FieldView& fv) { return pbmGrid(fv).nBlocks; }
+long long pbmMomentParts(const FieldView& fv, const uint32_t* pW, uint32_t rowWS, uint32_t planeWS) {
+  return pbmGrid(pbmView(fv, pW, rowWS, planeWS)).nBlocks;
+}
 
-hipError_t launchPBMMoments(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint32_t rowWS,
+hipError_t launchPBMMoments(const FieldView& fv0, const BinMapF64& m, uint32_t* pW, uint32_t rowWS,
                             uint32_t planeWS, double* part, double* mom, hipStream_t s) {
-  if (fv.X < 1 || fv.Y < 1 || fv.Z < 1) return hipErrorInvalidValue;
+  if (fv0.X < 1 || fv0.Y < 1 || fv0.Z < 1) return hipErrorInvalidValue;
+  const FieldView fv = pbmView(fv0, pW, rowWS, planeWS);
   // Per-lane counts are 32-bit: at most 16 voxels per row.
   if (pbmGrid(fv).rs.rowsPerWave > (1ll << 27)) return hipErrorInvalidValue;
   hipError_t e;

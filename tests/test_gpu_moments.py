@@ -72,7 +72,9 @@ def M_oracle():
     return O
 
 
-SHAPES = [(2, 2, 2), (3, 5, 31), (4, 4, 1025), (9, 17, 200), (40, 50, 333), (3, 2, 2049)]
+# (9, 70, 96) and (3, 13, 2080): rows of whole map words, taken a plane at a time
+SHAPES = [(2, 2, 2), (3, 5, 31), (4, 4, 1025), (9, 17, 200), (40, 50, 333), (3, 2, 2049), (9, 70, 96),
+          (3, 13, 2080)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -101,6 +103,13 @@ def test_moments_accumulate_are_deterministic_and_map_optional(lib, cuda):
     np.testing.assert_allclose(c, a + np.array(pre), rtol=1e-15)
     _check(a, host_np(f), ">", 0.1)
     assert a[4] + a[5] == X * Y * Z
+    # rows of whole map words (each plane taken as one row) alike
+    g = f[:, :, :640].contiguous()
+    fg = M.FieldInfo(g.data_ptr(), (640, Y, Z), M.ENC_F32)
+    a, _ = _moments(lib, fg, 640, Y, Z, ">", 0.1, cuda)
+    b, _ = _moments(lib, fg, 640, Y, Z, ">", 0.1, cuda, with_map=False)
+    assert a.tobytes() == b.tobytes()
+    _check(a, host_np(g), ">", 0.1)
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])

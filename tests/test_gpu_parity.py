@@ -316,6 +316,32 @@ def test_x_strided_field(lib, orc, cuda, sx):
     np.testing.assert_array_equal(hf2, h_ref2)
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("shape", [(6, 300, 32), (9, 70, 96), (5, 41, 160), (3, 13, 2080), (40, 200, 64)])
+def test_whole_word_rows(lib, orc, cuda, shape, dtype):
+    """Rows of whole map words (X a multiple of 32 but not of 1024): the map
+    pass takes each plane as one row of X Y voxels. Maps (poisoned buffers)
+    and histograms equal the oracle's."""
+    print(f"whole-word rows {shape} {dtype}", flush=True)
+    f = synth.grf_field(shape, sigma=2.0, seed=shape[2] + shape[1], device=cuda, dtype=dtype)
+    for relop, thr in ((">=", 0.1), ("<", -0.3)):
+        check_all_paths(lib, orc, f, relop, thr)
+
+
+def test_whole_word_window_of_a_wider_field(lib, orc, cuda):
+    """A 64-wide window (row stride 130) keeps its own rows: its map equals
+    the oracle's map of the window."""
+    full = synth.grf_field((12, 40, 130), sigma=2.0, seed=81, device=cuda, dtype=torch.float32)
+    sub = full[2:10, 3:35, 5:69]
+    Z, Y, X = sub.shape
+    fi = M.FieldInfo(sub.data_ptr(), (X, Y, Z), M.ENC_F32, strides=(1, 130, 130 * 40))
+    w_ref, h_ref = oracle_all(orc, np.ascontiguousarray(host_np(sub)), ">", 0.0)
+    w, h, hf = _all_paths_on(lib, fi, X, Y, Z, ">", 0.0, cuda)
+    np.testing.assert_array_equal(w, w_ref)
+    np.testing.assert_array_equal(h, h_ref)
+    np.testing.assert_array_equal(hf, h_ref)
+
+
 def test_invalid_inputs_return_null(lib, cuda):
     bm = M.bin_map(">=", 0.5)
     org = M.BMOrg()
