@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
 #include <utility>
 
 namespace mkf {
@@ -435,6 +436,32 @@ struct PbmRows4 {
   __device__ __forceinline__ uint32_t haloBit(long long) const { return 0u; }
 };
 
+// PBM rows of a narrow map (rows of at most 4 words) read in row groups: the
+// wave's lanes form groups of 8 GW lanes, each group reading GW words of its
+// own stack of rows (a per-lane row offset folded into b0 and b1), so one
+// wave walks several row stacks side by side. kp[r] holds the pair-word
+// bytes of the lane's cells of row pair r that exist and that this group
+// owns; the walk applies it to the row pair's word, so pairs() needs no mask.
+template <int TY>
+struct PbmRowsR : PbmRows4 {
+  static constexpr bool kPairKeep = true;
+  uint32_t kp[TY];
+
+  __device__ __forceinline__ uint32_t pairs(const Raw& r, uint64_t& R0) const {
+    R0 = 0;
+    const uint32_t v = __builtin_amdgcn_alignbit(r.hi, r.lo, sh) & 31u;
+    return (v * 0x204081u) & 0x03030303u;
+  }
+  __device__ __forceinline__ uint32_t pairKeep(int r) const { return kp[r]; }
+};
+
+// Whether a row source masks its cells per row pair (kPairKeep) rather than
+// per row.
+template <class S, class = void>
+struct HasPairKeep : std::false_type {};
+template <class S>
+struct HasPairKeep<S, std::void_t<decltype(S::kPairKeep)>> : std::bool_constant<S::kPairKeep> {};
+
 // Per-lane phase moments: sums in double of the values and their squares per
 // phase (map bit 0 or 1) and the counts. Selects, not differences, split the
 // phases, so a NaN stays in its own phase. The selects act on the value
@@ -819,7 +846,8 @@ __device__ __forceinline__ void walkStack(const Src& src, uint32_t (*xch)[TY + 1
     // against 0.743).
 #pragma unroll
     for (int r = 0; r < TY; ++r) {
-      const uint32_t h = P[r] | (P[r + 1] << 2);
+      uint32_t h = P[r] | (P[r + 1] << 2);
+      if constexpr (HasPairKeep<Src>::value) h &= src.pairKeep(r);
       if constexpr (Src::kSelfHalo) {
         Hc[r] = halfCodes(h);
       } else {

@@ -514,6 +514,70 @@ __global__ __launch_bounds__(1024) void kBPFD4(const uint32_t* __restrict__ pW, 
   flushHistogram(hist, &cnt, tally, cellsPerRow * q.nY * q.nZ, bpfd);
 }
 
+// Histogram of a narrow PBM (rows of at most GW <= 8 words, X <= 256): the
+// lanes form G = 8 / GW groups of 8 GW lanes, group k walking row pairs
+// yT + k TY .. yT + k TY + TY - 1 of the wave's tile, so a wave covers G TY
+// row pairs of 32 GW voxels instead of TY row pairs of a 256-voxel segment
+// that is mostly past the row's end. A group whose rows would run past the
+// field reads rows Y - 1 - TY .. Y - 1 instead and masks off the pairs that
+// belong to the group before it (kp), so every load stays inside the map and
+// every cell is counted once. The host requires Y - 1 >= TY. Each of the
+// workgroup's g.waves waves takes a tile of its own (consecutive tiles, which
+// share their halo rows), and the waves share one LDS histogram, so the
+// device histogram takes one flush per workgroup instead of one per tile.
+template <int TY, int GW>
+__global__ __launch_bounds__(1024) void kBPFD4R(const uint32_t* __restrict__ pW, int X, int Y, int Z,
+                                                uint32_t rowWS, uint32_t planeWS, TileGrid g,
+                                                unsigned long long* __restrict__ bpfd) {
+  constexpr int G = 8 / GW;
+  static_assert(GW == 1 || GW == 2 || GW == 4 || GW == 8, "1, 2, 4 or 8 words per row group");
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t cnt;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int b = blockIdx.x;
+  const long long L = (long long)((b & 7) * g.chunk + (b >> 3)) * g.waves + w;
+  clearHistogram(hist, &cnt);
+  Tally tally;
+  uint32_t cells = 0;
+  if (L < g.nTiles) {
+    const int yT = (int)(L % g.nYR) * (G * TY);
+    const int z0 = (int)(L / g.nYR) * g.TZ;
+    const int nZ = min(g.TZ, Z - 1 - z0);
+    const int yWant = yT + (lane / (8 * GW)) * TY;  // the group's first row pair
+    const int yBase = min(yT, Y - 1 - TY);           // group 0's rows (wave-uniform)
+    const int yk = min(yWant, Y - 1 - TY);           // this group's rows
+    const uint32_t wl = (uint32_t)((lane >> 3) & (GW - 1));
+    PbmRowsR<TY> src;
+    src.row = pW;
+    src.sy = rowWS;
+    src.sz = planeWS;
+    const uint32_t ro = (uint32_t)(yk - yBase) * rowWS;
+    src.b0 = 4u * (ro + min(wl, rowWS - 1u));
+    src.b1 = 4u * (ro + min(wl + 1u, rowWS - 1u));
+    src.sh = 4u * (uint32_t)(lane & 7);
+    const int x0 = 32 * (int)wl + 4 * (lane & 7);
+    uint32_t xkeep = 0, nx = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const bool ok = x0 + c < X - 1;
+      xkeep |= ok ? (0xFFu << (8 * c)) : 0u;
+      nx += ok ? 1u : 0u;
+    }
+    src.keep = xkeep;
+    uint32_t np = 0;
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+      const bool ok = yk + r >= yWant && yk + r < Y - 1;
+      src.kp[r] = ok ? xkeep : 0u;
+      np += ok ? 1u : 0u;
+    }
+    cells = waveSum(nx * np) * (uint32_t)nZ;
+    walkStack<TY, 4>(src, (uint32_t(*)[TY + 1][16]) nullptr, 0, kHaloNone, yBase, TY, z0, nZ, hist, tally);
+  }
+  flushHistogram(hist, &cnt, tally, cells, bpfd);
+}
+
 // ---------------------------------------------------------------------------
 // Fused histogram, float fields, 4 NG voxels per lane (voxelsPerLane 16 for
 // NG = 4, 8 for NG = 2). The lane layout of kPBM: lane l holds groups
@@ -891,6 +955,35 @@ static inline TileGrid makeGrid(int X, int Y, int Z, const Tiling& tl) {
   g.nZR = (int)ceilDiv(Z - 1, tl.TP);
   g.nTiles = g.nXT * g.nYR * g.nZR;
   g.chunk = (int)ceilDiv(g.nTiles, 8);
+  return g;
+}
+
+// Waves per workgroup of the narrow PBM histogram, each on a tile of its own.
+#ifndef MKF_NARROW_WAVES
+#define MKF_NARROW_WAVES 8
+#endif
+[[maybe_unused]] constexpr int kNarrowWaves = MKF_NARROW_WAVES;
+
+// Words per row group of the narrow PBM histogram (kBPFD4R) for a map of
+// rowWS-word rows, or 0 where the map takes kBPFD4 (rows of more than 8
+// words, or fewer row pairs than one group's stack).
+static inline int narrowGroupWords(uint32_t rowWS, int Y, const Tiling& tl) {
+  if (tl.V != 4 || rowWS < 1 || rowWS > 8 || Y - 1 < tl.Ty) return 0;
+  return rowWS == 1 ? 1 : (rowWS == 2 ? 2 : (rowWS <= 4 ? 4 : 8));
+}
+
+// Its grid: tiles of 8 / GW stacks of Ty row pairs by TP plane pairs, one per
+// wave, tl.ZW waves per workgroup.
+static inline TileGrid narrowGrid(int GW, int Y, int Z, const Tiling& tl) {
+  TileGrid g;
+  g.nSegX = 1;
+  g.nXT = 1;
+  g.waves = tl.ZW;
+  g.TZ = tl.TP;
+  g.nYR = (int)ceilDiv(Y - 1, (long long)(8 / GW) * tl.Ty);
+  g.nZR = (int)ceilDiv(Z - 1, tl.TP);
+  g.nTiles = g.nYR * g.nZR;
+  g.chunk = (int)ceilDiv(ceilDiv(g.nTiles, tl.ZW), 8);
   return g;
 }
 
@@ -1273,17 +1366,21 @@ Tiling chooseMomTiling(int X, int Y, int Z, int elemBytes) {
 // Histogram of an existing PBM: kBPFD4 with 4 row pairs.
 Tiling chooseBpfdTiling(int X, int Y, int Z) {
   X = std::max(X, 2);  // a row without cells takes the tiling of one cell
-  // 4 voxels per lane, 32 plane pairs per workgroup, or 16 where that gives
-  // fewer than 4 workgroups per CU. Measured on MI355X on GRF maps (HIP
-  // events, best of 3 rounds of 20 launches), against the 16-voxel kernel
-  // (kBPFD16) with its resident-slot slab depth:
-  //   1024^3 0.247 ms against 0.257; 1000 x 733 x 733 0.141 against 0.194;
-  //   1000 x 366 x 366 0.057 against 0.137; 1536 x 1000 x 700 0.294 against
-  //   0.376; 800 x 1000 x 1000 0.248 against 0.248; 2048 x 256 x 256 (16
-  //   plane pairs) 0.044 against 0.138; 1024 x 181 x 181 (16) 0.028 against
-  //   0.056.
-  // 256 plane pairs, the earlier choice, took 0.325 ms at 1024^3, and 64
-  // took 0.254; 16 took 0.41 there.
+  // 4 voxels per lane, 4 row pairs; slabs of 32 plane pairs, or 16 where
+  // that gives fewer than 4 workgroups per CU, and deeper (up to 128) while
+  // the grid has more than 4096 workgroups. Each workgroup adds its LDS
+  // histogram to the device's 256 bins with one atomic per bin, all on the
+  // same 256 addresses, which costs about 22 ns per workgroup: at 1024^3 GRF
+  // 8192 workgroups (32 plane pairs) took 0.219 ms, the same kernel with its
+  // atomics spread over 64 copies of the bins 0.166 ms, and 4096 workgroups
+  // (64 plane pairs) 0.175 ms. Other GRF maps (HIP events, 20-launch means),
+  // 32 / 64 / 128 plane pairs: 1000^3 0.217 / 0.176 / 0.187 ms; 2048 x 512 x
+  // 1024 0.174 / 0.178 / 0.189; 500 x 1000 x 2000 0.369 / 0.240 / 0.202;
+  // 300 x 1000 x 3000 0.539 / 0.325 / 0.280. Against the 16-voxel kernel
+  // (kBPFD16) with its resident-slot slab depth, at 32 plane pairs: 1000 x
+  // 733 x 733 0.141 against 0.194; 1000 x 366 x 366 0.057 against 0.137;
+  // 2048 x 256 x 256 (16) 0.044 against 0.138. Rows of at most 8 words take
+  // the row-group kernel (kBPFD4R) with kNarrowWaves waves per workgroup.
   int cus = 0, dev = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
@@ -1292,7 +1389,16 @@ Tiling chooseBpfdTiling(int X, int Y, int Z) {
   tl.Ty = 4;
   tl.TP = 32;
   tl.ZW = pickWaves(ceilDiv(X - 1, (long long)kWave * tl.V));
-  if ((long long)makeGrid(X, Y, Z, tl).nTiles < 4LL * cus) tl.TP = 16;
+  const int GW = narrowGroupWords((uint32_t)((X + 31) / 32), Y, tl);
+  if (GW) tl.ZW = kNarrowWaves;
+  auto groups = [&] {
+    return GW ? ceilDiv(narrowGrid(GW, Y, Z, tl).nTiles, tl.ZW) : (long long)makeGrid(X, Y, Z, tl).nTiles;
+  };
+  if (groups() < 4LL * cus) {
+    tl.TP = 16;
+  } else {
+    while (tl.TP < 128 && groups() > 4096) tl.TP *= 2;
+  }
   return tl;
 }
 
@@ -1868,6 +1974,32 @@ hipError_t launchBPFD(const uint32_t* pW, int X, int Y, int Z, uint32_t rowWS, u
       hipLaunchKernelGGL((kBPFD16<2>), dim3(nBlocks), dim3(kWave), 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd);
     else
       hipLaunchKernelGGL((kBPFD16<4>), dim3(nBlocks), dim3(kWave), 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd);
+    return hipGetLastError();
+  }
+  if (const int GW = narrowGroupWords(rowWS, Y, tl)) {
+    const TileGrid g = narrowGrid(GW, Y, Z, tl);
+    if (info) {
+      fillInfo(info, g, (long long)g.chunk * 8, tl);
+      return hipSuccess;
+    }
+    const dim3 grid((unsigned)g.chunk * 8u), block(g.waves * kWave);
+#define MKF_BPFD4R(TY, GWV) \
+  hipLaunchKernelGGL((kBPFD4R<TY, GWV>), grid, block, 0, s, pW, X, Y, Z, rowWS, planeWS, g, bpfd)
+    switch (tl.Ty * 8 + GW) {
+      case 2 * 8 + 1: MKF_BPFD4R(2, 1); break;
+      case 2 * 8 + 2: MKF_BPFD4R(2, 2); break;
+      case 2 * 8 + 4: MKF_BPFD4R(2, 4); break;
+      case 8 * 8 + 1: MKF_BPFD4R(8, 1); break;
+      case 8 * 8 + 2: MKF_BPFD4R(8, 2); break;
+      case 8 * 8 + 4: MKF_BPFD4R(8, 4); break;
+      case 2 * 8 + 8: MKF_BPFD4R(2, 8); break;
+      case 8 * 8 + 8: MKF_BPFD4R(8, 8); break;
+      case 4 * 8 + 1: MKF_BPFD4R(4, 1); break;
+      case 4 * 8 + 2: MKF_BPFD4R(4, 2); break;
+      case 4 * 8 + 8: MKF_BPFD4R(4, 8); break;
+      default: MKF_BPFD4R(4, 4); break;
+    }
+#undef MKF_BPFD4R
     return hipGetLastError();
   }
   const TileGrid g = makeGrid(X, Y, Z, tl);

@@ -106,6 +106,32 @@ def test_fused_and_pbm_histogram_tilings(lib, orc, cuda, tiling, vpl, dtype, X):
     np.testing.assert_array_equal(hb.cpu().numpy().view(np.uint64), h_ref)
 
 
+# Maps whose rows are 1 to 8 words: the histogram's lanes walk 8, 4, 2 or 1
+# row stacks side by side (257: 9 words, the per-segment kernel). Y - 1 from below one stack (the per-segment kernel) to
+# many stacks, with a tile's last groups past the field (read from
+# Y - 1 - rowPairs); 2, 4 and 8 row pairs; slab depths that do and do not
+# divide the planes; 1 to 16 waves per workgroup, each on a tile of its own
+# (some past the last tile); the default tiling.
+NARROW_CASES = [(3, 5, (4, 1, 4, 4)), (5, 9, (4, 1, 4, 4)), (3, 7, (3, 1, 2, 4)), (9, 6, (2, 1, 8, 4)),
+                (37, 11, (5, 3, 4, 4)), (70, 13, (32, 16, 2, 4)), (75, 10, (3, 8, 8, 4)), (41, 17, None),
+                (130, 40, (7, 2, 4, 4))]
+
+
+@pytest.mark.parametrize("X", [2, 3, 17, 31, 32, 33, 50, 64, 65, 96, 100, 127, 128, 129, 200, 256, 257])
+def test_pbm_histogram_of_narrow_rows(lib, orc, cuda, X):
+    for k, (Y, Z, tiling) in enumerate(NARROW_CASES):
+        print(f"narrow PBM histogram X={X} Y={Y} Z={Z} tiling {tiling}", flush=True)
+        f = synth.grf_field((Z, Y, X), sigma=1.5, seed=X * 16 + k, device=cuda, dtype=torch.float32)
+        for relop, thr in ((">=", 0.0), ("<", 0.3)):
+            _, h_ref = oracle_all(orc, host_np(f), relop, thr)
+            _, dw, org, _ = dev_pbm(lib, f, relop, thr)
+            hb = torch.zeros(256, dtype=torch.int64, device=cuda)
+            t = M.MkfTiling(*tiling) if tiling else None
+            M.check(lib.mkfBPFDAsync(hb.data_ptr(), C.byref(org), dw.data_ptr(), C.byref(t) if t else None, None))
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(hb.cpu().numpy().view(np.uint64), h_ref, err_msg=f"Y={Y} Z={Z} {relop}")
+
+
 # 2 voxels per lane on double rows whose last 128-voxel segment is partial:
 # the kernel form with a second walk for the partial waves. Workgroups as wide
 # as the row or split into x-tiles (the bit right of a tile from memory), 2, 4
