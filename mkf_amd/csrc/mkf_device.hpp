@@ -724,8 +724,12 @@ __device__ __forceinline__ void clearHistogram(uint32_t* hist, uint32_t* cnt) {
 // workgroup's cells (each wave passes its own count) minus every other bin.
 // Every thread of the workgroup calls it once, after its tally; hist[256]
 // and cnt are the workgroup's LDS histogram and a counter, cleared at start.
+// With rows, the workgroup instead stores its 256 bins to row blockIdx.x of
+// rows (plain stores), for kHistRowsReduce to add up: the device bins then
+// take a few atomics per bin instead of one per bin and workgroup.
 __device__ __forceinline__ void flushHistogram(uint32_t* hist, uint32_t* cnt, const Tally& tally, uint32_t cells,
-                                               unsigned long long* __restrict__ bpfd) {
+                                               unsigned long long* __restrict__ bpfd,
+                                               uint32_t* __restrict__ rows = nullptr) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t ones = waveSum(tally.ones);
   __syncthreads();
@@ -741,6 +745,11 @@ __device__ __forceinline__ void flushHistogram(uint32_t* hist, uint32_t* cnt, co
     if (lane == 0) hist[0] = *cnt - rest;
   }
   __syncthreads();
+  if (rows) {
+    uint32_t* const row = rows + (size_t)blockIdx.x * 256;
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) row[i] = hist[i];
+    return;
+  }
   for (int i = threadIdx.x; i < 256; i += blockDim.x) {
     const uint32_t v = hist[i];
     if (v) atomicAdd(bpfd + i, (unsigned long long)v);

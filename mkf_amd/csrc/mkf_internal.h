@@ -80,6 +80,9 @@ hipError_t launchPBM(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint
 // (device, 6 doubles). part is device scratch of
 // pbmMomentParts(fv, pW, rowWS, planeWS) x 6 doubles.
 long long pbmMomentParts(const FieldView& fv, const uint32_t* pW, uint32_t rowWS, uint32_t planeWS);
+
+// Adds nRows per-workgroup histograms (256 x u32 each, device) to bpfd.
+hipError_t launchHistRowsReduce(const uint32_t* rows, long long nRows, unsigned long long* bpfd, hipStream_t s);
 hipError_t launchPBMMoments(const FieldView& fv, const BinMapF64& m, uint32_t* pW, uint32_t rowWS,
                             uint32_t planeWS, double* part, double* mom, hipStream_t s);
 

@@ -381,17 +381,23 @@ __device__ __forceinline__ uint32_t cellKeep(int x0, int X, uint32_t& cellsPerRo
 // With MOM the kernel also writes the workgroup's phase moments of the
 // voxels it owns (see walkStack) to part[blockIdx.x]; blocks past the last
 // tile write zeros. PW compiles the walk for partial waves into the plain
-// 2-voxel double kernel too (for rows whose last segment is partial).
+// 2-voxel double kernel too (for rows whose last segment is partial). With
+// rows (forms without moments), each workgroup stores its histogram to
+// rows[blockIdx.x] (zeros past the last tile) instead of adding it to bpfd.
 template <typename T, int M, int V, int TY, bool G, bool MOM = false, bool PW = false>
 __global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr, int X, int Y, int Z, TileGrid g,
                                                     unsigned long long* __restrict__ bpfd,
-                                                    double* __restrict__ part = nullptr) {
+                                                    double* __restrict__ part = nullptr,
+                                                    uint32_t* __restrict__ rows = nullptr) {
   __shared__ uint32_t hist[256];
   __shared__ uint32_t cnt;
   __shared__ uint32_t xch[2][TY + 1][16];
   int xt, yr, zr;
   if (!tileOf(g, xt, yr, zr)) {
     if (MOM && threadIdx.x < kMomN) part[(size_t)blockIdx.x * kMomN + threadIdx.x] = 0.0;
+    if (!MOM && rows) {
+      for (int i = threadIdx.x; i < 256; i += blockDim.x) rows[(size_t)blockIdx.x * 256 + i] = 0u;
+    }
     return;
   }
   const int lane = threadIdx.x & (kWave - 1);
@@ -445,7 +451,7 @@ __global__ __launch_bounds__(1024) void kFusedBPFD(const FieldArgs<T> fa, T thr,
       momBlockPartial(mom.acc, part);
     } else {
       walkStack<TY, V>(src, xch, q.w, q.halo, q.y0, q.nY, q.z0, q.nZ, hist, tally);
-      flushHistogram(hist, &cnt, tally, cellsPerRow * q.nY * q.nZ, bpfd);
+      flushHistogram(hist, &cnt, tally, cellsPerRow * q.nY * q.nZ, bpfd, rows);
     }
   };
   if constexpr (!G && V > 1 && (MOM || V == 4 || PW)) {
@@ -957,6 +963,10 @@ static inline TileGrid makeGrid(int X, int Y, int Z, const Tiling& tl) {
   g.chunk = (int)ceilDiv(g.nTiles, 8);
   return g;
 }
+
+// Fused launches of more workgroups than this store per-workgroup histograms
+// and add them up with kHistRowsReduce.
+[[maybe_unused]] constexpr long long kHistRowsMin = 4096;
 
 // Waves per workgroup of the narrow PBM histogram, each on a tile of its own.
 #ifndef MKF_NARROW_WAVES
@@ -1575,6 +1585,33 @@ hipError_t launchPBMMoments(const FieldView& fv0, const BinMapF64& m, uint32_t* 
   return launchMomentsReduce(part, pbmGrid(fv).nBlocks, mom, s);
 }
 
+// Adds the per-workgroup histograms rows[0 .. nRows - 1] (256 bins each) to
+// bpfd: block j sums rows j, j + gridDim.x, ... bin by bin in 64 bits and
+// adds each nonzero sum with one atomic.
+__global__ __launch_bounds__(256) void kHistRowsReduce(const uint32_t* __restrict__ rows, long long nRows,
+                                                       unsigned long long* __restrict__ bpfd) {
+  constexpr int U = 8;  // rows in flight per thread
+  unsigned long long a = 0;
+  long long r = blockIdx.x;
+  const long long step = gridDim.x;
+  for (; r + (U - 1) * step < nRows; r += U * step) {
+    uint32_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = rows[(size_t)(r + u * step) * 256 + threadIdx.x];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a += v[u];
+  }
+  for (; r < nRows; r += step) a += rows[(size_t)r * 256 + threadIdx.x];
+  if (a) atomicAdd(bpfd + threadIdx.x, a);
+}
+
+hipError_t launchHistRowsReduce(const uint32_t* rows, long long nRows, unsigned long long* bpfd, hipStream_t s) {
+  if (nRows <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<long long>(nRows, 256);
+  hipLaunchKernelGGL(kHistRowsReduce, dim3(blocks), dim3(256), 0, s, rows, nRows, bpfd);
+  return hipGetLastError();
+}
+
 hipError_t launchMomentsReduce(const double* part, long long nParts, double* mom, hipStream_t s) {
   hipLaunchKernelGGL(kMomentsReduce, dim3(1), dim3(256), 0, s, part, nParts, mom);
   return hipGetLastError();
@@ -1583,49 +1620,49 @@ hipError_t launchMomentsReduce(const double* part, long long nParts, double* mom
 
 template <typename T, int M, int V, int TY>
 static hipError_t launchFusedT(const FieldView& fv, T thr, unsigned long long* bpfd, const TileGrid& g,
-                               hipStream_t s) {
+                               uint32_t* rows, hipStream_t s) {
   // Several fields or an x stride take the general form, one voxel per lane.
   const dim3 grid((unsigned)g.chunk * 8u), block(g.waves * kWave);
   if (fv.nF > 1 || fv.sx != 1) {
     if constexpr (V == 1)
       hipLaunchKernelGGL((kFusedBPFD<T, M, 1, TY, true>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X, fv.Y,
-                         fv.Z, g, bpfd);
+                         fv.Z, g, bpfd, nullptr, rows);
     else
       return hipErrorInvalidValue;
   } else if (V == 2 && fv.X % (kWave * V) > 1) {
     // Rows whose last segment is partial: the form with a second walk for
     // the partial waves (PW).
     hipLaunchKernelGGL((kFusedBPFD<T, M, V, TY, false, false, V == 2>), grid, block, 0, s, fieldArgs<T>(fv), thr,
-                       fv.X, fv.Y, fv.Z, g, bpfd);
+                       fv.X, fv.Y, fv.Z, g, bpfd, nullptr, rows);
   } else {
     hipLaunchKernelGGL((kFusedBPFD<T, M, V, TY, false>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X, fv.Y,
-                       fv.Z, g, bpfd);
+                       fv.Z, g, bpfd, nullptr, rows);
   }
   return hipGetLastError();
 }
 
 template <typename T, int V, int TY>
 static hipError_t dispatchFusedM(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd,
-                                 const TileGrid& g, hipStream_t s) {
+                                 const TileGrid& g, uint32_t* rows, hipStream_t s) {
   switch (m & 7u) {
-    case 0: return launchFusedT<T, 0, V, TY>(fv, thr, bpfd, g, s);
-    case 1: return launchFusedT<T, 1, V, TY>(fv, thr, bpfd, g, s);
-    case 2: return launchFusedT<T, 2, V, TY>(fv, thr, bpfd, g, s);
-    case 3: return launchFusedT<T, 3, V, TY>(fv, thr, bpfd, g, s);
-    case 4: return launchFusedT<T, 4, V, TY>(fv, thr, bpfd, g, s);
-    case 5: return launchFusedT<T, 5, V, TY>(fv, thr, bpfd, g, s);
-    case 6: return launchFusedT<T, 6, V, TY>(fv, thr, bpfd, g, s);
-    default: return launchFusedT<T, 7, V, TY>(fv, thr, bpfd, g, s);
+    case 0: return launchFusedT<T, 0, V, TY>(fv, thr, bpfd, g, rows, s);
+    case 1: return launchFusedT<T, 1, V, TY>(fv, thr, bpfd, g, rows, s);
+    case 2: return launchFusedT<T, 2, V, TY>(fv, thr, bpfd, g, rows, s);
+    case 3: return launchFusedT<T, 3, V, TY>(fv, thr, bpfd, g, rows, s);
+    case 4: return launchFusedT<T, 4, V, TY>(fv, thr, bpfd, g, rows, s);
+    case 5: return launchFusedT<T, 5, V, TY>(fv, thr, bpfd, g, rows, s);
+    case 6: return launchFusedT<T, 6, V, TY>(fv, thr, bpfd, g, rows, s);
+    default: return launchFusedT<T, 7, V, TY>(fv, thr, bpfd, g, rows, s);
   }
 }
 
 template <typename T, int V>
 static hipError_t dispatchFusedTY(const FieldView& fv, T thr, uint32_t m, unsigned long long* bpfd,
-                                  const TileGrid& g, int Ty, hipStream_t s) {
+                                  const TileGrid& g, int Ty, uint32_t* rows, hipStream_t s) {
   switch (Ty) {
-    case 2: return dispatchFusedM<T, V, 2>(fv, thr, m, bpfd, g, s);
-    case 4: return dispatchFusedM<T, V, 4>(fv, thr, m, bpfd, g, s);
-    default: return dispatchFusedM<T, V, 8>(fv, thr, m, bpfd, g, s);
+    case 2: return dispatchFusedM<T, V, 2>(fv, thr, m, bpfd, g, rows, s);
+    case 4: return dispatchFusedM<T, V, 4>(fv, thr, m, bpfd, g, rows, s);
+    default: return dispatchFusedM<T, V, 8>(fv, thr, m, bpfd, g, rows, s);
   }
 }
 
@@ -1651,8 +1688,23 @@ static hipError_t dispatchFused(const FieldView& fv, T thr, uint32_t m, unsigned
   const Tiling tl = groupTiling(fv, tl0);
   const TileGrid g = makeGrid(fv.X, fv.Y, fv.Z, tl);
   if (g.nTiles <= 0) return hipSuccess;
-  if (tl.V == 1) return dispatchFusedTY<T, 1>(fv, thr, m, bpfd, g, tl.Ty, s);
-  return dispatchFusedTY<T, VW>(fv, thr, m, bpfd, g, tl.Ty, s);
+  // Grids of many workgroups (narrow rows: one wave per workgroup) store
+  // per-workgroup histograms for kHistRowsReduce rather than each adding its
+  // 256 bins to the same device bins (about 22 ns of atomics per workgroup).
+  const long long nGroups = (long long)g.chunk * 8;
+  uint32_t* rows = nullptr;
+  if (nGroups > kHistRowsMin && hipMallocAsync((void**)&rows, (size_t)nGroups * 256 * sizeof(uint32_t), s) != hipSuccess) {
+    (void)hipGetLastError();
+    rows = nullptr;  // the atomic flush needs no scratch
+  }
+  hipError_t e = tl.V == 1 ? dispatchFusedTY<T, 1>(fv, thr, m, bpfd, g, tl.Ty, rows, s)
+                           : dispatchFusedTY<T, VW>(fv, thr, m, bpfd, g, tl.Ty, rows, s);
+  if (rows) {
+    if (e == hipSuccess) e = launchHistRowsReduce(rows, nGroups, bpfd, s);
+    const hipError_t ef = hipFreeAsync(rows, s);
+    if (e == hipSuccess) e = ef;
+  }
+  return e;
 }
 
 #if MKF_PART(2)

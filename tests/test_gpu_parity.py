@@ -132,6 +132,37 @@ def test_pbm_histogram_of_narrow_rows(lib, orc, cuda, X):
             np.testing.assert_array_equal(hb.cpu().numpy().view(np.uint64), h_ref, err_msg=f"Y={Y} Z={Z} {relop}")
 
 
+# Fused launches of more than 4096 workgroups store one histogram per
+# workgroup and add them up in a second kernel: 1, 2 and 4 voxels per lane,
+# whole and ragged rows, x-tiles with the halo bit from memory, the general
+# (two-field) form, and accumulation into a nonzero histogram.
+ROWS_CASES = [((40, 500, 64), torch.float32, (1, 16, 2, 1)), ((40, 500, 300), torch.float32, (1, 1, 2, 4)),
+              ((40, 500, 200), torch.float64, (1, 16, 2, 2)), ((33, 301, 97), torch.float64, (1, 16, 2, 1))]
+
+
+@pytest.mark.parametrize("case", range(len(ROWS_CASES)))
+def test_fused_many_workgroups_reduce_histogram_rows(lib, orc, cuda, case):
+    shape, dt, tiling = ROWS_CASES[case]
+    Z, Y, X = shape
+    print(f"fused histogram rows {shape} {dt} tiling {tiling}", flush=True)
+    f = synth.grf_field(shape, sigma=2.0, seed=500 + case, device=cuda, dtype=dt)
+    for relop, thr in ((">=", 0.0), ("<", 0.4)):
+        _, h_ref = oracle_all(orc, host_np(f), relop, thr)
+        h, _ = dev_fused(lib, f, relop, thr, tiling=tiling)
+        np.testing.assert_array_equal(h, h_ref, err_msg=f"{relop} {thr}")
+        h2, _ = dev_fused(lib, f, relop, thr, tiling=tiling, pre=h_ref)
+        np.testing.assert_array_equal(h2, 2 * h_ref)
+    g = synth.grf_field(shape, sigma=2.0, seed=600 + case, device=cuda, dtype=dt)
+    fi = M.FieldInfo([f.data_ptr(), g.data_ptr()], (X, Y, Z), enc_of(f))
+    bm = M.bin_map(">", 0.1)
+    h = torch.zeros(256, dtype=torch.int64, device=cuda)
+    t = M.MkfTiling(tiling[0], tiling[1], tiling[2], 1)  # the general form takes 1 voxel per lane
+    M.check(lib.mkfFusedBPFD(None, h.data_ptr(), None, fi.ref, C.byref(bm), C.byref(t), None), "two fields")
+    torch.cuda.synchronize()
+    _, h_ref = oracle_all(orc, host_np(f) + host_np(g), ">", 0.1)
+    np.testing.assert_array_equal(h.cpu().numpy().view(np.uint64), h_ref)
+
+
 # 2 voxels per lane on double rows whose last 128-voxel segment is partial:
 # the kernel form with a second walk for the partial waves. Workgroups as wide
 # as the row or split into x-tiles (the bit right of a tile from memory), 2, 4
