@@ -80,11 +80,17 @@ constexpr int kPbmStage = MKF_PBM_STAGE;  // map rows staged per wave
 struct RowSched {
   long long nRows, rowsPerWave, nChunks, chunkStep, rowStep;
   int dy, dz;
+  int Z;  // planes of the field (rows of row groups past the last plane are not there)
 };
 
 // VPL voxels per lane (16, 8 or 4): segments of 1024, 512 or 256 voxels, so
-// that narrow and ragged rows keep most lanes busy.
-template <typename T, int M, bool G, bool MOM, int VPL = kPbmVoxels>
+// that narrow and ragged rows keep most lanes busy. Row groups (GRP = 2, 4
+// or 8; float fields at 4 voxels per lane): the wave's lanes form GRP groups
+// of 64 / GRP lanes, group k reading row GRP R + k of row group R, so rows of
+// at most 256 / GRP voxels fill the wave. The schedule then counts row
+// groups, and each lane keeps its own row; lanes whose row is past the last
+// plane load the last row, keep no voxel and write nothing.
+template <typename T, int M, bool G, bool MOM, int VPL = kPbmVoxels, int GRP = 1>
 __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, T thr, int X, int Y, int nSegX,
                                                            const RowSched rs, uint32_t* __restrict__ pW,
                                                            uint32_t rowWS, uint32_t planeWS,
@@ -92,9 +98,12 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   constexpr int V = VPL;
   constexpr int E = 16 / sizeof(T);  // elements per 16-byte load
   static_assert(V % E == 0 && V >= E && V <= kPbmVoxels, "whole 16-byte loads, at most 16 voxels per lane");
+  static_assert(GRP == 1 || (V == E && E == 4 && !G), "row groups: one 16-byte load of floats per lane and row");
+  constexpr int SL = kWave / GRP;  // lanes per row
   constexpr int W = kWave * V;
   typedef T Vec __attribute__((ext_vector_type(E), aligned(sizeof(T))));
   const int lane = threadIdx.x & (kWave - 1);
+  const int ll = lane % SL;  // lane within its row
   const long long gw = (long long)blockIdx.x * kPbmWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int seg = (int)(gw % nSegX);
   // Waves past the last chunk (the grid is whole workgroups) take no rows:
@@ -104,8 +113,9 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   if (!MOM && chunk >= rs.nChunks) return;  // with moments every wave reaches the block reduction
   const long long n =
       chunk >= rs.nChunks ? 0ll : min(rs.rowsPerWave, (rs.nRows - r0 + rs.rowStep - 1) / rs.rowStep);
-  int y = (int)(r0 % Y);
-  long long z = r0 / Y;
+  const long long rl = GRP > 1 ? r0 * GRP + lane / SL : r0;  // this lane's first row
+  int y = (int)(rl % Y);
+  long long z = rl / Y;
   const int x0 = seg * W;
   // Lane-contiguous layout: load q of a row gives lane l the E voxels at
   // x0 + 64 E q + E l, so each 16-byte load of the wave covers 1 KiB of
@@ -119,11 +129,11 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   uint32_t kq[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    const int xs = x0 + kWave * E * q + E * lane;
+    const int xs = x0 + kWave * E * q + E * ll;
     kq[q] = (xs >= X) ? 0u : ((X - xs >= E) ? ((1u << E) - 1u) : ((1u << (X - xs)) - 1u));
   }
   const int jl = lane % LPW;  // lane within its word group; writers take word q = jl
-  const uint32_t word = (uint32_t)(x0 >> 5) + (uint32_t)(2 * E * jl + lane / LPW);
+  const uint32_t word = (uint32_t)(x0 >> 5) + (uint32_t)(2 * E * jl + ll / LPW);
   const bool writer = pW != nullptr && jl < NQ && word < rowWS;
   const uint32_t sh = (uint32_t)(E * jl);
   MomAcc acc;
@@ -140,7 +150,9 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   static_assert(kSt % kPbmAhead == 0 && (!STAGE || kSt % 8 == 0), "staged rows must be whole steps");
   __shared__ uint32_t stage[STAGE ? kPbmWaves * kSt * 32 : 1];
   uint32_t* const myStage = stage + (STAGE ? (threadIdx.x >> 6) * kSt * 32 : 0);
-  const uint32_t widx = (word - (uint32_t)(x0 >> 5)) & 31u;  // the writer's word in the segment
+  // The writer's slot in its wave's staging row: its word in the segment, or
+  // with row groups its group of LPW lanes.
+  const uint32_t widx = GRP > 1 ? (uint32_t)(lane / LPW) : ((word - (uint32_t)(x0 >> 5)) & 31u);
   int yb = y;  // first staged row
   long long zb = z;
   // Partial segment (not the general form): load q of a row is read from
@@ -152,7 +164,7 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
   int shq[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    const int start = kWave * E * q + E * lane;
+    const int start = kWave * E * q + E * ll;
     offq[q] = (uint32_t)(min(start, X - x0 - E) + E);
     shq[q] = start + E - (int)offq[q];
   }
@@ -164,11 +176,14 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
     for (long long i = 0; i < n; i += kPbmAhead) {
       T v[kPbmAhead][V];
       size_t wo[kPbmAhead];
+      bool okr[kPbmAhead];  // the lane's row is in the field (row groups)
 #pragma unroll
       for (int u = 0; u < kPbmAhead; ++u) {
         if (i + u < n) {
+          okr[u] = GRP == 1 || z < rs.Z;
           // Several fields are summed in table order (general form only).
-          const long long ro = (long long)y * fa.sy + z * fa.sz;
+          const long long ro = okr[u] ? (long long)y * fa.sy + z * fa.sz
+                                      : (long long)(Y - 1) * fa.sy + (long long)(rs.Z - 1) * fa.sz;
           for (int fi = 0; fi < (G ? fa.nF : 1); ++fi) {
             const T* row = fa.p[fi] + ro;
             T t[V];
@@ -233,16 +248,17 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
           uint32_t wq[NQ];
 #pragma unroll
           for (int q = 0; q < NQ; ++q) {
+            const uint32_t kqu = okr[u] ? kq[q] : 0u;
             uint32_t nib = laneBit(relMask<M>(v[u][E * q + E - 1], thr)) ? 1u : 0u;
 #pragma unroll
             for (int e = E - 2; e >= 0; --e) nib = dblAddLaneBit(nib, relMask<M>(v[u][E * q + e], thr));
-            nib &= kq[q];
+            nib &= kqu;
             uint32_t pw = orHalfMirror(orQuadXor2(orQuadXor1(nib << sh)));
             if constexpr (LPW == 16) pw = orRowMirror(pw);
             wq[q] = pw;
             if constexpr (MOM) {
 #pragma unroll
-              for (int e = 0; e < E; ++e) acc.add(v[u][E * q + e], ((nib >> e) & 1u) != 0, ((kq[q] >> e) & 1u) != 0);
+              for (int e = 0; e < E; ++e) acc.add(v[u][E * q + e], ((nib >> e) & 1u) != 0, ((kqu >> e) & 1u) != 0);
             }
           }
           uint32_t w32 = wq[0];
@@ -250,7 +266,7 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
           for (int q = 1; q < NQ; ++q) w32 = (jl == q) ? wq[q] : w32;
           if constexpr (STAGE) {
             if (writer) myStage[(int)((i + u) % kSt) * 32 + widx] = w32;
-          } else if (writer) {
+          } else if (writer && okr[u]) {
             pW[wo[u]] = w32;
           }
         }
@@ -266,7 +282,7 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
               if (k0 + j < nb) {
-                if (writer) pW[(size_t)yb * rowWS + (size_t)zb * planeWS + word] = t[j];
+                if (writer && (GRP == 1 || zb < rs.Z)) pW[(size_t)yb * rowWS + (size_t)zb * planeWS + word] = t[j];
                 yb += rs.dy;
                 zb += rs.dz;
                 if (yb >= Y) {
@@ -1474,11 +1490,19 @@ static FieldView pbmView(const FieldView& fv, const uint32_t* pW, uint32_t& rowW
   return v;
 }
 
+// Row groups of the map kernel: float rows of at most 128, 64 or 32 voxels
+// (contiguous rows, one field) take 2, 4 or 8 rows per wave step.
+static int pbmGroups(const FieldView& fv) {
+  if ((fv.enc & ENC_MASK_NUM) != 4 || fv.nF > 1 || fv.sx != 1 || fv.X < 4 || fv.X > 2 * kWave) return 1;
+  return fv.X <= 32 ? 8 : (fv.X <= 64 ? 4 : 2);
+}
+
 static PbmGrid pbmGrid(const FieldView& fv) {
   PbmGrid g;
   g.nSegX = (int)ceilDiv(fv.X, kWave * pbmVoxels(fv));
+  const int grp = pbmGroups(fv);
   RowSched& rs = g.rs;
-  rs.nRows = (long long)fv.Y * fv.Z;
+  rs.nRows = ceilDiv((long long)fv.Y * fv.Z, grp);  // rows, or row groups
   const long long wantChunks = std::max<long long>(1, (MKF_PBM_WAVE_TARGET + g.nSegX - 1) / g.nSegX);
   rs.rowsPerWave = ceilDiv(rs.nRows, wantChunks);
   const long long nChunks = ceilDiv(rs.nRows, rs.rowsPerWave);
@@ -1490,8 +1514,9 @@ static PbmGrid pbmGrid(const FieldView& fv) {
     rs.chunkStep = rs.rowsPerWave;
     rs.rowStep = 1;
   }
-  rs.dy = (int)(rs.rowStep % fv.Y);
-  rs.dz = (int)(rs.rowStep / fv.Y);
+  rs.dy = (int)(rs.rowStep * grp % fv.Y);
+  rs.dz = (int)(rs.rowStep * grp / fv.Y);
+  rs.Z = fv.Z;
   g.nBlocks = ceilDiv(g.nSegX * nChunks, kPbmWaves);
   return g;
 }
@@ -1516,6 +1541,24 @@ static hipError_t launchPBMT(const FieldView& fv, T thr, uint32_t* pW, uint32_t 
                            g.nSegX, g.rs, pW, rowWS, planeWS, part);
         break;
       default:
+        if constexpr (sizeof(T) == 4) {
+          switch (pbmGroups(fv)) {
+            case 8:
+              hipLaunchKernelGGL((kPBM<T, M, false, MOM, 4, 8>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X,
+                                 fv.Y, g.nSegX, g.rs, pW, rowWS, planeWS, part);
+              return hipGetLastError();
+            case 4:
+              hipLaunchKernelGGL((kPBM<T, M, false, MOM, 4, 4>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X,
+                                 fv.Y, g.nSegX, g.rs, pW, rowWS, planeWS, part);
+              return hipGetLastError();
+            case 2:
+              hipLaunchKernelGGL((kPBM<T, M, false, MOM, 4, 2>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X,
+                                 fv.Y, g.nSegX, g.rs, pW, rowWS, planeWS, part);
+              return hipGetLastError();
+            default:
+              break;
+          }
+        }
         hipLaunchKernelGGL((kPBM<T, M, false, MOM, 4>), grid, block, 0, s, fieldArgs<T>(fv), thr, fv.X, fv.Y,
                            g.nSegX, g.rs, pW, rowWS, planeWS, part);
         break;

@@ -72,9 +72,10 @@ def M_oracle():
     return O
 
 
-# (9, 70, 96) and (3, 13, 2080): rows of whole map words, taken a plane at a time
+# (9, 70, 96) and (3, 13, 2080): rows of whole map words, taken a plane at a
+# time; (3, 5, 31), (20, 33, 50) and (7, 9, 100): row groups of 8, 4 and 2
 SHAPES = [(2, 2, 2), (3, 5, 31), (4, 4, 1025), (9, 17, 200), (40, 50, 333), (3, 2, 2049), (9, 70, 96),
-          (3, 13, 2080)]
+          (3, 13, 2080), (20, 33, 50), (7, 9, 100)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)

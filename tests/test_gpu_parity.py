@@ -385,11 +385,13 @@ def test_whole_word_rows(lib, orc, cuda, shape, dtype):
         check_all_paths(lib, orc, f, relop, thr)
 
 
-def test_whole_word_window_of_a_wider_field(lib, orc, cuda):
-    """A 64-wide window (row stride 130) keeps its own rows: its map equals
-    the oracle's map of the window."""
+@pytest.mark.parametrize("width", [64, 50, 100, 29])
+def test_narrow_window_of_a_wider_field(lib, orc, cuda, width):
+    """A narrow window (row stride 130) keeps its own rows: its map equals
+    the oracle's map of the window (64: whole words; 50, 100, 29: row groups
+    of the map pass with a row stride of their own)."""
     full = synth.grf_field((12, 40, 130), sigma=2.0, seed=81, device=cuda, dtype=torch.float32)
-    sub = full[2:10, 3:35, 5:69]
+    sub = full[2:10, 3:35, 5:5 + width]
     Z, Y, X = sub.shape
     fi = M.FieldInfo(sub.data_ptr(), (X, Y, Z), M.ENC_F32, strides=(1, 130, 130 * 40))
     w_ref, h_ref = oracle_all(orc, np.ascontiguousarray(host_np(sub)), ">", 0.0)
