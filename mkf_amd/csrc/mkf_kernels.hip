@@ -180,10 +180,12 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
 #pragma unroll
       for (int u = 0; u < kPbmAhead; ++u) {
         if (i + u < n) {
-          okr[u] = GRP == 1 || z < rs.Z;
           // Several fields are summed in table order (general form only).
-          const long long ro = okr[u] ? (long long)y * fa.sy + z * fa.sz
-                                      : (long long)(Y - 1) * fa.sy + (long long)(rs.Z - 1) * fa.sz;
+          long long ro = (long long)y * fa.sy + z * fa.sz;
+          if constexpr (GRP > 1) {
+            okr[u] = z < rs.Z;
+            if (!okr[u]) ro = (long long)(Y - 1) * fa.sy + (long long)(rs.Z - 1) * fa.sz;
+          }
           for (int fi = 0; fi < (G ? fa.nF : 1); ++fi) {
             const T* row = fa.p[fi] + ro;
             T t[V];
@@ -248,7 +250,7 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
           uint32_t wq[NQ];
 #pragma unroll
           for (int q = 0; q < NQ; ++q) {
-            const uint32_t kqu = okr[u] ? kq[q] : 0u;
+            const uint32_t kqu = (GRP == 1 || okr[u]) ? kq[q] : 0u;
             uint32_t nib = laneBit(relMask<M>(v[u][E * q + E - 1], thr)) ? 1u : 0u;
 #pragma unroll
             for (int e = E - 2; e >= 0; --e) nib = dblAddLaneBit(nib, relMask<M>(v[u][E * q + e], thr));
@@ -266,7 +268,7 @@ __global__ __launch_bounds__(kPbmWaves* kWave) void kPBM(const FieldArgs<T> fa, 
           for (int q = 1; q < NQ; ++q) w32 = (jl == q) ? wq[q] : w32;
           if constexpr (STAGE) {
             if (writer) myStage[(int)((i + u) % kSt) * 32 + widx] = w32;
-          } else if (writer && okr[u]) {
+          } else if (writer && (GRP == 1 || okr[u])) {
             pW[wo[u]] = w32;
           }
         }
