@@ -601,6 +601,8 @@ def main():
     ap.add_argument("--pmc-workload", action="store_true",
                     help="for counter runs: 3 fused launches, then 3 copies of a 1 GiB buffer (the calibration "
                          "kernel for FETCH_SIZE / WRITE_SIZE); with --mode batch, 3 batched launches; prints no JSON")
+    ap.add_argument("--dims", type=str, default="",
+                    help="X,Y,Z of the --pmc-workload field (default: --size cubed)")
     ap.add_argument("--pmc-target", choices=("fused", "bpfd", "pbm"), default="fused",
                     help="with --pmc-workload: 'bpfd' makes the PBM once and runs 3 histograms of it instead "
                          "(with --tiling's tiling, else the library's default); 'pbm' runs 3 map passes (then the copies)")
@@ -637,9 +639,12 @@ def main():
         print(json.dumps(out), flush=True)
         return
     S = args.size
-    log(f"[bench] generating {S}^3 GRF sigma={args.sigma} seed={args.seed} ({args.dtype})")
+    if args.dims and not args.pmc_workload:
+        raise SystemExit("--dims applies to --pmc-workload only")
+    FD = tuple(int(v) for v in args.dims.split(",")) if args.dims else (S, S, S)  # X, Y, Z
+    log(f"[bench] generating {FD[0]}x{FD[1]}x{FD[2]} GRF sigma={args.sigma} seed={args.seed} ({args.dtype})")
     t0 = time.perf_counter()
-    field64 = synth.grf_field((S, S, S), args.sigma, args.seed, device=dev, dtype=torch.float64)
+    field64 = synth.grf_field(FD[::-1], args.sigma, args.seed, device=dev, dtype=torch.float64)
     field = field64.to(torch.float32) if args.dtype == "f32" else field64
     # The float64 leg runs on the same field before its rounding (default run only).
     f64_leg = args.dtype == "f32" and not args.no_f64 and not args.pmc_workload
@@ -652,7 +657,7 @@ def main():
 
     enc = M.ENC_F32 if args.dtype == "f32" else M.ENC_F64
     eb = 4 if args.dtype == "f32" else 8
-    fi = M.FieldInfo(field.data_ptr(), (S, S, S), enc)
+    fi = M.FieldInfo(field.data_ptr(), FD, enc)
     bm = M.bin_map(">=", args.threshold)
     stream = torch.cuda.current_stream()
     sptr = C.c_void_p(stream.cuda_stream)
@@ -667,7 +672,7 @@ def main():
     if args.pmc_workload and args.pmc_target == "bpfd":
         # The PBM of the field once, then 3 histogram launches of the PBM.
         org = M.BMOrg()
-        nwords = lib.setBMO(C.byref(org), M.dims((S, S, S)), b"\x00")
+        nwords = lib.setBMO(C.byref(org), M.dims(FD), b"\x00")
         pbm = torch.empty(nwords, dtype=torch.int32, device=dev)
         M.check(lib.mkfPBMAsync(pbm.data_ptr(), C.byref(org), fi.ref, C.byref(bm), sptr), "mkfPBMAsync")
         del field
@@ -683,7 +688,7 @@ def main():
         if args.pmc_target == "pbm":
             # 3 map passes of the field into a caller buffer.
             org = M.BMOrg()
-            nwords = lib.setBMO(C.byref(org), M.dims((S, S, S)), b"\x00")
+            nwords = lib.setBMO(C.byref(org), M.dims(FD), b"\x00")
             pbm = torch.empty(nwords, dtype=torch.int32, device=dev)
             for _ in range(3):
                 M.check(lib.mkfPBMAsync(pbm.data_ptr(), C.byref(org), fi.ref, C.byref(bm), sptr), "mkfPBMAsync")
